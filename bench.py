@@ -1,0 +1,172 @@
+"""Benchmark: particle-steps/s of the vortex-arrays RK4 workload at N = 1,048,576
+on one MI355X, FP32-compensated form (the headline value) next to the plain FP64
+form, with the pair kernel's roofline position and the CPU oracle baseline.
+
+Workload: vortex arrays, deterministic RK4 (D = 0), dt = 0.01, N = 1,048,576 in a
+periodic box at density 4000 (L = sqrt(N / 4000) = 16.19086), rho = 0.05,
+xi_a = 0.01, xi_r = 0.005, v = 0.025, mu+ = mu- = 100, kappa = 20; uniform
+random positions and phases (Philox, seed 42).  Linked cells rebuilt at every
+RHS evaluation, as the reference does.  Inputs are generated on the device and
+stay resident; a step is one full RK4 step (4 RHS evaluations + combine + wrap).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp32k": 157.3, "fp64": 78.6, "fp64f": 78.6}  # TFLOP/s, MI355X vector FP32 / FP64
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
+FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
+FLOP_VORTEX_PAIR = 16 + 2  # sqrt, W(r), sin(phi_j - phi_i), compensated add
+FLOP_COHESION = 12 + 2
+FLOP_EPILOGUE = 4 + 2
+
+
+def workload(n=1048576):
+    L = math.sqrt(n / 4000.0)
+    return dict(n=n, L=L, rho=0.05, velocity=0.025, mu_plus=100.0, mu_minus=100.0, xi_a=0.01, xi_r=0.005,
+                kappa=20.0, D=0.0, dt=0.01, seed=42)
+
+
+def make_sim(spc2, w, precision):
+    p = spc2.params("vortex", precision, n=w["n"], L=w["L"], rho=w["rho"], velocity=w["velocity"],
+                    mu_plus=w["mu_plus"], mu_minus=w["mu_minus"], xi_a=w["xi_a"], xi_r=w["xi_r"],
+                    kappa=w["kappa"], D=w["D"], seed=w["seed"])
+    s = spc2.Sim(p)
+    s.init_random_state(0.0)
+    return s
+
+
+def time_precision(spc2, w, precision, steps, warmup):
+    s = make_sim(spc2, w, precision)
+    s.step("rk4", w["dt"], 0, warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step("rk4", w["dt"], warmup, steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    R, _ = s.order_parameter()
+    s.close()
+    sec = t1 - t0
+    return {"particle_steps_per_s": w["n"] * steps / sec, "ms_per_step": 1e3 * sec / steps, "R_final": R}
+
+
+def profile_pair_kernel(spc2, w, precision, steps):
+    """HIP-event timing of the pair kernels on the handle's stream, with pair counts."""
+    s = make_sim(spc2, w, precision)
+    s.step("rk4", w["dt"], 0, 2)
+    s.set_counting(True)
+    s.set_profiling(True)
+    s.step("rk4", w["dt"], 2, steps)
+    ms, launches = s.profile()
+    cnt = s.counts()
+    s.close()
+    per = cnt / launches
+    flops = (FLOP_CANDIDATE * per[0] + FLOP_VORTEX_PAIR * per[1] + FLOP_COHESION * per[2] + FLOP_EPILOGUE * w["n"])
+    avg_s = ms / launches / 1e3
+    # minimal HBM bytes per launch: FP32 tile read (16 B) + slot->particle and
+    # cell (8 B) + FP64 phase (8 B) + k written (24 B); the last stage also reads
+    # k1..k3 (72 B) and reads and writes the state (48 B)
+    bytes_per = w["n"] * (56 + (72 + 48) / 4.0) if precision == "fp32k" else w["n"] * (72 + 24 + 8 + (120 / 4.0))
+    return {"avg_ms": ms / launches, "launches": launches, "flops_per_launch": flops,
+            "tflops": flops / avg_s / 1e12, "bytes_per_launch": bytes_per, "gbs": bytes_per / avg_s / 1e9,
+            "candidates_per_particle": per[0] / w["n"], "neighbours_per_particle": per[1] / w["n"],
+            "cohesion_per_particle": per[2] / w["n"]}
+
+
+def cpu_baseline(w, steps=2):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import numpy as np
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = w["n"]
+    st, _ = O.init_state(w["seed"], n, w["L"], w["L"], 0.0)
+    sysp = O.vortex(n, w["L"], rho=w["rho"], velocity=w["velocity"], mu_plus=w["mu_plus"], mu_minus=w["mu_minus"],
+                    xi_a=w["xi_a"], xi_r=w["xi_r"], kappa=w["kappa"], D=w["D"], threads=threads)
+    kbuf = np.zeros(12 * n)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = O.step_rk4(sysp, st, w["dt"], kbuf)
+    sec = time.perf_counter() - t0
+    return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK4 steps of the same N={n} vortex workload, FP64 reference method "
+                      f"(oracle/spc2_oracle.c, -O2), {threads} threads, contiguous particle slices",
+            "seconds": sec}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n", type=int, default=1048576)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every GPU box here has one MI355X and this build has no multi-GPU path "
+              "(the spatial-slab sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import spc2_langevin_integration_amd as spc2
+    w = workload(args.n)
+    res = {}
+    for prec in ("fp32k", "fp64", "fp64f"):
+        res[prec] = time_precision(spc2, w, prec, args.steps, args.warmup)
+        print(f"# {prec}: {res[prec]['particle_steps_per_s']:.4e} particle-steps/s, "
+              f"{res[prec]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    prof = profile_pair_kernel(spc2, w, "fp32k", args.profile_steps)
+    prof64 = profile_pair_kernel(spc2, w, "fp64", max(4, args.profile_steps // 4))
+    print(f"# pair kernel fp32k: {prof['avg_ms']:.4f} ms/launch, {prof['tflops']:.2f} TFLOP/s; "
+          f"fp64: {prof64['avg_ms']:.4f} ms/launch", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(w)
+
+    v = res["fp32k"]
+    line = {
+        "metric": "particle-steps/s at N=1M (FP32-comp vs FP64, same GPU)",
+        "value": v["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": v["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair sums (Kahan) + f64 state",
+        "data": "synthetic (Philox uniform positions and phases, seed 42)",
+        "config": {"workload": "vortex-arrays RK4, N=1048576, density 4000 (L=16.19086), rho=0.05, xi_a=0.01, "
+                               "xi_r=0.005, v=0.025, mu+-=100, kappa=20, D=0, dt=0.01, linked cells per RHS",
+                   "n": w["n"], "method": "rk4", "neighbour": "cells"},
+        "fp64_value": res["fp64"]["particle_steps_per_s"],
+        "fp64_factored_value": res["fp64f"]["particle_steps_per_s"],
+        "fp32_over_fp64": v["particle_steps_per_s"] / res["fp64"]["particle_steps_per_s"],
+        "order_param_abs_err_fp32_vs_fp64": abs(v["R_final"] - res["fp64"]["R_final"]),
+        "roofline": {
+            "bound": "valu", "kernel": "k_pass1_cells<VortAcc<P32K>> (vortex pair sums, FP32 Kahan)",
+            "achieved": prof["tflops"], "peak": PEAK["fp32k"], "unit": "TFLOP/s",
+            "frac": prof["tflops"] / PEAK["fp32k"], "traffic": None,
+            "avg_launch_ms": prof["avg_ms"], "flops_per_launch": prof["flops_per_launch"],
+            "hbm_achieved_GBs": prof["gbs"], "hbm_frac": prof["gbs"] / HBM_PEAK_GBS,
+            "candidates_per_particle": prof["candidates_per_particle"],
+            "neighbours_per_particle": prof["neighbours_per_particle"],
+            "fp64_pair_kernel_avg_ms": prof64["avg_ms"], "fp64_pair_kernel_tflops": prof64["tflops"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/*
+ * spc2_hip.h -- C ABI of the MI355X (gfx950) force-summation and stepping
+ * library for the chimera and vortex-arrays self-propelled particle models.
+ *
+ * Plain C: pointers, sizes and status codes, no C++ or torch types.  One HIP
+ * stream per handle; a handle is used by one host thread at a time, distinct
+ * handles are independent.  Host buffers belong to the caller, device memory
+ * to the handle.  State arrays use the reference's layout: AoS [x, y, phi] per
+ * particle in double (Definitions.hpp:17-21, kS = 3).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the reference tree).
+ */
+#ifndef SPC2_HIP_H
+#define SPC2_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum spc2_status {
+  SPC2_OK = 0,
+  SPC2_ERR_INVALID = 1,     /* bad argument or parameter combination */
+  SPC2_ERR_HIP = 2,         /* a HIP runtime call failed */
+  SPC2_ERR_NOMEM = 3,       /* device allocation failed */
+  SPC2_ERR_UNSUPPORTED = 4, /* valid request this build does not provide */
+} spc2_status;
+
+typedef enum spc2_system { SPC2_CHIMERA = 0, SPC2_VORTEX = 1 } spc2_system;
+
+/* Arithmetic of the pair sums.
+ *  SPC2_FP64: the reference method in FP64 (per-pair sin/cos/atan2, reference
+ *             operation order, reference min-image), the plain GPU form.
+ *  SPC2_FP32_KAHAN: pair sums in FP32 with Kahan compensation over cell-local
+ *             FP32 coordinates; the phase enters through cos/sin of each
+ *             particle's phase (angle-addition identity), so the pair loop has
+ *             no transcendentals.  State, wrap and reductions stay FP64.
+ *  SPC2_FP64_FACTORED: the SPC2_FP32_KAHAN algorithm carried out in FP64. */
+typedef enum spc2_precision { SPC2_FP64 = 0, SPC2_FP32_KAHAN = 1, SPC2_FP64_FACTORED = 2 } spc2_precision;
+
+typedef enum spc2_method { SPC2_SE = 0, SPC2_SRK = 1, SPC2_RK4 = 2 } spc2_method;
+
+/* Neighbour search.  AUTO follows the reference dispatch: linked cells iff
+ * rho <= L/4, all pairs otherwise (ChimeraSystem.cpp:90-98,
+ * VortexArraysSystemPtr.cpp:113-119). */
+typedef enum spc2_neighbour { SPC2_NEIGH_AUTO = 0, SPC2_NEIGH_CELLS = 1, SPC2_NEIGH_ALL_PAIRS = 2 } spc2_neighbour;
+
+/* Model and run parameters.  The reference hard-codes these in the engine
+ * (SimulationEnginePtr.cpp:82-91, :131-139) and in Definitions.hpp:20-22;
+ * here they are run time. */
+typedef struct spc2_params {
+  int32_t system;    /* spc2_system */
+  int32_t precision; /* spc2_precision */
+  int32_t neighbour; /* spc2_neighbour */
+  int32_t reserved;
+  int64_t n;         /* particles (kN) */
+  double Lx, Ly;     /* periodic box (kL) */
+  double rho;        /* interaction radius */
+  /* chimera (ChimeraSystemPtr.hpp:16-23); v0 is also the vortex velocity */
+  double v0, sigma, alpha, D;
+  /* vortex arrays (VortexArraysSystemPtr.hpp:16-25) */
+  double mu_plus, mu_minus, xi_a, xi_r, kappa;
+  uint64_t seed;     /* Philox key for initial state, natural frequencies and noise */
+} spc2_params;
+
+typedef struct spc2_sim *spc2_sim_t;
+
+/* Fills *p with the chimera defaults of the survey configs (v0 = 0.01,
+ * sigma = 1, rho = 0.1, alpha = 1.45, D = 0.01, L = 1, N = 1000, seed 42). */
+void spc2_default_params(spc2_params *p);
+
+/* Replaces the system constructors ChimeraSystemPtr::ChimeraSystemPtr
+ * (ChimeraSystemPtr.cpp:11-93) and VortexArraysSystemPtr::VortexArraysSystemPtr
+ * (VortexArraysSystemPtr.cpp:9-88) together with the engine's state allocation
+ * (SimulationEnginePtr.cpp:18-24). */
+spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out);
+spc2_status spc2_destroy(spc2_sim_t h);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *spc2_last_error(void);
+
+/* Host <-> device copies of the AoS state (3N doubles); synchronous.
+ * Replace the engine's ownership of system_state_ (SimulationEnginePtr.cpp:23)
+ * and BroadcastVector (ThreadSharedMemory.cpp:331-347). */
+spc2_status spc2_set_state(spc2_sim_t h, const double *state);
+spc2_status spc2_get_state(spc2_sim_t h, double *state);
+
+/* Philox-keyed random initial state: x, y ~ U[0, L), phi ~ U[0, 2 pi), and
+ * natural frequencies omega ~ N(0, omega_std).  Replaces
+ * SimulationEnginePtr::InitializeRandomSystemState (SimulationEnginePtr.cpp:31-50)
+ * and the frequency draw of StochasticRungeKuttaStepperPtr
+ * (StochasticRungeKuttaStepperPtr.hpp:25-35). */
+spc2_status spc2_init_random_state(spc2_sim_t h, double omega_std);
+
+/* Natural frequencies omega_i (N doubles), added to the phase drift as
+ * (omega + a_phi) dt (StochasticRungeKuttaStepperPtr.hpp:65). */
+spc2_status spc2_set_omega(spc2_sim_t h, const double *omega);
+
+/* k-form right-hand side at the stage s + k_coef * dt * k_prev; k_next (3N,
+ * in/out) keeps its phi entry where a particle has no partner.  Replaces
+ * ChimeraSystemPtr::EvaluateRhs(state, k_prev, k_next, k_coef, dt)
+ * (ChimeraSystemPtr.cpp:116-129) and the VortexArraysSystemPtr one
+ * (VortexArraysSystemPtr.cpp:107-120).  k_prev may equal k_next. */
+spc2_status spc2_eval_rhs(spc2_sim_t h, const double *k_prev, double k_coef, double dt, double *k_next);
+
+/* Ito-form right-hand side: derivative (3N, in/out) and additional (2N, in/out,
+ * [i][0], [i][1] per particle).  Replaces ChimeraSystemPtr::EvaluateRhs(state,
+ * derivative, additional_derivative, dt) (ChimeraSystemPtr.cpp:131-144).  The
+ * vortex-arrays Ito form returns SPC2_ERR_UNSUPPORTED in this build. */
+spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *additional);
+
+/* Advances the device-resident state nsteps steps of `method` with time step
+ * dt, the engine's periodic wrap fused in (SimulationEnginePtr.cpp:106-115).
+ * Step k uses noise of global step step0 + k.  noise == NULL draws Philox
+ * normals keyed (seed, step, particle); otherwise noise holds nsteps blocks of
+ * N (SE) or 2N (SRK, (U1, U2) per particle) normals.  Replaces
+ * StochasticEulerStepperPtr::DoStep (StochasticEulerStepperPtr.hpp:32-76),
+ * StochasticRungeKuttaStepperPtr::DoStep (StochasticRungeKuttaStepperPtr.hpp:41-97),
+ * RungeKutta4StepperPtr::DoStep (RungeKutta4StepperPtr.cpp:25-65) and
+ * PeriodicBoundaryConditions::ApplyPeriodicBoundaryConditions
+ * (PeriodicBoundaryConditions.cpp:133-141). */
+spc2_status spc2_step(spc2_sim_t h, int32_t method, double dt, int64_t step0, int64_t nsteps, const double *noise);
+
+/* The Philox normals spc2_step would draw at global step `step`: N (SE) or 2N
+ * (SRK) doubles.  Replaces the stepper's std::normal_distribution draws
+ * (StochasticRungeKuttaStepperPtr.hpp:49-56). */
+spc2_status spc2_noise(spc2_sim_t h, int32_t method, int64_t step, double *out);
+
+/* Kuramoto order parameter R e^{i psi} = (1/N) sum e^{i phi_j}, FP64.
+ * Replaces the reduction in BinaryObserverPtr::SaveSummaryStatistics
+ * (BinaryObserverPtr.cpp:237-247), which sums in float. */
+spc2_status spc2_order_parameter(spc2_sim_t h, double *R, double *psi);
+
+/* The linked-cell list of the current state in CSR form: cell_of (N),
+ * cell_start (nx*ny+1) and sorted (N, ascending particle index within each
+ * cell).  Replaces LinkedList::ConstructLinkedList + GetLinkedList
+ * (LinkedList.cpp:92-134).  Fails with SPC2_ERR_INVALID on the all-pairs path. */
+spc2_status spc2_get_cell_list(spc2_sim_t h, int32_t *cell_of, int32_t *cell_start, int32_t *sorted);
+spc2_status spc2_num_cells(spc2_sim_t h, int32_t *nx, int32_t *ny);
+
+/* Pair counters of the RHS evaluations since the last reset: candidates tested,
+ * pairs within rho, cohesion pairs.  Counting is off unless enabled. */
+spc2_status spc2_set_counting(spc2_sim_t h, int32_t enable);
+spc2_status spc2_get_counts(spc2_sim_t h, int64_t *counts3);
+
+/* HIP-event timing of the pair kernels on the handle's stream. */
+spc2_status spc2_set_profiling(spc2_sim_t h, int32_t enable);
+spc2_status spc2_get_profile(spc2_sim_t h, double *pair_kernel_ms, int64_t *launches);
+
+spc2_status spc2_synchronize(spc2_sim_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,911 @@
+/*
+ * spc2 oracle -- CPU restatement of the reference's force-summation and
+ * stepping path.  TEST INFRASTRUCTURE ONLY: the checker for tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The product
+ * (spc2_langevin_integration_amd) never links, loads or calls this file.
+ *
+ * Built with -O2 -ffp-contract=off: every expression below keeps the
+ * reference's evaluation order (left to right, no fused multiply-add), so on
+ * the same libm the results are bit-identical to the reference build in
+ * oracle/_ref.  Citations are into the reference tree.
+ *
+ * Pinning (tests/test_oracle.py):
+ *  - cell list, min-image, wraps, chimera RHS (both forms, linked-list and
+ *    all-pairs), SE / SRK / RK4 steps: bit-exact against golden vectors written
+ *    by the reference's own classes (oracle/ref_harness.cpp);
+ *  - mt19937 + std::normal_distribution replay: bit-exact against the noise
+ *    the reference consumed;
+ *  - vortex RHS (the reference has it only in MPI translation units, which
+ *    need MPICH and are not built): brute force on small N, W(r) closed forms
+ *    and continuity, the cohesion atan2 identity, phi-shift invariance, and
+ *    the two-body analytic values;
+ *  - Philox4x32-10: published known-answer vectors;
+ *  - order parameter: closed forms (all phases equal, equispaced phases).
+ */
+#define _GNU_SOURCE
+#include "spc2_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define KS 3 /* state stride (x, y, phi), Definitions.hpp:21 */
+
+/* ------------------------------------------------------------------------ */
+/* inputs                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static uint64_t splitmix_next(uint64_t *s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+static double splitmix_u01(uint64_t *s) { return (double)(splitmix_next(s) >> 11) * 0x1.0p-53; }
+
+void orc_splitmix_uniform(uint64_t seed, int64_t n, double lo, double hi, double *out) {
+  uint64_t s = seed;
+  for (int64_t i = 0; i < n; ++i) out[i] = lo + (hi - lo) * splitmix_u01(&s);
+}
+
+/* Same generator and modes as the golden-vector harness (MakeState). */
+void orc_make_state(uint64_t seed, int32_t mode, int64_t n, double L, double rho, double *st) {
+  uint64_t s = seed;
+  const double two_pi = 2.0 * M_PI;
+  for (int64_t i = 0; i < n; ++i) {
+    double ux = splitmix_u01(&s), uy = splitmix_u01(&s), up = splitmix_u01(&s);
+    if (mode == 0) {
+      st[KS * i] = ux * L;
+      st[KS * i + 1] = uy * L;
+    } else if (mode == 1) {
+      st[KS * i] = (ux * 1.5 - 0.25) * L;
+      st[KS * i + 1] = (uy * 1.5 - 0.25) * L;
+    } else {
+      int nc = (int)(L / rho);
+      double h = L / nc;
+      st[KS * i] = (double)(int)(ux * 4 * nc) * 0.25 * h;
+      st[KS * i + 1] = (double)(int)(uy * 4 * nc) * 0.25 * h;
+    }
+    st[KS * i + 2] = up * two_pi;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometry: PeriodicBoundaryConditions.cpp                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct geom {
+  double Lx, Ly, rLx, rLy; /* x_size_, y_size_, x_rsize_ = 1.0 / x_size (:13-14) */
+} geom;
+
+static geom make_geom(double Lx, double Ly) {
+  geom g;
+  g.Lx = Lx;
+  g.Ly = Ly;
+  g.rLx = 1.0 / Lx;
+  g.rLy = 1.0 / Ly;
+  return g;
+}
+
+/* ClassAEffectiveParticleDistance, PeriodicBoundaryConditions.cpp:22-34 */
+static inline void class_a(const geom *g, double xi, double yi, double xj, double yj, double *dx, double *dy) {
+  double ddx = xj - xi;
+  ddx -= (double)(int)(ddx * 2.0 * g->rLx) * g->Lx;
+  double ddy = yj - yi;
+  ddy -= (double)(int)(ddy * 2.0 * g->rLy) * g->Ly;
+  *dx = ddx;
+  *dy = ddy;
+}
+
+/* scalar ApplyPeriodicBoundaryConditions, PeriodicBoundaryConditions.cpp:117-121 */
+static inline void wrap_xy(const geom *g, double *x, double *y) {
+  *x = *x - floor(*x * g->rLx) * g->Lx;
+  *y = *y - floor(*y * g->rLy) * g->Ly;
+}
+
+/* the conditional wrap before binning / reading, ChimeraSystem.cpp:299-302, :402-405 */
+static inline void cond_wrap(const geom *g, double *x, double *y) {
+  if (*x < 0.0 || *x >= g->Lx || *y < 0.0 || *y >= g->Ly) wrap_xy(g, x, y);
+}
+
+void orc_min_image(double L, const double *p, int64_t m, double *dxy) {
+  geom g = make_geom(L, L);
+  for (int64_t k = 0; k < m; ++k) class_a(&g, p[4 * k], p[4 * k + 1], p[4 * k + 2], p[4 * k + 3], &dxy[2 * k], &dxy[2 * k + 1]);
+}
+
+void orc_wrap_points(double L, const double *in, int64_t m, double *out) {
+  geom g = make_geom(L, L);
+  for (int64_t k = 0; k < m; ++k) {
+    double x = in[2 * k], y = in[2 * k + 1];
+    wrap_xy(&g, &x, &y);
+    out[2 * k] = x;
+    out[2 * k + 1] = y;
+  }
+}
+
+/* array ApplyPeriodicBoundaryConditions, PeriodicBoundaryConditions.cpp:133-141
+ * (x and y only; phi is never wrapped) */
+void orc_wrap_state(const orc_sys *s, double *st) {
+  geom g = make_geom(s->Lx, s->Ly);
+  for (int64_t i = 0; i < s->n; ++i) {
+    st[KS * i] -= floor(st[KS * i] * g.rLx) * g.Lx;
+    st[KS * i + 1] -= floor(st[KS * i + 1] * g.rLy) * g.Ly;
+  }
+}
+
+/* num_subcells_ = int(L / rho), ChimeraSystem.cpp:32-33, VortexArraysSystemPtr.cpp:35-36 */
+int32_t orc_num_cells(double L, double rho) { return (int32_t)(L / rho); }
+
+/* ------------------------------------------------------------------------ */
+/* threads: contiguous particle slices, Thread.cpp:18-21 decomposition       */
+/* ------------------------------------------------------------------------ */
+
+typedef void (*range_fn)(void *ctx, int64_t lo, int64_t hi);
+typedef struct slice_job {
+  range_fn fn;
+  void *ctx;
+  int64_t lo, hi;
+} slice_job;
+
+static void *slice_main(void *p) {
+  slice_job *j = (slice_job *)p;
+  j->fn(j->ctx, j->lo, j->hi);
+  return NULL;
+}
+
+static void parallel_for(int32_t threads, int64_t n, range_fn fn, void *ctx) {
+  if (threads <= 1 || n < 2 * threads) {
+    fn(ctx, 0, n);
+    return;
+  }
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+  slice_job *jobs = (slice_job *)malloc(sizeof(slice_job) * (size_t)threads);
+  for (int32_t t = 0; t < threads; ++t) {
+    jobs[t].fn = fn;
+    jobs[t].ctx = ctx;
+    jobs[t].lo = n * t / threads;
+    jobs[t].hi = n * (t + 1) / threads;
+    pthread_create(&tid[t], NULL, slice_main, &jobs[t]);
+  }
+  for (int32_t t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+  free(tid);
+  free(jobs);
+}
+
+/* ------------------------------------------------------------------------ */
+/* cell list                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Binning after the conditional wrap, cell = int(y / L * n) * n + int(x / L * n)
+ * (ChimeraSystem.cpp:291-309, LinkedList.cpp:106-120), then members pushed in
+ * ascending particle index (ChimeraSystem.cpp:335-338, LinkedList.cpp:146-149);
+ * the result is the CSR form of linked_list_.  An index that rounds up to n
+ * (x just below L) is out of range in the reference; it is clamped to n - 1
+ * here and counted in *clamped. */
+int orc_cell_list(const orc_sys *s, int32_t ncx, int32_t ncy, const double *pos, int32_t *cell_of, int32_t *cell_start,
+                  int32_t *sorted, int64_t *clamped) {
+  geom g = make_geom(s->Lx, s->Ly);
+  int64_t nc = (int64_t)ncx * ncy, n = s->n, clamp = 0;
+  for (int64_t c = 0; c <= nc; ++c) cell_start[c] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    double x = pos[KS * i], y = pos[KS * i + 1];
+    cond_wrap(&g, &x, &y);
+    int cx = (int)(x / g.Lx * ncx);
+    int cy = (int)(y / g.Ly * ncy);
+    if (cx >= ncx) { cx = ncx - 1; ++clamp; }
+    if (cy >= ncy) { cy = ncy - 1; ++clamp; }
+    cell_of[i] = cy * ncx + cx;
+    cell_start[cell_of[i] + 1] += 1;
+  }
+  for (int64_t c = 0; c < nc; ++c) cell_start[c + 1] += cell_start[c];
+  int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)nc);
+  memcpy(fill, cell_start, sizeof(int32_t) * (size_t)nc);
+  for (int64_t i = 0; i < n; ++i) sorted[fill[cell_of[i]]++] = (int32_t)i;
+  free(fill);
+  if (clamped) *clamped = clamp;
+  return 0;
+}
+
+/* stencil order of neighboring_cells_, ChimeraSystem.cpp:39-44 */
+static const int kStencil[9][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {0, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
+
+/* AdjustNeighboringCellToPeriodicBoundaries, ChimeraSystem.cpp:905-922 */
+static inline int adjust_cell(int c, int n) {
+  if (c < 0) return n - 1;
+  if (c >= n) return 0;
+  return c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RHS evaluation                                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rhs_ctx {
+  const orc_sys *s;
+  geom g;
+  int32_t ncx, ncy;
+  const double *pos;   /* AoS state the pairs are read from (stage or state) */
+  const int32_t *cell_start, *sorted;
+  int use_ll;          /* 1: linked list with conditional wraps; 0: all pairs, no wraps */
+  /* all-pairs k-form builds the stage on the fly (ChimeraSystem.cpp:135-148) */
+  const double *base, *k_prev;
+  double k_coef, dt;
+  /* pass-1 outputs */
+  double *out;         /* k_next or derivative (AoS, stride 3) */
+  double *sumA, *card; /* chimera: alignment_force_, neighborhood_cardinality_ */
+  double *card3;       /* vortex: neighborhood_cardinality_[i][0..2] */
+  /* pass 2 */
+  double *additional;
+  int64_t *cnt;        /* per-thread counts are summed under no lock: per-slice arrays */
+} rhs_ctx;
+
+/* Reads particle j's coordinates the way each path does. */
+static inline void read_pos(const rhs_ctx *c, int64_t j, double *x, double *y, double *phi) {
+  if (c->base) {
+    /* all-pairs k-form: stage = s + k_coef * dt * k_prev, no wrap */
+    *x = c->base[KS * j] + c->k_coef * c->dt * c->k_prev[KS * j];
+    *y = c->base[KS * j + 1] + c->k_coef * c->dt * c->k_prev[KS * j + 1];
+    *phi = c->base[KS * j + 2] + c->k_coef * c->dt * c->k_prev[KS * j + 2];
+    return;
+  }
+  *x = c->pos[KS * j];
+  *y = c->pos[KS * j + 1];
+  *phi = c->pos[KS * j + 2];
+  if (c->use_ll) cond_wrap(&c->g, x, y);
+}
+
+/* Visits i's candidate partners j != i in the reference's order: stencil cells
+ * in kStencil order, members in ascending index (linked list), or j = 0..N-1
+ * (all pairs).  Returns the number of candidates. */
+#define FOR_EACH_PARTNER(c, i, xi, yi, ...)                                             \
+  do {                                                                                   \
+    if ((c)->use_ll) {                                                                   \
+      double bx_ = (xi), by_ = (yi);                                                     \
+      int cx_ = (int)(bx_ / (c)->g.Lx * (c)->ncx);                                       \
+      int cy_ = (int)(by_ / (c)->g.Ly * (c)->ncy);                                       \
+      if (cx_ >= (c)->ncx) cx_ = (c)->ncx - 1;                                           \
+      if (cy_ >= (c)->ncy) cy_ = (c)->ncy - 1;                                           \
+      for (int st_ = 0; st_ < 9; ++st_) {                                                \
+        int jx_ = adjust_cell(cx_ + kStencil[st_][0], (c)->ncx);                         \
+        int jy_ = adjust_cell(cy_ + kStencil[st_][1], (c)->ncy);                         \
+        int jc_ = jy_ * (c)->ncx + jx_;                                                  \
+        for (int32_t m_ = (c)->cell_start[jc_]; m_ < (c)->cell_start[jc_ + 1]; ++m_) {   \
+          int64_t j = (c)->sorted[m_];                                                   \
+          if (j != (i)) { __VA_ARGS__ }                                                         \
+        }                                                                                \
+      }                                                                                  \
+    } else {                                                                             \
+      for (int64_t j = 0; j < (c)->s->n; ++j) {                                          \
+        if (j != (i)) { __VA_ARGS__ }                                                           \
+      }                                                                                  \
+    }                                                                                    \
+  } while (0)
+
+/* chimera pass 1: ChimeraSystem.cpp:365-425 (k-form), :526-586 (Ito form),
+ * all pairs :131-167, :184-220 */
+static void chimera_pass1(void *vp, int64_t lo, int64_t hi) {
+  rhs_ctx *c = (rhs_ctx *)vp;
+  const orc_sys *s = c->s;
+  const double rho2 = s->rho * s->rho; /* rho_squared_, ChimeraSystem.cpp:24 */
+  int64_t cand = 0, hits = 0;
+  for (int64_t i = lo; i < hi; ++i) {
+    double xi, yi, phii;
+    read_pos(c, i, &xi, &yi, &phii);
+    double A = 0.0, n = 0.0;
+    FOR_EACH_PARTNER(c, i, xi, yi, {
+      double xj, yj, phij, dx, dy;
+      read_pos(c, j, &xj, &yj, &phij);
+      class_a(&c->g, xi, yi, xj, yj, &dx, &dy);
+      double dr2 = dx * dx + dy * dy;
+      ++cand;
+      if (dr2 <= rho2) {
+        A += sin(phij - phii - s->alpha);
+        n += 1.0;
+        ++hits;
+      }
+    });
+    c->sumA[i] = A;
+    c->card[i] = n;
+    c->out[KS * i] = s->v0 * cos(phii);
+    c->out[KS * i + 1] = s->v0 * sin(phii);
+    if (n != 0.0) c->out[KS * i + 2] = s->sigma * A / n;
+  }
+  /* per-slice counts: slot = thread's lo index is unique; summed by caller */
+  __atomic_fetch_add(&c->cnt[0], cand, __ATOMIC_RELAXED);
+  __atomic_fetch_add(&c->cnt[1], hits, __ATOMIC_RELAXED);
+}
+
+/* chimera pass 2 (Ito form only): ChimeraSystem.cpp:589-644 / :223-254.
+ * No sigma in these terms (reference as is). */
+static void chimera_pass2(void *vp, int64_t lo, int64_t hi) {
+  rhs_ctx *c = (rhs_ctx *)vp;
+  const orc_sys *s = c->s;
+  const double rho2 = s->rho * s->rho;
+  const double *d = c->out;
+  for (int64_t i = lo; i < hi; ++i) {
+    double xi, yi, phii;
+    read_pos(c, i, &xi, &yi, &phii);
+    double n = c->card[i];
+    double a0 = c->additional[2 * i], a1 = c->additional[2 * i + 1];
+    FOR_EACH_PARTNER(c, i, xi, yi, {
+      double xj, yj, phij, dx, dy;
+      read_pos(c, j, &xj, &yj, &phij);
+      class_a(&c->g, xi, yi, xj, yj, &dx, &dy);
+      double dr2 = dx * dx + dy * dy;
+      if ((dr2 <= rho2) && (n != 0)) {
+        a0 += cos(phij - phii - s->alpha) / n;
+        a1 += (d[KS * j + 2] - d[KS * i + 2]) * cos(phij - phii - s->alpha) / n;
+      }
+    });
+    a0 *= -sqrt(2.0 * s->D);
+    a1 += -2.0 * s->D * d[KS * i + 2];
+    c->additional[2 * i] = a0;
+    c->additional[2 * i + 1] = a1;
+  }
+}
+
+/* AlignmentStrength, VortexArraysSystemPtr.cpp:737-749 */
+static inline double vortex_w(const orc_sys *s, double r) {
+  if (r < s->xi_a) return s->mu_plus * (1.0 - r * r / (s->xi_a * s->xi_a));
+  if (r < s->rho) return -s->mu_minus * 4.0 * (r - s->xi_a) * (s->rho - r) / (s->rho - s->xi_a) / (s->rho - s->xi_a);
+  return 0.0;
+}
+
+/* AlignmentStrengthDerivative, VortexArraysSystemPtr.cpp:751-763 */
+static inline double vortex_dw(const orc_sys *s, double r) {
+  if (r < s->xi_a) return s->mu_plus * (-2.0 * r / s->xi_a / s->xi_a);
+  if (r < s->rho) return -s->mu_minus * 4.0 * (-2.0 * r + s->rho + s->xi_a) / (s->rho - s->xi_a) / (s->rho - s->xi_a);
+  return 0.0;
+}
+
+/* vortex pass 1: VortexArraysSystemPtr.cpp:433-505 (k-form), :584-655 (Ito
+ * form), all pairs :166-215, :249-298 */
+static void vortex_pass1(void *vp, int64_t lo, int64_t hi) {
+  rhs_ctx *c = (rhs_ctx *)vp;
+  const orc_sys *s = c->s;
+  int64_t cand = 0, hits = 0, coh = 0;
+  for (int64_t i = lo; i < hi; ++i) {
+    double xi, yi, phii;
+    read_pos(c, i, &xi, &yi, &phii);
+    double A = 0.0, B = 0.0, C = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    FOR_EACH_PARTNER(c, i, xi, yi, {
+      double xj, yj, phij, dx, dy;
+      read_pos(c, j, &xj, &yj, &phij);
+      class_a(&c->g, xi, yi, xj, yj, &dx, &dy);
+      double dr = sqrt(dx * dx + dy * dy);
+      ++cand;
+      if (dr <= s->xi_a) {
+        A += vortex_w(s, dr) * sin(phij - phii);
+        n0 += 1.0;
+        ++hits;
+      } else if (dr <= s->rho) {
+        B += vortex_w(s, dr) * sin(phij - phii);
+        n1 += 1.0;
+        ++hits;
+      }
+      if (dr <= s->xi_r) {
+        C += -s->kappa * sin(atan2(dy, dx) - phii);
+        n2 += 1.0;
+        ++coh;
+      }
+    });
+    c->card3[3 * i] = n0;
+    c->card3[3 * i + 1] = n1;
+    c->card3[3 * i + 2] = n2;
+    c->out[KS * i] = s->v0 * cos(phii);
+    c->out[KS * i + 1] = s->v0 * sin(phii);
+    if ((n0 + n1) > 0) c->out[KS * i + 2] = (A + B) / (n0 + n1);
+    if (n2 > 0) c->out[KS * i + 2] += C / n2;
+  }
+  __atomic_fetch_add(&c->cnt[0], cand, __ATOMIC_RELAXED);
+  __atomic_fetch_add(&c->cnt[1], hits, __ATOMIC_RELAXED);
+  __atomic_fetch_add(&c->cnt[2], coh, __ATOMIC_RELAXED);
+}
+
+/* vortex pass 2 (Ito form): VortexArraysSystemPtr.cpp:662-733 / :305-353 */
+static void vortex_pass2(void *vp, int64_t lo, int64_t hi) {
+  rhs_ctx *c = (rhs_ctx *)vp;
+  const orc_sys *s = c->s;
+  const double *d = c->out;
+  for (int64_t i = lo; i < hi; ++i) {
+    double xi, yi, phii;
+    read_pos(c, i, &xi, &yi, &phii);
+    double a0 = c->additional[2 * i], a1 = c->additional[2 * i + 1];
+    double norm_1 = c->card3[3 * i] + c->card3[3 * i + 1];
+    double norm_2 = c->card3[3 * i + 2];
+    FOR_EACH_PARTNER(c, i, xi, yi, {
+      double xj, yj, phij, dx, dy;
+      read_pos(c, j, &xj, &yj, &phij);
+      class_a(&c->g, xi, yi, xj, yj, &dx, &dy);
+      double dr = sqrt(dx * dx + dy * dy);
+      if ((dr <= s->rho) && (norm_1 > 0.0)) {
+        double cph = cos(phij - phii);
+        double sph = sin(phij - phii);
+        double w = vortex_w(s, dr);
+        double dw = vortex_dw(s, dr);
+        a0 += -w * cph / norm_1;
+        a1 += ((d[KS * j] - d[KS * i]) * dw * dx / dr + (d[KS * j + 1] - d[KS * i + 1]) * dw * dy / dr) * sph / norm_1 +
+              (d[KS * j + 2] - d[KS * i + 2]) * w * cph / norm_1 - 2.0 * s->D * sph * w / norm_1;
+      }
+      if ((dr <= s->xi_r) && (norm_2 > 0.0)) {
+        double cpp = cos(atan2(dy, dx) - phii);
+        double spp = sin(atan2(dy, dx) - phii);
+        a0 += s->kappa * cpp / norm_2;
+        a1 += -s->kappa *
+                  ((d[KS * j] - d[KS * i]) * (-dy) / dr / dr + (d[KS * j + 1] - d[KS * i + 1]) * dx / dr / dr -
+                   d[KS * i + 2]) *
+                  cpp / norm_2 +
+              s->kappa * s->D * spp / norm_2;
+      }
+    });
+    a0 *= sqrt(2.0 * s->D);
+    c->additional[2 * i] = a0;
+    c->additional[2 * i + 1] = a1;
+  }
+}
+
+/* linked list iff rho <= L/4 (ChimeraSystem.cpp:90, VortexArraysSystemPtr.cpp:113) */
+static int use_linked_list(const orc_sys *s) { return s->rho <= 0.25 * s->Lx; }
+
+typedef struct cell_bufs {
+  int32_t *cell_of, *cell_start, *sorted;
+} cell_bufs;
+
+static int build_cells(const orc_sys *s, rhs_ctx *c, cell_bufs *b) {
+  c->ncx = orc_num_cells(s->Lx, s->rho);
+  c->ncy = orc_num_cells(s->Ly, s->rho);
+  int64_t nc = (int64_t)c->ncx * c->ncy;
+  b->cell_of = (int32_t *)malloc(sizeof(int32_t) * (size_t)s->n);
+  b->cell_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nc + 1));
+  b->sorted = (int32_t *)malloc(sizeof(int32_t) * (size_t)s->n);
+  if (!b->cell_of || !b->cell_start || !b->sorted) return -1;
+  orc_cell_list(s, c->ncx, c->ncy, c->pos, b->cell_of, b->cell_start, b->sorted, NULL);
+  c->cell_start = b->cell_start;
+  c->sorted = b->sorted;
+  return 0;
+}
+
+static void free_cells(cell_bufs *b) {
+  free(b->cell_of);
+  free(b->cell_start);
+  free(b->sorted);
+}
+
+/* EvaluateRhs k-form: ChimeraSystem.cpp:78-100 + :258-426 / :117-168;
+ * VortexArraysSystemPtr.cpp:107-120 + :357-507 / :136-217.
+ * k_next is in/out: its phi entry is left as is when no partner is found. */
+int orc_rhs_k(const orc_sys *s, const double *state, const double *k_prev, double k_coef, double dt, double *k_next,
+              int64_t *counts) {
+  int64_t n = s->n;
+  int64_t cnt[3] = {0, 0, 0};
+  rhs_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.s = s;
+  c.g = make_geom(s->Lx, s->Ly);
+  c.use_ll = use_linked_list(s);
+  c.out = k_next;
+  c.cnt = cnt;
+  double *stage = NULL;
+  cell_bufs cb;
+  memset(&cb, 0, sizeof(cb));
+  if (c.use_ll) {
+    /* rk_system_state = s + k_coef * dt * k_prev for every component (:274-276) */
+    stage = (double *)malloc(sizeof(double) * (size_t)(KS * n));
+    for (int64_t i = 0; i < KS * n; ++i) stage[i] = state[i] + k_coef * dt * k_prev[i];
+    c.pos = stage;
+    if (build_cells(s, &c, &cb)) return -1;
+  } else {
+    c.base = state;
+    c.k_prev = k_prev;
+    c.k_coef = k_coef;
+    c.dt = dt;
+  }
+  /* stage snapshot: k_prev may alias k_next (RungeKutta4Stepper.cpp:101) */
+  double *kp_copy = NULL;
+  if (!c.use_ll && k_prev == k_next) {
+    kp_copy = (double *)malloc(sizeof(double) * (size_t)(KS * n));
+    memcpy(kp_copy, k_prev, sizeof(double) * (size_t)(KS * n));
+    c.k_prev = kp_copy;
+  }
+  if (s->system == ORC_CHIMERA) {
+    c.sumA = (double *)malloc(sizeof(double) * (size_t)n);
+    c.card = (double *)malloc(sizeof(double) * (size_t)n);
+    parallel_for(s->threads, n, chimera_pass1, &c);
+    free(c.sumA);
+    free(c.card);
+  } else {
+    c.card3 = (double *)malloc(sizeof(double) * (size_t)(3 * n));
+    parallel_for(s->threads, n, vortex_pass1, &c);
+    free(c.card3);
+  }
+  free(kp_copy);
+  free(stage);
+  free_cells(&cb);
+  if (counts) memcpy(counts, cnt, sizeof(cnt));
+  return 0;
+}
+
+/* EvaluateRhs Ito form: ChimeraSystem.cpp:102-114 + :429-645 / :171-255;
+ * VortexArraysSystemPtr.cpp:122-134 + :509-735 / :219-355.
+ * derivative and additional are in/out, as in the reference. */
+int orc_rhs_ito(const orc_sys *s, const double *state, double *derivative, double *additional, int64_t *counts) {
+  int64_t n = s->n;
+  int64_t cnt[3] = {0, 0, 0};
+  rhs_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.s = s;
+  c.g = make_geom(s->Lx, s->Ly);
+  c.use_ll = use_linked_list(s);
+  c.pos = state;
+  c.out = derivative;
+  c.additional = additional;
+  c.cnt = cnt;
+  cell_bufs cb;
+  memset(&cb, 0, sizeof(cb));
+  if (c.use_ll && build_cells(s, &c, &cb)) return -1;
+  if (s->system == ORC_CHIMERA) {
+    c.sumA = (double *)malloc(sizeof(double) * (size_t)n);
+    c.card = (double *)malloc(sizeof(double) * (size_t)n);
+    parallel_for(s->threads, n, chimera_pass1, &c);
+    parallel_for(s->threads, n, chimera_pass2, &c);
+    free(c.sumA);
+    free(c.card);
+  } else {
+    c.card3 = (double *)malloc(sizeof(double) * (size_t)(3 * n));
+    parallel_for(s->threads, n, vortex_pass1, &c);
+    parallel_for(s->threads, n, vortex_pass2, &c);
+    free(c.card3);
+  }
+  free_cells(&cb);
+  if (counts) memcpy(counts, cnt, sizeof(cnt));
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* steppers                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Stochastic Euler.  variant 0 (StochasticEulerStepperPtr.hpp:44-53, with the
+ * natural frequency added as (omega + a_phi) * dt like
+ * StochasticRungeKuttaStepperPtr.hpp:65):
+ *   x += a_x dt; y += a_y dt; phi += (omega + a_phi) dt + sqrt(2D) (sqrt(dt) xi)
+ * variant 1 (StochasticEulerStepper.hpp:136-145 + ChimeraSystem::AddNoise
+ * ChimeraSystem.cpp:924-932, no omega):
+ *   x += dt a_x + sqrt(dt) 0;  phi += dt a_phi + sqrt(dt) (sqrt(2D) xi)
+ * The RHS is the k-form with zero-filled k (derivative untouched if n_i = 0). */
+int orc_step_se(const orc_sys *s, double *st, double dt, const double *omega, const double *xi, int32_t variant) {
+  int64_t n = s->n;
+  double *d = (double *)calloc((size_t)(KS * n), sizeof(double));
+  if (!d) return -1;
+  int rc = orc_rhs_k(s, st, d, 0.0, dt, d, NULL);
+  if (rc) { free(d); return rc; }
+  if (variant == 0) {
+    const double sq2d = sqrt(2.0 * s->D);
+    for (int64_t i = 0; i < n; ++i) {
+      double dW = sqrt(dt) * xi[i];
+      double w = omega ? omega[i] : 0.0;
+      st[KS * i] += d[KS * i] * dt;
+      st[KS * i + 1] += d[KS * i + 1] * dt;
+      st[KS * i + 2] += (w + d[KS * i + 2]) * dt + sq2d * dW;
+    }
+  } else {
+    const double intensity = sqrt(2.0 * s->D);
+    for (int64_t i = 0; i < n; ++i) {
+      double stoch = intensity * xi[i];
+      st[KS * i] += dt * d[KS * i] + sqrt(dt) * 0.0;
+      st[KS * i + 1] += dt * d[KS * i + 1] + sqrt(dt) * 0.0;
+      st[KS * i + 2] += dt * d[KS * i + 2] + sqrt(dt) * stoch;
+    }
+  }
+  free(d);
+  orc_wrap_state(s, st);
+  return 0;
+}
+
+/* Stochastic Runge-Kutta (order-1.5 Ito-Taylor), StochasticRungeKuttaStepperPtr.hpp:41-68
+ * (chimera, with omega) and :70-97 (vortex, no omega).  xi2 holds (U1, U2) per
+ * particle. */
+int orc_step_srk(const orc_sys *s, double *st, double dt, const double *omega, const double *xi2) {
+  int64_t n = s->n;
+  double *d = (double *)calloc((size_t)(KS * n), sizeof(double));
+  double *add = (double *)calloc((size_t)(2 * n), sizeof(double));
+  if (!d || !add) { free(d); free(add); return -1; }
+  int rc = orc_rhs_ito(s, st, d, add, NULL);
+  if (rc) { free(d); free(add); return rc; }
+  const double D = s->D;
+  for (int64_t i = 0; i < n; ++i) {
+    double U_1 = xi2[2 * i], U_2 = xi2[2 * i + 1];
+    double delta_W = U_1 * sqrt(dt);
+    double delta_Z = 0.5 * dt * sqrt(dt) * (U_1 + U_2 / sqrt(3.0));
+    int64_t ii = KS * i;
+    st[ii] += d[ii] * dt - sqrt(2.0 * D) * d[ii + 1] * delta_Z - 0.5 * (d[ii + 2] * d[ii + 1] + D * d[ii]) * dt * dt;
+    st[ii + 1] += d[ii + 1] * dt + sqrt(2.0 * D) * d[ii] * delta_Z + 0.5 * (d[ii + 2] * d[ii] - D * d[ii + 1]) * dt * dt;
+    if (s->system == ORC_CHIMERA) {
+      double w = omega ? omega[i] : 0.0;
+      st[ii + 2] += (w + d[ii + 2]) * dt + sqrt(2.0 * D) * delta_W + add[2 * i] * delta_Z + 0.5 * add[2 * i + 1] * dt * dt;
+    } else {
+      st[ii + 2] += d[ii + 2] * dt + sqrt(2.0 * D) * delta_W + add[2 * i] * delta_Z + 0.5 * add[2 * i + 1] * dt * dt;
+    }
+  }
+  free(d);
+  free(add);
+  orc_wrap_state(s, st);
+  return 0;
+}
+
+/* Classic RK4, RungeKutta4Stepper.cpp:95-127 / RungeKutta4StepperPtr.cpp:25-65.
+ * kbuf = k_1_ | k_2_ | k_3_ | k_4_ (4 x 3N), persistent across steps: a phi
+ * entry with no partner keeps its previous value. */
+int orc_step_rk4(const orc_sys *s, double *st, double dt, double *kbuf) {
+  int64_t n = s->n, m = KS * n;
+  double *k1 = kbuf, *k2 = kbuf + m, *k3 = kbuf + 2 * m, *k4 = kbuf + 3 * m;
+  int rc = orc_rhs_k(s, st, k1, 0.0, dt, k1, NULL);
+  if (!rc) rc = orc_rhs_k(s, st, k1, 0.5, dt, k2, NULL);
+  if (!rc) rc = orc_rhs_k(s, st, k2, 0.5, dt, k3, NULL);
+  if (!rc) rc = orc_rhs_k(s, st, k3, 1.0, dt, k4, NULL);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t ii = KS * i;
+    st[ii] += (k1[ii] + 2.0 * k2[ii] + 2.0 * k3[ii] + k4[ii]) * dt / 6.0;
+    st[ii + 1] += (k1[ii + 1] + 2.0 * k2[ii + 1] + 2.0 * k3[ii + 1] + k4[ii + 1]) * dt / 6.0;
+    st[ii + 2] += (k1[ii + 2] + 2.0 * k2[ii + 2] + 2.0 * k3[ii + 2] + k4[ii + 2]) / 6.0 * dt;
+  }
+  orc_wrap_state(s, st);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* order parameter, BinaryObserverPtr.cpp:237-247                              */
+/* ------------------------------------------------------------------------ */
+
+void orc_order_param(const double *st, int64_t n, double *R, double *psi) {
+  double cr = 0.0, ci = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    cr += cos(st[KS * i + 2]);
+    ci += sin(st[KS * i + 2]);
+  }
+  cr /= (double)n;
+  ci /= (double)n;
+  *R = hypot(cr, ci);
+  *psi = atan2(ci, cr);
+}
+
+/* the reference's own float form: complex<float> sum, /= kN, abs, arg */
+void orc_order_param_f32(const double *st, int64_t n, float *R, float *psi) {
+  float cr = 0.0f, ci = 0.0f;
+  for (int64_t i = 0; i < n; ++i) {
+    cr += (float)cos(st[KS * i + 2]);
+    ci += (float)sin(st[KS * i + 2]);
+  }
+  cr /= (float)n;
+  ci /= (float)n;
+  *R = hypotf(cr, ci);
+  *psi = atan2f(ci, cr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and the spc2 normal      */
+/* transform, restated independently of the device code.                       */
+/* ------------------------------------------------------------------------ */
+
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+/* log(x) for x in (0, 1] from IEEE basic operations only:
+ * x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1),
+ * series to s^23; log x = e ln2_hi + (e ln2_lo + log m) (Cody-Waite split). */
+static double nlog_basic(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  double s = (m - 1.0) / (m + 1.0);
+  double z = s * s;
+  double p = 0.043478260869565216; /* 1/23 */
+  p = p * z + 0.047619047619047616; /* 1/21 */
+  p = p * z + 0.052631578947368418; /* 1/19 */
+  p = p * z + 0.058823529411764705; /* 1/17 */
+  p = p * z + 0.066666666666666666; /* 1/15 */
+  p = p * z + 0.076923076923076927; /* 1/13 */
+  p = p * z + 0.090909090909090912; /* 1/11 */
+  p = p * z + 0.11111111111111110;  /* 1/9 */
+  p = p * z + 0.14285714285714285;  /* 1/7 */
+  p = p * z + 0.20000000000000001;  /* 1/5 */
+  p = p * z + 0.33333333333333331;  /* 1/3 */
+  double logm = 2.0 * s + 2.0 * s * (z * p);
+  double de = (double)e;
+  return de * 6.93147180369123816490e-01 + (de * 1.90821492927058770002e-10 + logm);
+}
+
+/* sin and cos of (pi/2) g for |g| <= 1/2 by Taylor series (basic ops only) */
+static void sincos_quarter(double g, double *sn, double *cs) {
+  double x = g * 1.5707963267948966;
+  double z = x * x;
+  double ps = 2.8114572543455206e-15; /* 1/17! */
+  ps = ps * z - 7.6471637318198164e-13; /* 1/15! */
+  ps = ps * z + 1.6059043836821613e-10; /* 1/13! */
+  ps = ps * z - 2.5052108385441720e-08; /* 1/11! */
+  ps = ps * z + 2.7557319223985893e-06; /* 1/9! */
+  ps = ps * z - 1.9841269841269841e-04; /* 1/7! */
+  ps = ps * z + 8.3333333333333332e-03; /* 1/5! */
+  ps = ps * z - 1.6666666666666666e-01; /* 1/3! */
+  *sn = x + x * (z * ps);
+  double pc = 1.5619206968586225e-16; /* 1/18! */
+  pc = pc * z - 4.7794773323873853e-14; /* 1/16! */
+  pc = pc * z + 1.1470745597729725e-11; /* 1/14! */
+  pc = pc * z - 2.0876756987868100e-09; /* 1/12! */
+  pc = pc * z + 2.7557319223985888e-07; /* 1/10! */
+  pc = pc * z - 2.4801587301587302e-05; /* 1/8! */
+  pc = pc * z + 1.3888888888888889e-03; /* 1/6! */
+  pc = pc * z - 4.1666666666666664e-02; /* 1/4! */
+  pc = pc * z + 0.5;                    /* 1/2! */
+  *cs = 1.0 - z * pc;
+}
+
+/* Box-Muller pair from one Philox block:
+ * u1 = (((w1 w0) >> 11) + 1) 2^-53 in (0, 1], u2 = ((w3 w2) >> 11) 2^-53 in [0, 1),
+ * r = sqrt(-2 log u1), (z0, z1) = r (cos 2 pi u2, sin 2 pi u2); 2 pi u2 is
+ * reduced exactly by quadrant: q = nearest integer to 4 u2, g = 4 u2 - q. */
+static void box_muller(const uint32_t w[4], double *z0, double *z1) {
+  uint64_t a = ((uint64_t)w[1] << 32) | w[0];
+  uint64_t b = ((uint64_t)w[3] << 32) | w[2];
+  double u1 = (double)((a >> 11) + 1) * 0x1.0p-53;
+  double u2 = (double)(b >> 11) * 0x1.0p-53;
+  double r = sqrt(-2.0 * nlog_basic(u1));
+  double f = u2 * 4.0;
+  double q = floor(f + 0.5);
+  double g = f - q;
+  double sn, cs;
+  sincos_quarter(g, &sn, &cs);
+  int iq = ((int)q) & 3;
+  double c, s;
+  if (iq == 0) { c = cs; s = sn; }
+  else if (iq == 1) { c = -sn; s = cs; }
+  else if (iq == 2) { c = -cs; s = -sn; }
+  else { c = sn; s = -cs; }
+  *z0 = r * c;
+  *z1 = r * s;
+}
+
+/* Step noise: counter (i, stream, step_lo, step_hi), key (seed_lo, seed_hi).
+ * per_particle = 1 writes z0 per particle; 2 writes (z0, z1) per particle. */
+void orc_philox_normals(uint64_t seed, uint64_t step, uint32_t stream, int64_t n, int32_t per_particle, double *out) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t ctr[4] = {(uint32_t)i, stream, (uint32_t)step, (uint32_t)(step >> 32)};
+    uint32_t w[4];
+    orc_philox4x32_10(ctr, key, w);
+    double z0, z1;
+    box_muller(w, &z0, &z1);
+    if (per_particle == 1) {
+      out[i] = z0;
+    } else {
+      out[2 * i] = z0;
+      out[2 * i + 1] = z1;
+    }
+  }
+}
+
+/* Philox-keyed initial state (the device's spc2_init_random_state):
+ * (ux, uy) from stream 1, up from stream 2, omega from the first normal of
+ * stream 3, all at step 0; x = ux Lx, y = uy Ly, phi = up 2 pi. */
+void orc_init_state(uint64_t seed, int64_t n, double Lx, double Ly, double omega_std, double *st, double *omega) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t w[4];
+    uint32_t c1[4] = {(uint32_t)i, 1u, 0u, 0u};
+    orc_philox4x32_10(c1, key, w);
+    double ux = (double)((((uint64_t)w[1] << 32) | w[0]) >> 11) * 0x1.0p-53;
+    double uy = (double)((((uint64_t)w[3] << 32) | w[2]) >> 11) * 0x1.0p-53;
+    uint32_t c2[4] = {(uint32_t)i, 2u, 0u, 0u};
+    orc_philox4x32_10(c2, key, w);
+    double up = (double)((((uint64_t)w[1] << 32) | w[0]) >> 11) * 0x1.0p-53;
+    uint32_t c3[4] = {(uint32_t)i, 3u, 0u, 0u};
+    orc_philox4x32_10(c3, key, w);
+    double z0, z1;
+    box_muller(w, &z0, &z1);
+    st[KS * i] = ux * Lx;
+    st[KS * i + 1] = uy * Ly;
+    st[KS * i + 2] = up * 6.283185307179586;
+    omega[i] = z0 * omega_std;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* mt19937 (Matsumoto & Nishimura 1998) and libstdc++'s (GCC 11)              */
+/* std::normal_distribution<double>: Marsaglia polar method over               */
+/* generate_canonical<double, 53> (two 32-bit draws, w0 + w1 2^32, / 2^64).    */
+/* Replays the noise the reference's steppers consume (main.cpp:9,             */
+/* StochasticRungeKuttaStepper.hpp:42-49, ChimeraSystem.cpp:926-930).          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct mt_state {
+  uint32_t mt[624];
+  int idx;
+} mt_state;
+
+size_t orc_mt_state_size(void) { return sizeof(mt_state); }
+
+void orc_mt_seed(void *vp, uint32_t seed) {
+  mt_state *m = (mt_state *)vp;
+  m->mt[0] = seed;
+  for (int i = 1; i < 624; ++i) m->mt[i] = 1812433253u * (m->mt[i - 1] ^ (m->mt[i - 1] >> 30)) + (uint32_t)i;
+  m->idx = 624;
+}
+
+static uint32_t mt_next(mt_state *m) {
+  if (m->idx >= 624) {
+    for (int i = 0; i < 624; ++i) {
+      uint32_t y = (m->mt[i] & 0x80000000u) | (m->mt[(i + 1) % 624] & 0x7fffffffu);
+      m->mt[i] = m->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    m->idx = 0;
+  }
+  uint32_t y = m->mt[m->idx++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+static double mt_canonical(mt_state *m) {
+  double sum = 0.0, tmp = 1.0;
+  for (int k = 0; k < 2; ++k) {
+    sum += (double)mt_next(m) * tmp;
+    tmp *= 4294967296.0;
+  }
+  double r = sum / tmp;
+  if (r >= 1.0) r = nextafter(1.0, 0.0);
+  return r;
+}
+
+/* `count` draws from a fresh distribution (empty pair cache). */
+void orc_mt_normals(void *vp, int64_t count, double *out) {
+  mt_state *m = (mt_state *)vp;
+  int saved_ok = 0;
+  double saved = 0.0;
+  for (int64_t k = 0; k < count; ++k) {
+    double ret;
+    if (saved_ok) {
+      saved_ok = 0;
+      ret = saved;
+    } else {
+      double x, y, r2;
+      do {
+        x = 2.0 * mt_canonical(m) - 1.0;
+        y = 2.0 * mt_canonical(m) - 1.0;
+        r2 = x * x + y * y;
+      } while (r2 > 1.0 || r2 == 0.0);
+      double mult = sqrt(-2 * log(r2) / r2);
+      saved = x * mult;
+      saved_ok = 1;
+      ret = y * mult;
+    }
+    out[k] = ret * 1.0 + 0.0;
+  }
+}

@@ -1,0 +1,233 @@
+"""ctypes binding of the C ABI declared in include/spc2_hip.h.
+
+The product's compute path is libspc2_hip.so; this module only moves numpy
+arrays across the ABI for tests, benchmarks and Python callers.  Loading fails
+loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libspc2_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "spc2_hip.h")
+
+CHIMERA, VORTEX = 0, 1
+FP64, FP32_KAHAN, FP64_FACTORED = 0, 1, 2
+SE, SRK, RK4 = 0, 1, 2
+NEIGH_AUTO, NEIGH_CELLS, NEIGH_ALL_PAIRS = 0, 1, 2
+
+PRECISIONS = {"fp64": FP64, "fp32k": FP32_KAHAN, "fp64f": FP64_FACTORED}
+METHODS = {"se": SE, "srk": SRK, "rk4": RK4}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("system", C.c_int32), ("precision", C.c_int32), ("neighbour", C.c_int32), ("reserved", C.c_int32),
+        ("n", C.c_int64), ("Lx", C.c_double), ("Ly", C.c_double), ("rho", C.c_double),
+        ("v0", C.c_double), ("sigma", C.c_double), ("alpha", C.c_double), ("D", C.c_double),
+        ("mu_plus", C.c_double), ("mu_minus", C.c_double), ("xi_a", C.c_double), ("xi_r", C.c_double),
+        ("kappa", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+class Spc2Error(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads libspc2_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Spc2Error(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                        "(make -C spc2_langevin_integration_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    h = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    st = C.c_int
+    sig = {
+        "spc2_default_params": (None, [C.POINTER(Params)]),
+        "spc2_create": (st, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+        "spc2_destroy": (st, [h]),
+        "spc2_last_error": (C.c_char_p, []),
+        "spc2_set_state": (st, [h, dp]),
+        "spc2_get_state": (st, [h, dp]),
+        "spc2_init_random_state": (st, [h, C.c_double]),
+        "spc2_set_omega": (st, [h, dp]),
+        "spc2_eval_rhs": (st, [h, dp, C.c_double, C.c_double, dp]),
+        "spc2_eval_rhs_ito": (st, [h, dp, dp]),
+        "spc2_step": (st, [h, C.c_int32, C.c_double, C.c_int64, C.c_int64, dp]),
+        "spc2_noise": (st, [h, C.c_int32, C.c_int64, dp]),
+        "spc2_order_parameter": (st, [h, dp, dp]),
+        "spc2_get_cell_list": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "spc2_num_cells": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "spc2_set_counting": (st, [h, C.c_int32]),
+        "spc2_get_counts": (st, [h, C.POINTER(C.c_int64)]),
+        "spc2_set_profiling": (st, [h, C.c_int32]),
+        "spc2_get_profile": (st, [h, dp, C.POINTER(C.c_int64)]),
+        "spc2_synchronize": (st, [h]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().spc2_last_error().decode()
+        raise Spc2Error(f"spc2 status {rc}: {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def default_params() -> Params:
+    p = Params()
+    load().spc2_default_params(C.byref(p))
+    return p
+
+
+def params(system="chimera", precision="fp64", neighbour="auto", **kw) -> Params:
+    p = default_params()
+    p.system = CHIMERA if system == "chimera" else VORTEX
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p.neighbour = {"auto": NEIGH_AUTO, "cells": NEIGH_CELLS, "all_pairs": NEIGH_ALL_PAIRS}[neighbour]
+    if "L" in kw:
+        L = kw.pop("L")
+        p.Lx = p.Ly = L
+    if "velocity" in kw:
+        kw["v0"] = kw.pop("velocity")
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Sim:
+    """One device-resident simulation (an spc2_sim_t handle)."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        self.n = int(p.n)
+        h = C.c_void_p()
+        _check(load().spc2_create(C.byref(p), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().spc2_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, state):
+        st = np.ascontiguousarray(state, np.float64)
+        assert st.size == 3 * self.n
+        _check(load().spc2_set_state(self.h, _dp(st)))
+
+    def get_state(self):
+        st = np.empty(3 * self.n, np.float64)
+        _check(load().spc2_get_state(self.h, _dp(st)))
+        return st
+
+    def init_random_state(self, omega_std=0.0):
+        _check(load().spc2_init_random_state(self.h, omega_std))
+
+    def set_omega(self, omega):
+        om = np.ascontiguousarray(omega, np.float64)
+        assert om.size == self.n
+        _check(load().spc2_set_omega(self.h, _dp(om)))
+
+    def eval_rhs(self, k_prev, k_coef, dt, k_next):
+        kp = np.ascontiguousarray(k_prev, np.float64)
+        kn = np.array(k_next, np.float64, copy=True)
+        _check(load().spc2_eval_rhs(self.h, _dp(kp), k_coef, dt, _dp(kn)))
+        return kn
+
+    def eval_rhs_ito(self, derivative=None, additional=None):
+        d = np.zeros(3 * self.n) if derivative is None else np.array(derivative, np.float64, copy=True)
+        a = np.zeros(2 * self.n) if additional is None else np.array(additional, np.float64, copy=True)
+        _check(load().spc2_eval_rhs_ito(self.h, _dp(d), _dp(a)))
+        return d, a
+
+    def step(self, method, dt, step0=0, nsteps=1, noise=None):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, np.float64)
+        _check(load().spc2_step(self.h, m, dt, step0, nsteps, _dp(nz) if nz is not None else None))
+
+    def noise(self, method, step):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        out = np.empty(self.n * (2 if m == SRK else 1), np.float64)
+        _check(load().spc2_noise(self.h, m, step, _dp(out)))
+        return out
+
+    def order_parameter(self):
+        R, psi = C.c_double(), C.c_double()
+        _check(load().spc2_order_parameter(self.h, C.byref(R), C.byref(psi)))
+        return R.value, psi.value
+
+    def num_cells(self):
+        nx, ny = C.c_int32(), C.c_int32()
+        _check(load().spc2_num_cells(self.h, C.byref(nx), C.byref(ny)))
+        return nx.value, ny.value
+
+    def cell_list(self):
+        nx, ny = self.num_cells()
+        cell_of = np.empty(self.n, np.int32)
+        start = np.empty(nx * ny + 1, np.int32)
+        srt = np.empty(self.n, np.int32)
+        _check(load().spc2_get_cell_list(self.h, _ip(cell_of), _ip(start), _ip(srt)))
+        return cell_of, start, srt
+
+    def set_counting(self, on=True):
+        _check(load().spc2_set_counting(self.h, int(on)))
+
+    def counts(self):
+        c = np.zeros(3, np.int64)
+        _check(load().spc2_get_counts(self.h, c.ctypes.data_as(C.POINTER(C.c_int64))))
+        return c
+
+    def set_profiling(self, on=True):
+        _check(load().spc2_set_profiling(self.h, int(on)))
+
+    def profile(self):
+        ms, n = C.c_double(), C.c_int64()
+        _check(load().spc2_get_profile(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def synchronize(self):
+        _check(load().spc2_synchronize(self.h))
+
+
+def declared_symbols(header=HEADER_PATH):
+    """Names of the functions include/spc2_hip.h declares."""
+    import re
+    txt = open(header).read()
+    return sorted(set(re.findall(r"\b(spc2_[a-z0-9_]+)\s*\(", txt)))

@@ -1,0 +1,512 @@
+// spc2 C ABI: handle lifetime, host <-> device copies and the per-method
+// pipelines that chain the kernels of spc2_kernels.hip on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "spc2_internal.h"
+
+using namespace spc2;
+
+struct spc2_sim {
+  Sim s;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+spc2_status fail(spc2_status st, const std::string &msg) {
+  g_err = msg;
+  return st;
+}
+
+#define HIP_TRY(expr)                                                                           \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail(SPC2_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+// Checks the launches queued since the last check.
+spc2_status check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SPC2_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return SPC2_OK;
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+  return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(count, 1));
+}
+
+void free_all(Sim &s) {
+  auto f = [](void *p) {
+    if (p) (void)hipFree(p);
+  };
+  f(s.sx); f(s.sy); f(s.sp); f(s.omega);
+  for (auto &kk : s.k)
+    for (auto &p : kk) f(p);
+  for (auto &p : s.der) f(p);
+  for (auto &p : s.add) f(p);
+  f(s.gx); f(s.gy); f(s.gp);
+  f(s.cell_of); f(s.count); f(s.local); f(s.start); f(s.tmp); f(s.sorted);
+  f(s.t.x); f(s.t.y); f(s.t.p); f(s.t.c); f(s.t.s); f(s.t.f4); f(s.t.cell);
+  f(s.aphi_slot); f(s.n_slot); f(s.aphi_slot_f);
+  f(s.noise); f(s.counts); f(s.red);
+}
+
+void fill_kparams(Sim &s) {
+  const spc2_params &P = s.P;
+  KParams &k = s.kp;
+  k.n = s.n;
+  k.nx = s.nx;
+  k.ny = s.ny;
+  k.linked = s.linked;
+  k.Lx = P.Lx;
+  k.Ly = P.Ly;
+  k.rLx = 1.0 / P.Lx;
+  k.rLy = 1.0 / P.Ly;
+  k.hx = s.nx > 0 ? P.Lx / s.nx : P.Lx;
+  k.hy = s.ny > 0 ? P.Ly / s.ny : P.Ly;
+  k.hxf = (float)k.hx;
+  k.hyf = (float)k.hy;
+  k.rho = P.rho;
+  k.rho2 = P.rho * P.rho;  // rho_squared_, ChimeraSystem.cpp:24
+  k.rho2f = (float)k.rho2;
+  k.v0 = P.v0;
+  k.sigma = P.sigma;
+  k.alpha = P.alpha;
+  k.D = P.D;
+  k.mu_plus = P.mu_plus;
+  k.mu_minus = P.mu_minus;
+  k.xi_a = P.xi_a;
+  k.xi_r = P.xi_r;
+  k.kappa = P.kappa;
+  double outer = P.system == SPC2_VORTEX ? std::max(P.rho, P.xi_r) : P.rho;
+  k.outer2 = outer * outer;
+  k.xa2f = (float)(P.xi_a * P.xi_a);
+  k.xr2f = (float)(P.xi_r * P.xi_r);
+  k.outer2f = (float)k.outer2;
+  k.xaf = (float)P.xi_a;
+  k.rhof = (float)P.rho;
+  k.mu_pf = (float)P.mu_plus;
+  k.mu_p_over_xa2f = P.xi_a > 0 ? (float)(P.mu_plus / (P.xi_a * P.xi_a)) : 0.0f;
+  k.c4f = P.rho > P.xi_a ? (float)(-P.mu_minus * 4.0 / ((P.rho - P.xi_a) * (P.rho - P.xi_a))) : 0.0f;
+  k.kappaf = (float)P.kappa;
+  // FP32 r^2 from cell-local coordinates is within a few 1e-7 relative of the
+  // FP64 value; pairs closer than 1e-5 relative to a cutoff take the FP64 test.
+  k.band_rho2f = (float)(1e-5 * k.rho2);
+  k.band_xr2f = (float)(1e-5 * P.xi_r * P.xi_r);
+}
+
+spc2_status alloc_all(Sim &s) {
+  const size_t n = (size_t)s.n, nc = (size_t)std::max(s.nc, 1);
+#define A(ptr, cnt) HIP_TRY(dalloc(&(ptr), (cnt)))
+  A(s.sx, n); A(s.sy, n); A(s.sp, n); A(s.omega, n);
+  for (auto &kk : s.k)
+    for (auto &p : kk) A(p, n);
+  for (auto &p : s.der) A(p, n);
+  for (auto &p : s.add) A(p, n);
+  A(s.gx, n); A(s.gy, n); A(s.gp, n);
+  A(s.cell_of, n); A(s.count, nc); A(s.local, n); A(s.start, nc + 1); A(s.tmp, n); A(s.sorted, n);
+  A(s.t.cell, n);
+  if (s.prec == P32K) {
+    A(s.t.f4, n);
+    A(s.t.x, n); A(s.t.y, n);  // FP64 positions for the cutoff-band re-check
+  } else {
+    A(s.t.x, n); A(s.t.y, n); A(s.t.p, n);
+    if (s.prec == P64F) {
+      A(s.t.c, n); A(s.t.s, n);
+    }
+  }
+  A(s.aphi_slot, n); A(s.n_slot, n); A(s.aphi_slot_f, n);
+  A(s.counts, 3); A(s.red, 2 * 1024 + 8);
+#undef A
+  HIP_TRY(hipMemset(s.count, 0, sizeof(int) * nc));
+  HIP_TRY(hipMemset(s.omega, 0, sizeof(double) * n));
+  for (auto &kk : s.k)
+    for (auto &p : kk) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
+  for (auto &p : s.der) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
+  for (auto &p : s.add) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
+  HIP_TRY(hipMemset(s.counts, 0, sizeof(unsigned long long) * 3));
+  HIP_TRY(hipMemset(s.sx, 0, sizeof(double) * n));
+  HIP_TRY(hipMemset(s.sy, 0, sizeof(double) * n));
+  HIP_TRY(hipMemset(s.sp, 0, sizeof(double) * n));
+  return SPC2_OK;
+}
+
+// AoS [x, y, phi] host array <-> three device arrays
+spc2_status upload_aos(Sim &s, const double *aos, int stride, double *const *dst, int comps) {
+  std::vector<double> buf((size_t)s.n);
+  for (int c = 0; c < comps; ++c) {
+    for (int i = 0; i < s.n; ++i) buf[(size_t)i] = aos[(size_t)i * stride + c];
+    HIP_TRY(hipMemcpyAsync(dst[c], buf.data(), sizeof(double) * (size_t)s.n, hipMemcpyHostToDevice, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+  }
+  return SPC2_OK;
+}
+
+spc2_status download_aos(Sim &s, double *aos, int stride, double *const *src, int comps) {
+  std::vector<double> buf((size_t)s.n);
+  for (int c = 0; c < comps; ++c) {
+    HIP_TRY(hipMemcpyAsync(buf.data(), src[c], sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    for (int i = 0; i < s.n; ++i) aos[(size_t)i * stride + c] = buf[(size_t)i];
+  }
+  return SPC2_OK;
+}
+
+// Stage + cell list + tiles for one RHS evaluation.
+void build(Sim &s, double *const *kprev, double kcdt) {
+  if (kprev)
+    launch_stage_bin(s, kprev[0], kprev[1], kprev[2], kcdt);
+  else
+    launch_stage_bin(s, nullptr, nullptr, nullptr, 0.0);
+  if (s.linked) {
+    launch_scan(s);
+    launch_place(s);
+    launch_rank_tiles(s);
+  }
+}
+
+StepArgs step_args(const Sim &s, double dt, uint64_t step) {
+  StepArgs a{};
+  a.dt = dt;
+  a.sqdt = std::sqrt(dt);
+  a.sq2D = std::sqrt(2.0 * s.P.D);
+  a.h1 = 0.5 * dt * std::sqrt(dt);
+  a.sqrt3 = std::sqrt(3.0);
+  a.D = s.P.D;
+  a.seed = s.P.seed;
+  a.step = step;
+  a.noise = nullptr;
+  a.omega = s.omega;
+  a.zero_in = 0;
+  return a;
+}
+
+spc2_status check_handle(spc2_sim_t h) {
+  if (!h) return fail(SPC2_ERR_INVALID, "null handle");
+  return SPC2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void spc2_default_params(spc2_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->system = SPC2_CHIMERA;
+  p->precision = SPC2_FP64;
+  p->neighbour = SPC2_NEIGH_AUTO;
+  p->n = 1000;
+  p->Lx = p->Ly = 1.0;
+  p->rho = 0.1;
+  p->v0 = 0.01;
+  p->sigma = 1.0;
+  p->alpha = 1.45;
+  p->D = 0.01;
+  p->mu_plus = p->mu_minus = 100.0;
+  p->xi_a = 0.01;
+  p->xi_r = 0.005;
+  p->kappa = 20.0;
+  p->seed = 42;
+}
+
+const char *spc2_last_error(void) { return g_err.c_str(); }
+
+spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
+  g_err.clear();
+  if (!p || !out) return fail(SPC2_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (p->n <= 0 || p->n >= (1LL << 31) - 1) return fail(SPC2_ERR_INVALID, "n must be in [1, 2^31 - 1)");
+  if (!(p->Lx > 0.0) || !(p->Ly > 0.0) || !(p->rho > 0.0)) return fail(SPC2_ERR_INVALID, "Lx, Ly, rho must be > 0");
+  if (p->system != SPC2_CHIMERA && p->system != SPC2_VORTEX) return fail(SPC2_ERR_INVALID, "unknown system");
+  if (p->precision < SPC2_FP64 || p->precision > SPC2_FP64_FACTORED)
+    return fail(SPC2_ERR_INVALID, "unknown precision");
+  if (p->neighbour < SPC2_NEIGH_AUTO || p->neighbour > SPC2_NEIGH_ALL_PAIRS)
+    return fail(SPC2_ERR_INVALID, "unknown neighbour mode");
+  if (!(p->D >= 0.0)) return fail(SPC2_ERR_INVALID, "D must be >= 0");
+  spc2_sim *h = new (std::nothrow) spc2_sim();
+  if (!h) return fail(SPC2_ERR_NOMEM, "host allocation failed");
+  Sim &s = h->s;
+  s.P = *p;
+  s.prec = p->precision == SPC2_FP64 ? P64 : (p->precision == SPC2_FP32_KAHAN ? P32K : P64F);
+  s.n = (int)p->n;
+  if (p->neighbour == SPC2_NEIGH_AUTO)
+    s.linked = p->rho <= 0.25 * p->Lx;  // ChimeraSystem.cpp:90, VortexArraysSystemPtr.cpp:113
+  else
+    s.linked = p->neighbour == SPC2_NEIGH_CELLS;
+  if (s.linked) {
+    double fx = p->Lx / p->rho, fy = p->Ly / p->rho;
+    if (fx > 2.0e9 || fy > 2.0e9 || fx * fy > 1.0e9) {
+      delete h;
+      return fail(SPC2_ERR_INVALID, "too many cells for the linked-cell path");
+    }
+    s.nx = (int)(p->Lx / p->rho);  // num_subcells_, ChimeraSystem.cpp:32-33
+    s.ny = (int)(p->Ly / p->rho);
+    if (s.nx < 3 || s.ny < 3) {
+      delete h;
+      return fail(SPC2_ERR_INVALID, "linked cells need at least 3 cells per side (rho <= L/3)");
+    }
+    s.nc = s.nx * s.ny;
+  }
+  fill_kparams(s);
+  hipError_t e = hipGetDevice(&s.device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    std::string m = std::string("HIP init: ") + hipGetErrorString(e);
+    delete h;
+    return fail(SPC2_ERR_HIP, m);
+  }
+  spc2_status st = alloc_all(s);
+  if (st != SPC2_OK) {
+    std::string m = g_err;
+    free_all(s);
+    (void)hipStreamDestroy(s.stream);
+    delete h;
+    return fail(SPC2_ERR_NOMEM, m);
+  }
+  *out = h;
+  return SPC2_OK;
+}
+
+spc2_status spc2_destroy(spc2_sim_t h) {
+  if (!h) return SPC2_OK;
+  Sim &s = h->s;
+  (void)hipStreamSynchronize(s.stream);
+  for (auto &pr : h->ev_pending) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  for (auto &pr : h->ev_free) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  if (s.ev0) (void)hipEventDestroy(s.ev0);
+  if (s.ev1) (void)hipEventDestroy(s.ev1);
+  free_all(s);
+  (void)hipStreamDestroy(s.stream);
+  delete h;
+  return SPC2_OK;
+}
+
+spc2_status spc2_set_state(spc2_sim_t h, const double *state) {
+  if (check_handle(h) || !state) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  double *d[3] = {s.sx, s.sy, s.sp};
+  return upload_aos(s, state, 3, d, 3);
+}
+
+spc2_status spc2_get_state(spc2_sim_t h, double *state) {
+  if (check_handle(h) || !state) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  double *d[3] = {s.sx, s.sy, s.sp};
+  return download_aos(s, state, 3, d, 3);
+}
+
+spc2_status spc2_init_random_state(spc2_sim_t h, double omega_std) {
+  if (check_handle(h)) return SPC2_ERR_INVALID;
+  if (!(omega_std >= 0.0)) return fail(SPC2_ERR_INVALID, "omega_std must be >= 0");
+  Sim &s = h->s;
+  launch_init_state(s, omega_std);
+  if (spc2_status st = check_launch("init_state")) return st;
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+spc2_status spc2_set_omega(spc2_sim_t h, const double *omega) {
+  if (check_handle(h) || !omega) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  HIP_TRY(hipMemcpyAsync(s.omega, omega, sizeof(double) * (size_t)s.n, hipMemcpyHostToDevice, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+spc2_status spc2_eval_rhs(spc2_sim_t h, const double *k_prev, double k_coef, double dt, double *k_next) {
+  if (check_handle(h) || !k_prev || !k_next) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  // k_prev -> k[1] (scratch), k_next (in/out) -> k[0]
+  if (spc2_status st = upload_aos(s, k_prev, 3, s.k[1], 3)) return st;
+  if (spc2_status st = upload_aos(s, k_next, 3, s.k[0], 3)) return st;
+  build(s, s.k[1], k_coef * dt);
+  StepArgs sa = step_args(s, dt, 0);
+  launch_pass1(s, EPI_K, s.k[0][0], s.k[0][1], s.k[0][2], sa, nullptr, nullptr, nullptr);
+  if (spc2_status st = check_launch("eval_rhs")) return st;
+  return download_aos(s, k_next, 3, s.k[0], 3);
+}
+
+spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *additional) {
+  if (check_handle(h) || !derivative || !additional) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  if (s.P.system != SPC2_CHIMERA)
+    return fail(SPC2_ERR_UNSUPPORTED, "the vortex-arrays Ito form (second pass) is not provided by this build");
+  if (spc2_status st = upload_aos(s, derivative, 3, s.der, 3)) return st;
+  if (spc2_status st = upload_aos(s, additional, 2, s.add, 2)) return st;
+  build(s, nullptr, 0.0);
+  StepArgs sa = step_args(s, 0.0, 0);
+  launch_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
+  launch_pass2(s, EPI_ITO2, s.add[0], s.add[1], sa);
+  if (spc2_status st = check_launch("eval_rhs_ito")) return st;
+  if (spc2_status st = download_aos(s, derivative, 3, s.der, 3)) return st;
+  return download_aos(s, additional, 2, s.add, 2);
+}
+
+spc2_status spc2_step(spc2_sim_t h, int32_t method, double dt, int64_t step0, int64_t nsteps, const double *noise) {
+  if (check_handle(h)) return SPC2_ERR_INVALID;
+  Sim &s = h->s;
+  if (method < SPC2_SE || method > SPC2_RK4) return fail(SPC2_ERR_INVALID, "unknown method");
+  if (!(dt > 0.0) || nsteps < 0 || step0 < 0) return fail(SPC2_ERR_INVALID, "dt must be > 0, steps >= 0");
+  if (method == SPC2_SRK && s.P.system != SPC2_CHIMERA)
+    return fail(SPC2_ERR_UNSUPPORTED, "SRK for the vortex-arrays system needs its Ito second pass (not provided)");
+  const int npp = method == SPC2_SRK ? 2 : 1;
+  if (noise && method != SPC2_RK4) {
+    size_t need = (size_t)nsteps * npp * (size_t)s.n;
+    if (need > s.noise_cap) {
+      if (s.noise) (void)hipFree(s.noise);
+      s.noise = nullptr;
+      s.noise_cap = 0;
+      HIP_TRY(dalloc(&s.noise, need));
+      s.noise_cap = need;
+    }
+    HIP_TRY(hipMemcpyAsync(s.noise, noise, sizeof(double) * need, hipMemcpyHostToDevice, s.stream));
+  }
+  for (int64_t k = 0; k < nsteps; ++k) {
+    StepArgs sa = step_args(s, dt, (uint64_t)(step0 + k));
+    if (noise && method != SPC2_RK4) sa.noise = s.noise + (size_t)k * npp * (size_t)s.n;
+    if (method == SPC2_SE) {
+      build(s, nullptr, 0.0);
+      launch_pass1(s, EPI_SE, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
+    } else if (method == SPC2_SRK) {
+      build(s, nullptr, 0.0);
+      sa.zero_in = 1;  // derivative zero-filled each step (StochasticRungeKuttaStepperPtr.hpp:43-46)
+      launch_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
+      launch_pass2(s, EPI_SRK, nullptr, nullptr, sa);
+    } else {
+      // RungeKutta4StepperPtr.cpp:25-43: k1 = f(s, k1, 0), k2 = f(s, k1, 1/2),
+      // k3 = f(s, k2, 1/2), k4 = f(s, k3, 1), stage = s + (c * dt) * k
+      double *const *k1 = s.k[0], *const *k2 = s.k[1], *const *k3 = s.k[2], *const *k4 = s.k[3];
+      build(s, k1, 0.0 * dt);
+      launch_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);
+      build(s, k1, 0.5 * dt);
+      launch_pass1(s, EPI_K, k2[0], k2[1], k2[2], sa, nullptr, nullptr, nullptr);
+      build(s, k2, 0.5 * dt);
+      launch_pass1(s, EPI_K, k3[0], k3[1], k3[2], sa, nullptr, nullptr, nullptr);
+      build(s, k3, 1.0 * dt);
+      launch_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
+    }
+    if (spc2_status st = check_launch("step")) return st;
+  }
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+spc2_status spc2_noise(spc2_sim_t h, int32_t method, int64_t step, double *out) {
+  if (check_handle(h) || !out) return fail(SPC2_ERR_INVALID, "null argument");
+  if (step < 0) return fail(SPC2_ERR_INVALID, "step must be >= 0");
+  Sim &s = h->s;
+  int npp = method == SPC2_SRK ? 2 : 1;
+  double *buf = nullptr;
+  HIP_TRY(dalloc(&buf, (size_t)npp * s.n));
+  launch_noise(s, npp, (uint64_t)step, buf);
+  spc2_status st = check_launch("noise");
+  hipError_t e = st ? hipSuccess
+                    : hipMemcpyAsync(out, buf, sizeof(double) * (size_t)npp * s.n, hipMemcpyDeviceToHost, s.stream);
+  if (e == hipSuccess && !st) e = hipStreamSynchronize(s.stream);
+  (void)hipFree(buf);
+  if (st) return st;
+  if (e != hipSuccess) return fail(SPC2_ERR_HIP, std::string("noise copy: ") + hipGetErrorString(e));
+  return SPC2_OK;
+}
+
+spc2_status spc2_order_parameter(spc2_sim_t h, double *R, double *psi) {
+  if (check_handle(h) || !R || !psi) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  double *out2 = s.red + 2 * 1024;
+  launch_order_param(s, out2);
+  if (spc2_status st = check_launch("order_parameter")) return st;
+  double cs[2];
+  HIP_TRY(hipMemcpyAsync(cs, out2, sizeof(cs), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  double cr = cs[0] / (double)s.n, ci = cs[1] / (double)s.n;
+  *R = std::hypot(cr, ci);
+  *psi = std::atan2(ci, cr);
+  return SPC2_OK;
+}
+
+spc2_status spc2_get_cell_list(spc2_sim_t h, int32_t *cell_of, int32_t *cell_start, int32_t *sorted) {
+  if (check_handle(h) || !cell_of || !cell_start || !sorted) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  if (!s.linked) return fail(SPC2_ERR_INVALID, "no cell list on the all-pairs path");
+  build(s, nullptr, 0.0);
+  if (spc2_status st = check_launch("cell list")) return st;
+  HIP_TRY(hipMemcpyAsync(cell_of, s.cell_of, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipMemcpyAsync(cell_start, s.start, sizeof(int) * ((size_t)s.nc + 1), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipMemcpyAsync(sorted, s.sorted, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+spc2_status spc2_num_cells(spc2_sim_t h, int32_t *nx, int32_t *ny) {
+  if (check_handle(h) || !nx || !ny) return fail(SPC2_ERR_INVALID, "null argument");
+  *nx = h->s.nx;
+  *ny = h->s.ny;
+  return SPC2_OK;
+}
+
+spc2_status spc2_set_counting(spc2_sim_t h, int32_t enable) {
+  if (check_handle(h)) return SPC2_ERR_INVALID;
+  Sim &s = h->s;
+  s.counting = enable != 0;
+  HIP_TRY(hipMemsetAsync(s.counts, 0, sizeof(unsigned long long) * 3, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+spc2_status spc2_get_counts(spc2_sim_t h, int64_t *counts3) {
+  if (check_handle(h) || !counts3) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  unsigned long long c[3];
+  HIP_TRY(hipMemcpyAsync(c, s.counts, sizeof(c), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  for (int i = 0; i < 3; ++i) counts3[i] = (int64_t)c[i];
+  return SPC2_OK;
+}
+
+spc2_status spc2_set_profiling(spc2_sim_t h, int32_t enable) {
+  if (check_handle(h)) return SPC2_ERR_INVALID;
+  Sim &s = h->s;
+  if (!s.ev0) {
+    HIP_TRY(hipEventCreate(&s.ev0));
+    HIP_TRY(hipEventCreate(&s.ev1));
+  }
+  s.profiling = enable != 0;
+  s.prof_ms = 0.0;
+  s.prof_launches = 0;
+  return SPC2_OK;
+}
+
+spc2_status spc2_get_profile(spc2_sim_t h, double *pair_kernel_ms, int64_t *launches) {
+  if (check_handle(h) || !pair_kernel_ms || !launches) return fail(SPC2_ERR_INVALID, "null argument");
+  *pair_kernel_ms = h->s.prof_ms;
+  *launches = h->s.prof_launches;
+  return SPC2_OK;
+}
+
+spc2_status spc2_synchronize(spc2_sim_t h) {
+  if (check_handle(h)) return SPC2_ERR_INVALID;
+  HIP_TRY(hipStreamSynchronize(h->s.stream));
+  return SPC2_OK;
+}
+
+}  // extern "C"

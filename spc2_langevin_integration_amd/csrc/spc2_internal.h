@@ -1,0 +1,109 @@
+// Host-side internals shared by the spc2 HIP translation units: the device
+// buffers of one simulation and the launchers each kernel file provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/spc2_hip.h"
+
+namespace spc2 {
+
+enum Prec : int { P64 = 0, P32K = 1, P64F = 2 };
+
+// What a force kernel does with the per-particle result (its "epilogue").
+enum Epi : int {
+  EPI_K = 0,          // k-form: write k_next (phi only when a partner was found)
+  EPI_ITO1 = 1,       // Ito pass 1: write derivative; keep a_phi per slot for pass 2
+  EPI_SE = 2,         // pass 1 + Stochastic Euler update + wrap
+  EPI_RK4_FINAL = 3,  // 4th RK4 stage + combine + wrap
+  EPI_ITO2 = 4,       // Ito pass 2: write additional
+  EPI_SRK = 5,        // pass 2 + stochastic Runge-Kutta update + wrap
+};
+
+// Scalars every force kernel needs (one copy per launch, by value).
+struct KParams {
+  int n, nx, ny, linked;
+  double Lx, Ly, rLx, rLy, hx, hy;
+  float hxf, hyf;
+  double rho, rho2;
+  float rho2f;
+  double v0, sigma, alpha, D;
+  // vortex
+  double mu_plus, mu_minus, xi_a, xi_r, kappa;
+  double outer2;  // max(rho, xi_r)^2: widest test radius
+  float xa2f, xr2f, outer2f, xaf, rhof, mu_pf, mu_p_over_xa2f, c4f, kappaf;
+  // FP32 cutoff band: |r^2 - cutoff^2| <= band * cutoff^2 is re-decided in FP64
+  float band_rho2f, band_xr2f;
+};
+
+// Tile arrays in cell-sorted slot order, written by the rank kernel.
+struct Tiles {
+  double *x, *y, *p, *c, *s;  // FP64 forms: wrapped stage position, stage phase, cos/sin
+  float4 *f4;                 // FP32 form: cell-local x, y and cos, sin of the phase
+  int *cell;                  // cell of each slot
+};
+
+struct StepArgs {
+  double dt, sqdt, sq2D, h1, sqrt3, D;  // h1 = 0.5 * dt * sqrt(dt)
+  uint64_t seed, step;
+  const double *noise;  // optional materialised noise for this step (by particle)
+  const double *omega;  // natural frequencies (chimera) or null
+  int zero_in;          // 1: the phi output starts from 0 (zero-filled derivative), else in/out
+};
+
+struct Sim {
+  spc2_params P;
+  int prec = P64;
+  int n = 0, nx = 0, ny = 0, nc = 0;
+  bool linked = true;
+  KParams kp{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // master state, FP64 SoA, original particle order
+  double *sx = nullptr, *sy = nullptr, *sp = nullptr, *omega = nullptr;
+  // k buffers (RK4 k_1..k_4, also the k-form in/out), SoA by particle
+  double *k[4][3] = {};
+  // Ito outputs
+  double *der[3] = {}, *add[2] = {};
+  // stage positions (wrapped for binning when linked), by particle
+  double *gx = nullptr, *gy = nullptr, *gp = nullptr;
+  // cell list
+  int *cell_of = nullptr, *count = nullptr, *local = nullptr, *start = nullptr, *tmp = nullptr,
+      *sorted = nullptr;
+  Tiles t{};
+  // pass-1 results kept per slot for pass 2
+  double *aphi_slot = nullptr, *n_slot = nullptr;
+  float *aphi_slot_f = nullptr;
+  // materialised noise (optional)
+  double *noise = nullptr;
+  size_t noise_cap = 0;
+  // counts: candidates, interactions, cohesion pairs
+  unsigned long long *counts = nullptr;
+  bool counting = false;
+  // reduction scratch
+  double *red = nullptr;
+  // profiling: events around the pair kernels
+  bool profiling = false;
+  double prof_ms = 0.0;
+  long long prof_launches = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// ---- launchers (spc2_kernels.hip) -------------------------------------------
+void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
+void launch_scan(Sim &s);
+void launch_place(Sim &s);
+void launch_rank_tiles(Sim &s);
+void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
+                  double *const k2[3], double *const k3[3]);
+void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
+void launch_order_param(Sim &s, double *out2);
+void launch_init_state(Sim &s, double omega_std);
+void launch_noise(Sim &s, int per_particle, uint64_t step, double *out);
+void launch_fill(Sim &s, double *p, double v, size_t n);
+
+}  // namespace spc2
