@@ -17,7 +17,6 @@ using namespace spc2;
 
 struct spc2_sim {
   Sim s;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
 };
 
 namespace {
@@ -61,7 +60,7 @@ void free_all(Sim &s) {
   f(s.cell_of); f(s.count); f(s.local); f(s.start); f(s.tmp); f(s.sorted);
   f(s.t.x); f(s.t.y); f(s.t.p); f(s.t.c); f(s.t.s); f(s.t.f4); f(s.t.cell);
   f(s.aphi_slot); f(s.n_slot); f(s.aphi_slot_f);
-  f(s.noise); f(s.counts); f(s.red);
+  f(s.noise); f(s.counts); f(s.red); f(s.bsum);
 }
 
 void fill_kparams(Sim &s) {
@@ -129,7 +128,7 @@ spc2_status alloc_all(Sim &s) {
     }
   }
   A(s.aphi_slot, n); A(s.n_slot, n); A(s.aphi_slot_f, n);
-  A(s.counts, 3); A(s.red, 2 * 1024 + 8);
+  A(s.counts, 3); A(s.red, 2 * 1024 + 8); A(s.bsum, (size_t)scan_blocks(std::max(s.nc, 1)) + 1);
 #undef A
   HIP_TRY(hipMemset(s.count, 0, sizeof(int) * nc));
   HIP_TRY(hipMemset(s.omega, 0, sizeof(double) * n));
@@ -262,6 +261,7 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
     s.nc = s.nx * s.ny;
   }
   fill_kparams(s);
+  configure_segments(s);
   hipError_t e = hipGetDevice(&s.device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -285,16 +285,11 @@ spc2_status spc2_destroy(spc2_sim_t h) {
   if (!h) return SPC2_OK;
   Sim &s = h->s;
   (void)hipStreamSynchronize(s.stream);
-  for (auto &pr : h->ev_pending) {
+  s.prof_collect();
+  for (auto &pr : s.prof_free) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
   }
-  for (auto &pr : h->ev_free) {
-    (void)hipEventDestroy(pr.first);
-    (void)hipEventDestroy(pr.second);
-  }
-  if (s.ev0) (void)hipEventDestroy(s.ev0);
-  if (s.ev1) (void)hipEventDestroy(s.ev1);
   free_all(s);
   (void)hipStreamDestroy(s.stream);
   delete h;
@@ -486,10 +481,7 @@ spc2_status spc2_get_counts(spc2_sim_t h, int64_t *counts3) {
 spc2_status spc2_set_profiling(spc2_sim_t h, int32_t enable) {
   if (check_handle(h)) return SPC2_ERR_INVALID;
   Sim &s = h->s;
-  if (!s.ev0) {
-    HIP_TRY(hipEventCreate(&s.ev0));
-    HIP_TRY(hipEventCreate(&s.ev1));
-  }
+  s.prof_collect();
   s.profiling = enable != 0;
   s.prof_ms = 0.0;
   s.prof_launches = 0;
@@ -498,6 +490,7 @@ spc2_status spc2_set_profiling(spc2_sim_t h, int32_t enable) {
 
 spc2_status spc2_get_profile(spc2_sim_t h, double *pair_kernel_ms, int64_t *launches) {
   if (check_handle(h) || !pair_kernel_ms || !launches) return fail(SPC2_ERR_INVALID, "null argument");
+  h->s.prof_collect();
   *pair_kernel_ms = h->s.prof_ms;
   *launches = h->s.prof_launches;
   return SPC2_OK;

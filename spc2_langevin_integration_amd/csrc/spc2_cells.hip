@@ -1,0 +1,207 @@
+// Linked-cell build on gfx950: stage + binning + histogram, a three-phase
+// wavefront scan of the histogram, and a stable counting sort that writes the
+// cell-sorted tiles the pair kernels read.  Compiled with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "spc2_device.h"
+#include "spc2_internal.h"
+
+namespace spc2 {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kScanItems = 4;                     // counts per thread
+constexpr int kScanTile = kBlock * kScanItems;    // counts per workgroup
+
+inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// K1: stage s + (k_coef * dt) * k (ChimeraSystem.cpp:274-276, evaluated as
+// (k_coef * dt) * k_prev like the reference); on the linked-cell path the
+// position is wrapped if outside the box and binned (ChimeraSystem.cpp:291-309).
+// `local` is the arrival rank inside the cell, used only to place the particle
+// before the stable re-ranking.
+__global__ __launch_bounds__(kBlock) void k_stage_bin(int n, int linked, Box b, int nx, int ny, const double *sx,
+                                                      const double *sy, const double *sp, const double *kx,
+                                                      const double *ky, const double *kp, double kcdt, double *gx,
+                                                      double *gy, double *gp, int *cell_of, int *count, int *local) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = sx[i], y = sy[i], p = sp[i];
+  if (kx) {
+    x = x + kcdt * kx[i];
+    y = y + kcdt * ky[i];
+    p = p + kcdt * kp[i];
+  }
+  if (linked) {
+    cond_wrap(b, x, y);
+    int cx = cell_coord(x, b.Lx, nx);
+    int cy = cell_coord(y, b.Ly, ny);
+    int c = cy * nx + cx;
+    cell_of[i] = c;
+    local[i] = atomicAdd(&count[c], 1);
+  }
+  gx[i] = x;
+  gy[i] = y;
+  gp[i] = p;
+}
+
+// Inclusive scan of one int per thread across a 256-thread workgroup:
+// shuffles inside each 64-lane wavefront, wavefront totals through LDS.
+__device__ __forceinline__ int block_inclusive_scan(int v, int *wtot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) wtot[w] = v;
+  __syncthreads();
+  int add = 0;
+#pragma unroll
+  for (int q = 0; q < kBlock / 64; ++q) add += (q < w) ? wtot[q] : 0;
+  __syncthreads();
+  return v + add;
+}
+
+// K2a: per-workgroup totals of kScanTile consecutive counts.
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int *count, int nc, int *bsum) {
+  __shared__ int wtot[kBlock / 64];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) s += (base + q < nc) ? count[base + q] : 0;
+  int incl = block_inclusive_scan(s, wtot);
+  if (threadIdx.x == kBlock - 1) bsum[blockIdx.x] = incl;
+}
+
+// K2b: exclusive scan of the workgroup totals (one workgroup, carried tiles);
+// writes the grand total to start[nc].
+__global__ __launch_bounds__(kBlock) void k_scan_bsum(int *bsum, int nb, int *start, int nc) {
+  __shared__ int wtot[kBlock / 64];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += kBlock) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    int incl = block_inclusive_scan(v, wtot);
+    int c0 = carry;
+    if (i < nb) bsum[i] = c0 + incl - v;
+    __syncthreads();
+    if (threadIdx.x == kBlock - 1) carry = c0 + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[nc] = carry;
+}
+
+// K2c: exclusive scan inside each tile plus the tile offset -> start[];
+// clears the histogram for the next build.
+__global__ __launch_bounds__(kBlock) void k_scan_apply(int *count, int nc, const int *bsum, int *start) {
+  __shared__ int wtot[kBlock / 64];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    v[q] = (base + q < nc) ? count[base + q] : 0;
+    s += v[q];
+  }
+  int run = block_inclusive_scan(s, wtot) - s + bsum[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q < nc) {
+      start[base + q] = run;
+      count[base + q] = 0;
+    }
+    run += v[q];
+  }
+}
+
+// K3a: place each particle at cell start + arrival rank.
+__global__ __launch_bounds__(kBlock) void k_place(int n, const int *cell_of, const int *local, const int *start,
+                                                  int *tmp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  tmp[start[cell_of[i]] + local[i]] = i;
+}
+
+// K3b: stable counting sort.  The final position of particle i inside its cell
+// is the number of cell members with a smaller index, which reproduces the
+// reference's ascending push_back order (ChimeraSystem.cpp:335-338).  The same
+// thread gathers i's stage data into the cell-sorted tile arrays.
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_rank_tiles(int n, int nx, double hx, double hy, const int *tmp,
+                                                       const int *cell_of, const int *start, const double *gx,
+                                                       const double *gy, const double *gp, int *sorted, Tiles t) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int i = tmp[k];
+  int c = cell_of[i];
+  int lo = start[c], hi = start[c + 1];
+  int r = 0;
+  for (int m = lo; m < hi; ++m) r += tmp[m] < i;
+  int dst = lo + r;
+  sorted[dst] = i;
+  t.cell[dst] = c;
+  double x = gx[i], y = gy[i], p = gp[i];
+  if (PREC == P32K) {
+    int cx = c % nx, cy = c / nx;
+    double sn, cs;
+    sincos(p, &sn, &cs);
+    t.f4[dst] = make_float4((float)(x - (double)cx * hx), (float)(y - (double)cy * hy), (float)cs, (float)sn);
+    t.x[dst] = x;
+    t.y[dst] = y;
+  } else {
+    t.x[dst] = x;
+    t.y[dst] = y;
+    t.p[dst] = p;
+    if (PREC == P64F) {
+      double sn, cs;
+      sincos(p, &sn, &cs);
+      t.c[dst] = cs;
+      t.s[dst] = sn;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt) {
+  Box b{s.kp.Lx, s.kp.Ly, s.kp.rLx, s.kp.rLy};
+  hipLaunchKernelGGL(k_stage_bin, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, (int)s.linked, b, s.nx,
+                     s.ny, s.sx, s.sy, s.sp, kx, ky, kp, kcdt, s.gx, s.gy, s.gp, s.cell_of, s.count, s.local);
+}
+
+int scan_blocks(int nc) { return (nc + kScanTile - 1) / kScanTile; }
+
+void launch_scan(Sim &s) {
+  const int nb = scan_blocks(s.nc);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s.stream, s.count, s.nc, s.bsum);
+  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kBlock), 0, s.stream, s.bsum, nb, s.start, s.nc);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s.stream, s.count, s.nc, s.bsum, s.start);
+}
+
+void launch_place(Sim &s) {
+  hipLaunchKernelGGL(k_place, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.cell_of, s.local, s.start,
+                     s.tmp);
+}
+
+void launch_rank_tiles(Sim &s) {
+  dim3 g(blocks_for(s.n)), b(kBlock);
+  switch (s.prec) {
+    case P64:
+      hipLaunchKernelGGL(k_rank_tiles<P64>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.cell_of, s.start,
+                         s.gx, s.gy, s.gp, s.sorted, s.t);
+      break;
+    case P64F:
+      hipLaunchKernelGGL(k_rank_tiles<P64F>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.cell_of,
+                         s.start, s.gx, s.gy, s.gp, s.sorted, s.t);
+      break;
+    default:
+      hipLaunchKernelGGL(k_rank_tiles<P32K>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.cell_of,
+                         s.start, s.gx, s.gy, s.gp, s.sorted, s.t);
+  }
+}
+
+}  // namespace spc2

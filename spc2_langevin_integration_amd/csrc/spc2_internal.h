@@ -6,6 +6,8 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/spc2_hip.h"
 
@@ -37,6 +39,9 @@ struct KParams {
   float xa2f, xr2f, outer2f, xaf, rhof, mu_pf, mu_p_over_xa2f, c4f, kappaf;
   // FP32 cutoff band: |r^2 - cutoff^2| <= band * cutoff^2 is re-decided in FP64
   float band_rho2f, band_xr2f;
+  // pair-kernel decomposition: row segments of seg_w cells, segs per row,
+  // LDS capacity in tile elements per workgroup
+  int seg_w, segs, lds_cap;
 };
 
 // Tile arrays in cell-sorted slot order, written by the rank kernel.
@@ -84,23 +89,52 @@ struct Sim {
   // counts: candidates, interactions, cohesion pairs
   unsigned long long *counts = nullptr;
   bool counting = false;
-  // reduction scratch
+  // scan workgroup totals, reduction scratch
+  int *bsum = nullptr;
   double *red = nullptr;
-  // profiling: events around the pair kernels
+  // profiling: HIP events around the pair kernels, read lazily
   bool profiling = false;
   double prof_ms = 0.0;
   long long prof_launches = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending, prof_free;
+  void prof_take(hipEvent_t &b, hipEvent_t &e) {
+    if (!prof_free.empty()) {
+      b = prof_free.back().first;
+      e = prof_free.back().second;
+      prof_free.pop_back();
+      return;
+    }
+    (void)hipEventCreate(&b);
+    (void)hipEventCreate(&e);
+  }
+  // folds the finished event pairs into prof_ms (waits for the last one)
+  void prof_collect() {
+    if (!prof_pending.empty()) (void)hipEventSynchronize(prof_pending.back().second);
+    for (auto &pr : prof_pending) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        prof_ms += ms;
+        prof_launches += 1;
+      }
+      prof_free.push_back(pr);
+    }
+    prof_pending.clear();
+  }
 };
 
-// ---- launchers (spc2_kernels.hip) -------------------------------------------
+// ---- launchers -----------------------------------------------------------------
+// spc2_cells.hip
 void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
+int scan_blocks(int nc);
 void launch_scan(Sim &s);
 void launch_place(Sim &s);
 void launch_rank_tiles(Sim &s);
+// spc2_pairs.hip
+void configure_segments(Sim &s);
 void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
                   double *const k2[3], double *const k3[3]);
 void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
+// spc2_misc.hip
 void launch_order_param(Sim &s, double *out2);
 void launch_init_state(Sim &s, double omega_std);
 void launch_noise(Sim &s, int per_particle, uint64_t step, double *out);

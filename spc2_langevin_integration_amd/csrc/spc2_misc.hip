@@ -1,0 +1,130 @@
+// Order-parameter reduction, Philox-keyed initial state, materialised noise
+// and a fill kernel (graph-safe replacement for hipMemsetAsync).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "spc2_device.h"
+#include "spc2_internal.h"
+
+namespace spc2 {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 1024;
+constexpr double kTwoPi = 6.283185307179586;  // 2.0 * M_PI
+
+inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// Kuramoto order parameter, two-level FP64 reduction (deterministic: fixed
+// partition and fixed tree); replaces the float sum of
+// BinaryObserverPtr::SaveSummaryStatistics (BinaryObserverPtr.cpp:237-247).
+__global__ __launch_bounds__(kBlock) void k_order_partial(int n, const double *sp, double *red) {
+  __shared__ double sc[kBlock], ss[kBlock];
+  double c = 0.0, s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double sn, cs;
+    sincos(sp[i], &sn, &cs);
+    c += cs;
+    s += sn;
+  }
+  sc[threadIdx.x] = c;
+  ss[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) {
+      sc[threadIdx.x] += sc[threadIdx.x + d];
+      ss[threadIdx.x] += ss[threadIdx.x + d];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    red[2 * blockIdx.x] = sc[0];
+    red[2 * blockIdx.x + 1] = ss[0];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_order_final(int nb, const double *red, double *out2) {
+  __shared__ double sc[kBlock], ss[kBlock];
+  double c = 0.0, s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    c += red[2 * b];
+    s += red[2 * b + 1];
+  }
+  sc[threadIdx.x] = c;
+  ss[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) {
+      sc[threadIdx.x] += sc[threadIdx.x + d];
+      ss[threadIdx.x] += ss[threadIdx.x + d];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out2[0] = sc[0];
+    out2[1] = ss[0];
+  }
+}
+
+// x, y ~ U[0, L), phi ~ U[0, 2 pi), omega ~ N(0, omega_std) from Philox
+// streams 1, 2, 3 at step 0 (replaces SimulationEnginePtr.cpp:31-50).
+__global__ __launch_bounds__(kBlock) void k_init_state(int n, double Lx, double Ly, uint64_t seed, double omega_std,
+                                                       double *sx, double *sy, double *sp, double *omega) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ux, uy, up, unused;
+  uniform_pair(seed, 0, kStreamInitPos, (uint32_t)i, ux, uy);
+  uniform_pair(seed, 0, kStreamInitPhase, (uint32_t)i, up, unused);
+  sx[i] = ux * Lx;
+  sy[i] = uy * Ly;
+  sp[i] = up * kTwoPi;
+  double z0, z1;
+  normal_pair(seed, 0, kStreamOmega, (uint32_t)i, z0, z1);
+  omega[i] = z0 * omega_std;
+}
+
+__global__ __launch_bounds__(kBlock) void k_noise(int n, int per_particle, uint64_t seed, uint64_t step,
+                                                  double *out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double z0, z1;
+  normal_pair(seed, step, kStreamStep, (uint32_t)i, z0, z1);
+  if (per_particle == 1) {
+    out[i] = z0;
+  } else {
+    out[2 * i] = z0;
+    out[2 * i + 1] = z1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double *p, double v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+void launch_order_param(Sim &s, double *out2) {
+  int nb = std::min(kRedBlocks, std::max(1, blocks_for(s.n)));
+  hipLaunchKernelGGL(k_order_partial, dim3(nb), dim3(kBlock), 0, s.stream, s.n, s.sp, s.red);
+  hipLaunchKernelGGL(k_order_final, dim3(1), dim3(kBlock), 0, s.stream, nb, s.red, out2);
+}
+
+void launch_init_state(Sim &s, double omega_std) {
+  hipLaunchKernelGGL(k_init_state, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.P.Lx, s.P.Ly, s.P.seed,
+                     omega_std, s.sx, s.sy, s.sp, s.omega);
+}
+
+void launch_noise(Sim &s, int per_particle, uint64_t step, double *out) {
+  hipLaunchKernelGGL(k_noise, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, per_particle, s.P.seed, step,
+                     out);
+}
+
+void launch_fill(Sim &s, double *p, double v, size_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s.stream, p, v, n);
+}
+
+}  // namespace spc2

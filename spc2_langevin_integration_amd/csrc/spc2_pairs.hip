@@ -1,0 +1,1282 @@
+// Periodic 3x3-stencil pair sums on gfx950 (chimera and vortex arrays), the
+// fused stepper epilogues, and the all-pairs path.  Compiled with
+// -ffp-contract=off (see spc2_device.h).
+//
+// Work decomposition (linked cells): one 256-lane workgroup per row segment of
+// W cells.  It stages the 3 x (W + 2) halo cells' tiles in LDS with coalesced
+// reads (cells of a row segment are contiguous in the cell-sorted arrays), then
+// each lane takes one centre particle.  In LDS the three stencil cells of one
+// stencil row are adjacent, so a particle visits its 9 cells as 3 contiguous
+// ranges, in the reference order (cells (-1,dy), (0,dy), (1,dy), ascending
+// particle index inside a cell).  A segment whose halo does not fit the LDS
+// capacity falls back to reading the same tiles from global memory with the
+// same arithmetic and order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+
+#include "spc2_device.h"
+#include "spc2_internal.h"
+
+namespace spc2 {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxSegW = 64;
+constexpr int kMaxSlots = 3 * (kMaxSegW + 2);
+
+inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// ---- tile sources -------------------------------------------------------------
+// Src32 reads the FP32 tile (cell-local x, y and cos, sin of the phase); `slot`
+// maps an LDS index to its global slot (null: the index is the slot).
+struct Src32 {
+  const float4 *f4;
+  const int *slot;
+  const float *aj;
+  __device__ __forceinline__ int gslot(int m) const { return slot ? slot[m] : m; }
+};
+struct Src64 {
+  const double *x, *y, *p, *c, *s, *aj;
+};
+
+template <int PREC>
+struct SrcOf {
+  using type = Src64;
+};
+template <>
+struct SrcOf<P32K> {
+  using type = Src32;
+};
+
+// AlignmentStrength, VortexArraysSystemPtr.cpp:737-749
+__device__ __forceinline__ double vortex_w(const KParams &P, double r) {
+  if (r < P.xi_a) return P.mu_plus * (1.0 - r * r / (P.xi_a * P.xi_a));
+  if (r < P.rho) return -P.mu_minus * 4.0 * (r - P.xi_a) * (P.rho - r) / (P.rho - P.xi_a) / (P.rho - P.xi_a);
+  return 0.0;
+}
+
+// FP64 distance^2 of slots k and m with the reference min-image: decides the
+// rare FP32 pairs that fall inside a cutoff band.
+__device__ __forceinline__ double exact_r2(const double *tx, const double *ty, int k, int m, double Lx, double rLx,
+                                           double Ly, double rLy) {
+  double dx = min_image(tx[m] - tx[k], Lx, rLx);
+  double dy = min_image(ty[m] - ty[k], Ly, rLy);
+  return dx * dx + dy * dy;
+}
+
+// Per-particle result of pass 1.
+struct Pass1Out {
+  double phase;    // FP64 stage phase of i
+  double aphi;     // phi derivative when `written`
+  int written;     // 0: phi entry left as is (chimera n_i = 0; vortex n0+n1 = 0 and n2 = 0)
+  int vortex_add;  // vortex: 1 if only the cohesion term applies (added to the old value)
+  double coh;      // vortex cohesion / n2 for vortex_add
+  int n;           // partners within rho
+  int n2;          // cohesion partners
+};
+
+// FP32 x offset of the candidate's cell column relative to i's: m < b1 is the
+// left stencil column, b1 <= m < b2 the middle one, m >= b2 the right one.
+// Written as -x_i + sel * h with sel in {-1, 0, 1} (exact), so that the
+// compiler keeps every member in registers (a select between members can be
+// turned into a select between their addresses, which pushes the whole
+// accumulator to scratch memory).
+struct ColOffsets {
+  float oM, hxf, oy;
+  int b1, b2;
+  __device__ __forceinline__ void set(float xl, float h) {
+    oM = -xl;
+    hxf = h;
+  }
+  __device__ __forceinline__ float ox(int m) const {
+    float sel = (float)((m >= b2) - (m < b1));
+    return __builtin_fmaf(sel, hxf, oM);
+  }
+};
+
+// =============================================================================
+// Chimera pass 1 (ChimeraSystem.cpp:391-424)
+// =============================================================================
+template <int PREC>
+struct ChimAcc;
+
+// FP64 reference form: A += sin(phi_j - phi_i - alpha) over |r_ij| <= rho
+template <>
+struct ChimAcc<P64> {
+  double xi, yi, pi, A;
+  int n;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = s.p[e];
+    A = 0.0;
+    n = 0;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 <= P.rho2) {
+      A += sin(s.p[m] - pi - P.alpha);
+      ++n;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double, Pass1Out &o) const {
+    o.phase = pi;
+    o.n = n;
+    o.n2 = 0;
+    o.written = n != 0;
+    o.vortex_add = 0;
+    o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
+  }
+};
+
+// FP64 factored: sum sin(phi_j - phi_i - alpha)
+//   = cos(phi_i + alpha) sum sin phi_j - sin(phi_i + alpha) sum cos phi_j
+template <>
+struct ChimAcc<P64F> {
+  double xi, yi, pi, S, C;
+  int n;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = t.p[k];
+    S = 0.0;
+    C = 0.0;
+    n = 0;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 <= P.rho2) {
+      S += s.s[m];
+      C += s.c[m];
+      ++n;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double, Pass1Out &o) const {
+    double sa, ca;
+    sincos(pi + P.alpha, &sa, &ca);
+    double A = ca * S - sa * C;
+    o.phase = pi;
+    o.n = n;
+    o.n2 = 0;
+    o.written = n != 0;
+    o.vortex_add = 0;
+    o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
+  }
+};
+
+// FP32 + Kahan over cell-local coordinates; cutoff band re-decided in FP64
+template <>
+struct ChimAcc<P32K> {
+  float xl, yl;
+  ColOffsets co;
+  Kahan S, C;
+  int n, k;
+  const double *tx, *ty;
+  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P) {
+    float4 v = s.f4[e];
+    xl = v.x;
+    yl = v.y;
+    co.set(xl, P.hxf);
+    S.init();
+    C.init();
+    n = 0;
+    k = kk;
+    tx = t.x;
+    ty = t.y;
+  }
+  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
+    co.oy = (float)dyo * P.hyf - yl;
+    co.b1 = b1;
+    co.b2 = b2;
+  }
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    float4 v = s.f4[m];
+    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+    float r2 = __builtin_fmaf(dx, dx, dy * dy);
+    bool in = r2 <= P.rho2f;
+    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
+      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    S.add(in ? v.w : 0.0f);
+    C.add(in ? v.z : 0.0f);
+    n += in;
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
+    double sa, ca;
+    sincos(phase + P.alpha, &sa, &ca);
+    double A = ca * S.value() - sa * C.value();
+    o.phase = phase;
+    o.n = n;
+    o.n2 = 0;
+    o.written = n != 0;
+    o.vortex_add = 0;
+    o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
+  }
+};
+
+// =============================================================================
+// Vortex arrays pass 1 (VortexArraysSystemPtr.cpp:459-504)
+// =============================================================================
+template <int PREC>
+struct VortAcc;
+
+template <>
+struct VortAcc<P64> {
+  double xi, yi, pi, A, B, Cc;
+  int n0, n1, n2;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = s.p[e];
+    A = B = Cc = 0.0;
+    n0 = n1 = n2 = 0;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 > P.outer2 * 1.000000000001) return;  // cannot pass the sqrt tests below
+    double dr = sqrt(r2);
+    double pj = s.p[m];
+    if (dr <= P.xi_a) {
+      A += vortex_w(P, dr) * sin(pj - pi);
+      ++n0;
+    } else if (dr <= P.rho) {
+      B += vortex_w(P, dr) * sin(pj - pi);
+      ++n1;
+    }
+    if (dr <= P.xi_r) {
+      Cc += -P.kappa * sin(atan2(dy, dx) - pi);
+      ++n2;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &, double, Pass1Out &o) const {
+    o.phase = pi;
+    o.n = n0 + n1;
+    o.n2 = n2;
+    double n01 = (double)n0 + (double)n1;
+    if (n0 + n1 > 0) {
+      o.aphi = (A + B) / n01;
+      if (n2 > 0) o.aphi += Cc / (double)n2;
+      o.written = 1;
+      o.vortex_add = 0;
+    } else {
+      o.aphi = 0.0;
+      o.written = 0;
+      o.vortex_add = n2 > 0;
+      o.coh = n2 > 0 ? Cc / (double)n2 : 0.0;
+    }
+  }
+};
+
+// factored: sum W sin(phi_j - phi_i) = cos phi_i sum W sin phi_j
+//   - sin phi_i sum W cos phi_j;  sin(atan2(dy, dx) - phi_i) = (dy cos phi_i - dx sin phi_i) / r
+template <>
+struct VortAcc<P64F> {
+  double xi, yi, pi, SW, CW, Tx, Ty;
+  int n01, n2;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = t.p[k];
+    SW = CW = Tx = Ty = 0.0;
+    n01 = n2 = 0;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 > P.outer2 * 1.000000000001) return;
+    double dr = sqrt(r2);
+    if (dr <= P.rho) {
+      double w = vortex_w(P, dr);
+      SW += w * s.s[m];
+      CW += w * s.c[m];
+      ++n01;
+    }
+    if (dr <= P.xi_r) {
+      if (dr > 0.0) {
+        Tx += dx / dr;
+        Ty += dy / dr;
+      } else {
+        Tx += 1.0;  // atan2(+0, +0) = 0
+      }
+      ++n2;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double, Pass1Out &o) const {
+    double sp, cp;
+    sincos(pi, &sp, &cp);
+    o.phase = pi;
+    o.n = n01;
+    o.n2 = n2;
+    double coh = n2 > 0 ? -P.kappa * (cp * Ty - sp * Tx) / (double)n2 : 0.0;
+    if (n01 > 0) {
+      o.aphi = (cp * SW - sp * CW) / (double)n01 + coh;
+      o.written = 1;
+      o.vortex_add = 0;
+    } else {
+      o.aphi = 0.0;
+      o.written = 0;
+      o.vortex_add = n2 > 0;
+      o.coh = coh;
+    }
+  }
+};
+
+template <>
+struct VortAcc<P32K> {
+  float xl, yl;
+  ColOffsets co;
+  Kahan SW, CW, Tx, Ty;
+  int n01, n2, k;
+  const double *tx, *ty;
+  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P) {
+    float4 v = s.f4[e];
+    xl = v.x;
+    yl = v.y;
+    co.set(xl, P.hxf);
+    SW.init();
+    CW.init();
+    Tx.init();
+    Ty.init();
+    n01 = n2 = 0;
+    k = kk;
+    tx = t.x;
+    ty = t.y;
+  }
+  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
+    co.oy = (float)dyo * P.hyf - yl;
+    co.b1 = b1;
+    co.b2 = b2;
+  }
+  // Branch-free except for the rare cases (cutoff band, cohesion range): every
+  // lane evaluates W and adds W sin, W cos masked to zero outside rho.
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    float4 v = s.f4[m];
+    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+    float r2 = __builtin_fmaf(dx, dx, dy * dy);
+    bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f;
+    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f || __builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f) {
+      // the reference tests sqrt(r^2) <= cutoff in FP64 (VortexArraysSystemPtr.cpp:473-488)
+      double d = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy));
+      inr = d <= P.rho;
+      inc = d <= P.xi_r;
+    }
+    float r = __builtin_amdgcn_sqrtf(r2);
+    float w_in = __builtin_fmaf(-P.mu_p_over_xa2f, r2, P.mu_pf);
+    float w_an = P.c4f * (r - P.xaf) * (P.rhof - r);
+    float w = r2 <= P.xa2f ? w_in : w_an;
+    w = inr ? w : 0.0f;
+    SW.add(w * v.w);
+    CW.add(w * v.z);
+    n01 += inr;
+    if (inc) {
+      if (r2 > 0.0f) {
+        float ir = __builtin_amdgcn_rsqf(r2);
+        Tx.add(dx * ir);
+        Ty.add(dy * ir);
+      } else {
+        Tx.add(1.0f);  // atan2(+0, +0) = 0
+      }
+      ++n2;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
+    double sp, cp;
+    sincos(phase, &sp, &cp);
+    o.phase = phase;
+    o.n = n01;
+    o.n2 = n2;
+    double coh = n2 > 0 ? -P.kappa * (cp * Ty.value() - sp * Tx.value()) / (double)n2 : 0.0;
+    if (n01 > 0) {
+      o.aphi = (cp * SW.value() - sp * CW.value()) / (double)n01 + coh;
+      o.written = 1;
+      o.vortex_add = 0;
+    } else {
+      o.aphi = 0.0;
+      o.written = 0;
+      o.vortex_add = n2 > 0;
+      o.coh = coh;
+    }
+  }
+};
+
+// =============================================================================
+// Chimera pass 2 (Ito form, ChimeraSystem.cpp:589-644): over the same partners,
+// cos(phi_j - phi_i - alpha) / n_i and (a_j - a_i) cos(...) / n_i (no sigma in
+// these terms, as in the reference)
+// =============================================================================
+struct Pass2Out {
+  double phase, a0, a1, ai;
+};
+
+template <int PREC>
+struct Chim2Acc;
+
+template <>
+struct Chim2Acc<P64> {
+  double xi, yi, pi, ai, n, a0, a1;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &, double a_i, double n_i, double in0,
+                       double in1) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = s.p[e];
+    ai = a_i;
+    n = n_i;
+    a0 = in0;
+    a1 = in1;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if ((r2 <= P.rho2) && (n != 0)) {
+      double cph = cos(s.p[m] - pi - P.alpha);
+      a0 += cph / n;
+      a1 += (s.aj[m] - ai) * cph / n;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
+    o.phase = pi;
+    o.ai = ai;
+    o.a0 = a0 * -sqrt(2.0 * P.D);
+    o.a1 = a1 + -2.0 * P.D * ai;
+  }
+};
+
+template <>
+struct Chim2Acc<P64F> {
+  double xi, yi, pi, ai, n, in0, in1, C, S, Pc, Qs;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &, double a_i, double n_i,
+                       double i0, double i1) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = t.p[k];
+    ai = a_i;
+    n = n_i;
+    in0 = i0;
+    in1 = i1;
+    C = S = Pc = Qs = 0.0;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 <= P.rho2) {
+      double aj = s.aj[m];
+      C += s.c[m];
+      S += s.s[m];
+      Pc += aj * s.c[m];
+      Qs += aj * s.s[m];
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
+    double sa, ca;
+    sincos(pi + P.alpha, &sa, &ca);
+    double a0 = in0, a1 = in1;
+    if (n != 0) {
+      double sc = ca * C + sa * S;
+      a0 += sc / n;
+      a1 += (ca * Pc + sa * Qs - ai * sc) / n;
+    }
+    o.phase = pi;
+    o.ai = ai;
+    o.a0 = a0 * -sqrt(2.0 * P.D);
+    o.a1 = a1 + -2.0 * P.D * ai;
+  }
+};
+
+template <>
+struct Chim2Acc<P32K> {
+  float xl, yl;
+  ColOffsets co;
+  double ai, n, in0, in1;
+  Kahan C, S, Pc, Qs;
+  int k;
+  const double *tx, *ty;
+  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P, double a_i, double n_i,
+                       double i0, double i1) {
+    float4 v = s.f4[e];
+    xl = v.x;
+    yl = v.y;
+    co.set(xl, P.hxf);
+    ai = a_i;
+    n = n_i;
+    in0 = i0;
+    in1 = i1;
+    C.init();
+    S.init();
+    Pc.init();
+    Qs.init();
+    k = kk;
+    tx = t.x;
+    ty = t.y;
+  }
+  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
+    co.oy = (float)dyo * P.hyf - yl;
+    co.b1 = b1;
+    co.b2 = b2;
+  }
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    float4 v = s.f4[m];
+    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+    float r2 = __builtin_fmaf(dx, dx, dy * dy);
+    bool in = r2 <= P.rho2f;
+    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
+      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    float aj = in ? s.aj[m] : 0.0f;
+    float cz = in ? v.z : 0.0f, sw = in ? v.w : 0.0f;
+    C.add(cz);
+    S.add(sw);
+    Pc.add(aj * cz);
+    Qs.add(aj * sw);
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass2Out &o) const {
+    double sa, ca;
+    sincos(phase + P.alpha, &sa, &ca);
+    double a0 = in0, a1 = in1;
+    if (n != 0) {
+      double sc = ca * C.value() + sa * S.value();
+      a0 += sc / n;
+      a1 += (ca * Pc.value() + sa * Qs.value() - ai * sc) / n;
+    }
+    o.phase = phase;
+    o.ai = ai;
+    o.a0 = a0 * -sqrt(2.0 * P.D);
+    o.a1 = a1 + -2.0 * P.D * ai;
+  }
+};
+
+// =============================================================================
+// Epilogues
+// =============================================================================
+struct EpiCtx {
+  double *ox, *oy, *op;           // k_next / derivative / k_4 by particle (phi in/out)
+  const double *k1x, *k1y, *k1p;  // RK4 combine inputs by particle
+  const double *k2x, *k2y, *k2p;
+  const double *k3x, *k3y, *k3p;
+  double *sx, *sy, *sp;           // master state by particle
+  double *aphi_slot, *n_slot;     // pass-1 results by slot (pass-2 inputs)
+  float *aphi_slot_f;
+  double *a0, *a1;                // additional by particle (in/out)
+  StepArgs sa;
+};
+
+__device__ __forceinline__ void wrap_state(const KParams &P, double &x, double &y) {
+  // the array form of the wrap, applied to every particle (PeriodicBoundaryConditions.cpp:133-141)
+  x = x - floor(x * P.rLx) * P.Lx;
+  y = y - floor(y * P.rLy) * P.Ly;
+}
+
+// Final phi derivative given the in/out value `old` (k_next or derivative entry
+// before the call): the reference writes it only when partners exist.
+__device__ __forceinline__ double phi_out(const Pass1Out &o, double old) {
+  if (o.written) return o.aphi;
+  if (o.vortex_add) return old + o.coh;
+  return old;
+}
+
+template <int EPI>
+__device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int id, int slot, const Pass1Out &o) {
+  double sn, cs;
+  sincos(o.phase, &sn, &cs);
+  double ax = P.v0 * cs, ay = P.v0 * sn;
+  if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
+    double ap = phi_out(o, e.sa.zero_in ? 0.0 : e.op[id]);
+    e.ox[id] = ax;
+    e.oy[id] = ay;
+    e.op[id] = ap;
+    if (EPI == EPI_ITO1) {
+      e.aphi_slot[slot] = ap;
+      e.aphi_slot_f[slot] = (float)ap;
+      e.n_slot[slot] = (double)o.n;
+    }
+    if (EPI == EPI_RK4_FINAL) {
+      // RungeKutta4StepperPtr.cpp:72-77: (k1 + 2 k2 + 2 k3 + k4) * dt / 6 for
+      // x, y and (...) / 6 * dt for phi, evaluated left to right
+      const double dt = e.sa.dt;
+      double x = e.sx[id], y = e.sy[id], p = e.sp[id];
+      x += (e.k1x[id] + 2.0 * e.k2x[id] + 2.0 * e.k3x[id] + ax) * dt / 6.0;
+      y += (e.k1y[id] + 2.0 * e.k2y[id] + 2.0 * e.k3y[id] + ay) * dt / 6.0;
+      p += (e.k1p[id] + 2.0 * e.k2p[id] + 2.0 * e.k3p[id] + ap) / 6.0 * dt;
+      wrap_state(P, x, y);
+      e.sx[id] = x;
+      e.sy[id] = y;
+      e.sp[id] = p;
+    }
+  } else if (EPI == EPI_SE) {
+    // StochasticEulerStepperPtr.hpp:44-53 with a zero-filled derivative and the
+    // natural frequency in the drift as (omega + a_phi) dt
+    double ap = phi_out(o, 0.0);
+    double xi;
+    if (e.sa.noise) {
+      xi = e.sa.noise[id];
+    } else {
+      double z1;
+      normal_pair(e.sa.seed, e.sa.step, kStreamStep, (uint32_t)id, xi, z1);
+    }
+    const double dt = e.sa.dt;
+    double dW = e.sa.sqdt * xi;
+    double w = e.sa.omega ? e.sa.omega[id] : 0.0;
+    double x = e.sx[id], y = e.sy[id], p = e.sp[id];
+    x += ax * dt;
+    y += ay * dt;
+    p += (w + ap) * dt + e.sa.sq2D * dW;
+    wrap_state(P, x, y);
+    e.sx[id] = x;
+    e.sy[id] = y;
+    e.sp[id] = p;
+  }
+}
+
+template <int EPI>
+__device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int id, const Pass2Out &o) {
+  if (EPI == EPI_ITO2) {
+    e.a0[id] = o.a0;
+    e.a1[id] = o.a1;
+  } else {
+    // StochasticRungeKuttaStepperPtr.hpp:52-66
+    double sn, cs;
+    sincos(o.phase, &sn, &cs);
+    const double ax = P.v0 * cs, ay = P.v0 * sn, ap = o.ai;
+    double U_1, U_2;
+    if (e.sa.noise) {
+      U_1 = e.sa.noise[2 * id];
+      U_2 = e.sa.noise[2 * id + 1];
+    } else {
+      normal_pair(e.sa.seed, e.sa.step, kStreamStep, (uint32_t)id, U_1, U_2);
+    }
+    const double dt = e.sa.dt, D = e.sa.D, sq2D = e.sa.sq2D;
+    double delta_W = U_1 * e.sa.sqdt;
+    double delta_Z = e.sa.h1 * (U_1 + U_2 / e.sa.sqrt3);
+    double w = e.sa.omega ? e.sa.omega[id] : 0.0;
+    double x = e.sx[id], y = e.sy[id], p = e.sp[id];
+    x += ax * dt - sq2D * ay * delta_Z - 0.5 * (ap * ay + D * ax) * dt * dt;
+    y += ay * dt + sq2D * ax * delta_Z + 0.5 * (ap * ax - D * ay) * dt * dt;
+    p += (w + ap) * dt + sq2D * delta_W + o.a0 * delta_Z + 0.5 * o.a1 * dt * dt;
+    wrap_state(P, x, y);
+    e.sx[id] = x;
+    e.sy[id] = y;
+    e.sp[id] = p;
+  }
+}
+
+__device__ __forceinline__ void count_add(unsigned long long *counts, long long a, long long b, long long c) {
+  // wavefront reduction, one atomic per wavefront
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    a += __shfl_xor(a, d, 64);
+    b += __shfl_xor(b, d, 64);
+    c += __shfl_xor(c, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&counts[0], (unsigned long long)a);
+    atomicAdd(&counts[1], (unsigned long long)b);
+    atomicAdd(&counts[2], (unsigned long long)c);
+  }
+}
+
+// =============================================================================
+// Row-segment setup and LDS staging
+// =============================================================================
+
+__device__ __forceinline__ int block_inclusive_scan(int v, int *wtot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) wtot[w] = v;
+  __syncthreads();
+  int add = 0;
+#pragma unroll
+  for (int q = 0; q < kBlock / 64; ++q) add += (q < w) ? wtot[q] : 0;
+  __syncthreads();
+  return v + add;
+}
+
+struct SegShared {
+  int off[kMaxSlots + 1];  // LDS offset of halo cell (r, q), flattened r * (W + 2) + q; off[3(W+2)] = total
+  int gst[kMaxSlots];      // its first global slot
+  int wtot[kBlock / 64];
+};
+
+// Blocks are dealt to the 8 XCDs round robin; map them so that each XCD takes
+// a contiguous run of row segments and its L2 sees neighbouring halo rows.
+__device__ __forceinline__ int xcd_segment(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+struct Seg {
+  int cy, cx0, W, ncol, ns, total;
+};
+
+__device__ __forceinline__ Seg seg_setup(const KParams &P, const int *start, SegShared &sh) {
+  Seg g;
+  const int seg = xcd_segment(blockIdx.x, gridDim.x);
+  g.cy = seg / P.segs;
+  g.cx0 = (seg - g.cy * P.segs) * P.seg_w;
+  g.W = min(P.seg_w, P.nx - g.cx0);
+  g.ncol = g.W + 2;
+  g.ns = 3 * g.ncol;
+  int cnt = 0;
+  if ((int)threadIdx.x < g.ns) {
+    int r = threadIdx.x / g.ncol, q = threadIdx.x - r * g.ncol;
+    int cyr = adjust_cell(g.cy + r - 1, P.ny);
+    int cxq = g.cx0 - 1 + q;
+    if (cxq < 0) cxq += P.nx;
+    else if (cxq >= P.nx) cxq -= P.nx;
+    int cell = cyr * P.nx + cxq;
+    int g0 = start[cell];
+    cnt = start[cell + 1] - g0;
+    sh.gst[threadIdx.x] = g0;
+  }
+  int incl = block_inclusive_scan(cnt, sh.wtot);
+  if ((int)threadIdx.x < g.ns) sh.off[threadIdx.x] = incl - cnt;
+  if ((int)threadIdx.x == g.ns - 1) sh.off[g.ns] = incl;
+  __syncthreads();
+  g.total = sh.off[g.ns];
+  return g;
+}
+
+// largest halo slot s with off[s] <= e
+__device__ __forceinline__ int find_slot(const SegShared &sh, int ns, int e) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (sh.off[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// LDS element arrays of one precision and pass, carved from dynamic LDS.
+template <int PREC, int PASS>
+struct LdsTile;
+
+template <int PASS>
+struct LdsTile<P32K, PASS> {
+  float4 *f4;
+  int *slot;
+  float *aj;
+  static constexpr int bytes = 20 + (PASS == 2 ? 4 : 0);
+  __device__ __forceinline__ LdsTile(void *base, int cap) {
+    f4 = (float4 *)base;
+    slot = (int *)(f4 + cap);
+    aj = (float *)(slot + cap);
+  }
+  __device__ __forceinline__ void load(int e, int g, const Tiles &t, const EpiCtx &ec) {
+    f4[e] = t.f4[g];
+    slot[e] = g;
+    if (PASS == 2) aj[e] = ec.aphi_slot_f[g];
+  }
+  __device__ __forceinline__ Src32 src() const { return Src32{f4, slot, aj}; }
+};
+
+template <int PREC, int PASS>
+struct LdsTile {  // P64 / P64F
+  double *x, *y, *a, *b, *aj;
+  static constexpr int ncomp = (PREC == P64 ? 3 : 4) + (PASS == 2 ? 1 : 0);
+  static constexpr int bytes = 8 * ncomp;
+  __device__ __forceinline__ LdsTile(void *base, int cap) {
+    x = (double *)base;
+    y = x + cap;
+    a = y + cap;
+    b = a + cap;
+    aj = (PREC == P64 ? a : b) + cap;
+  }
+  __device__ __forceinline__ void load(int e, int g, const Tiles &t, const EpiCtx &ec) {
+    x[e] = t.x[g];
+    y[e] = t.y[g];
+    if (PREC == P64) {
+      a[e] = t.p[g];
+    } else {
+      a[e] = t.c[g];
+      b[e] = t.s[g];
+    }
+    if (PASS == 2) aj[e] = ec.aphi_slot[g];
+  }
+  __device__ __forceinline__ Src64 src() const {
+    return PREC == P64 ? Src64{x, y, a, nullptr, nullptr, aj} : Src64{x, y, nullptr, a, b, aj};
+  }
+};
+
+// global-memory source over the cell-sorted tiles (overflow fallback)
+template <int PREC>
+__device__ __forceinline__ typename SrcOf<PREC>::type global_src(const Tiles &t, const EpiCtx &e, int pass) {
+  if constexpr (PREC == P32K) {
+    return Src32{t.f4, nullptr, pass == 2 ? e.aphi_slot_f : nullptr};
+  } else {
+    return Src64{t.x, t.y, t.p, t.c, t.s, pass == 2 ? e.aphi_slot : nullptr};
+  }
+}
+
+// Visits the candidates of the centre particle at LDS index ei (halo column qi)
+// in reference order; returns the number of candidates.
+template <class Acc, class S>
+__device__ __forceinline__ int visit_lds(Acc &acc, const S &src, const SegShared &sh, int ncol, int qi, int ei,
+                                         const KParams &P) {
+  int cand = 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int base = r * ncol + qi;
+    const int lo = sh.off[base - 1], b1 = sh.off[base], b2 = sh.off[base + 1], hi = sh.off[base + 2];
+    acc.row(r - 1, b1, b2, P);
+    cand += hi - lo;
+    if (r == 1) {
+#pragma unroll 2
+      for (int m = lo; m < ei; ++m) acc.pair(src, m, P);
+#pragma unroll 2
+      for (int m = ei + 1; m < hi; ++m) acc.pair(src, m, P);
+    } else {
+#pragma unroll 2
+      for (int m = lo; m < hi; ++m) acc.pair(src, m, P);
+    }
+  }
+  return cand - 1;
+}
+
+// Same visit over the global tiles, cell by cell (slot k in cell c).
+template <class Acc, class S>
+__device__ __forceinline__ int visit_global(Acc &acc, const S &src, const int *start, int c, int k, const KParams &P) {
+  const int cx = c % P.nx, cy = c / P.nx;
+  int cand = 0;
+  for (int dyo = -1; dyo <= 1; ++dyo) {
+    const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
+    for (int dxo = -1; dxo <= 1; ++dxo) {
+      const int jc = row + adjust_cell(cx + dxo, P.nx);
+      const int m0 = start[jc], m1 = start[jc + 1];
+      // column offsets from the boundary trick: all of [m0, m1) is one column
+      if (dxo < 0) acc.row(dyo, m1, m1, P);
+      else if (dxo == 0) acc.row(dyo, m0, m1, P);
+      else acc.row(dyo, m0, m0, P);
+      cand += m1 - m0;
+      for (int m = m0; m < m1; ++m)
+        if (m != k) acc.pair(src, m, P);
+    }
+  }
+  return cand - 1;
+}
+
+// =============================================================================
+// Pass-1 and pass-2 kernels over linked cells
+// =============================================================================
+
+template <class Acc, int PREC, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass1_lds(KParams P, const int *__restrict__ start, Tiles t,
+                                                      const int *__restrict__ sorted, const double *gp, EpiCtx e,
+                                                      unsigned long long *counts) {
+  __shared__ SegShared sh;
+  extern __shared__ float4 dyn_lds[];
+  Seg g = seg_setup(P, start, sh);
+  const bool staged = g.total <= P.lds_cap;
+  LdsTile<PREC, 1> lt(dyn_lds, P.lds_cap);
+  if (staged) {
+    for (int el = threadIdx.x; el < g.total; el += kBlock) {
+      int s = find_slot(sh, g.ns, el);
+      lt.load(el, sh.gst[s] + (el - sh.off[s]), t, e);
+    }
+  }
+  __syncthreads();
+  const int e0 = sh.off[g.ncol + 1];
+  const int ncen = sh.off[g.ncol + g.W + 1] - e0;
+  const int k0 = sh.gst[g.ncol + 1];
+  long long cand_sum = 0, n_sum = 0, n2_sum = 0;
+  for (int p = threadIdx.x; p < ncen; p += kBlock) {
+    const int k = k0 + p, ei = e0 + p;
+    Acc acc;
+    int cand;
+    if (staged) {
+      int lo = 1, hi = g.W;  // centre column of ei
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sh.off[g.ncol + mid] <= ei) lo = mid;
+        else hi = mid - 1;
+      }
+      acc.init(lt.src(), ei, k, t, P);
+      cand = visit_lds(acc, lt.src(), sh, g.ncol, lo, ei, P);
+    } else {
+      auto gs = global_src<PREC>(t, e, 1);
+      acc.init(gs, k, k, t, P);
+      cand = visit_global(acc, gs, start, t.cell[k], k, P);
+    }
+    const int id = sorted[k];
+    Pass1Out o{};
+    acc.finish(P, PREC == P32K ? gp[id] : 0.0, o);
+    epilogue1<EPI>(P, e, id, k, o);
+    cand_sum += cand;
+    n_sum += o.n;
+    n2_sum += o.n2;
+  }
+  if (counts) count_add(counts, cand_sum, n_sum, n2_sum);
+}
+
+template <int PREC, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__restrict__ start, Tiles t,
+                                                      const int *__restrict__ sorted, const double *gp, EpiCtx e) {
+  __shared__ SegShared sh;
+  extern __shared__ float4 dyn_lds[];
+  Seg g = seg_setup(P, start, sh);
+  const bool staged = g.total <= P.lds_cap;
+  LdsTile<PREC, 2> lt(dyn_lds, P.lds_cap);
+  if (staged) {
+    for (int el = threadIdx.x; el < g.total; el += kBlock) {
+      int s = find_slot(sh, g.ns, el);
+      lt.load(el, sh.gst[s] + (el - sh.off[s]), t, e);
+    }
+  }
+  __syncthreads();
+  const int e0 = sh.off[g.ncol + 1];
+  const int ncen = sh.off[g.ncol + g.W + 1] - e0;
+  const int k0 = sh.gst[g.ncol + 1];
+  for (int p = threadIdx.x; p < ncen; p += kBlock) {
+    const int k = k0 + p, ei = e0 + p;
+    const int id = sorted[k];
+    double in0 = 0.0, in1 = 0.0;
+    if (EPI == EPI_ITO2) {
+      in0 = e.a0[id];
+      in1 = e.a1[id];
+    }
+    Chim2Acc<PREC> acc;
+    if (staged) {
+      int lo = 1, hi = g.W;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sh.off[g.ncol + mid] <= ei) lo = mid;
+        else hi = mid - 1;
+      }
+      acc.init(lt.src(), ei, k, t, P, e.aphi_slot[k], e.n_slot[k], in0, in1);
+      visit_lds(acc, lt.src(), sh, g.ncol, lo, ei, P);
+    } else {
+      auto gs = global_src<PREC>(t, e, 2);
+      acc.init(gs, k, k, t, P, e.aphi_slot[k], e.n_slot[k], in0, in1);
+      visit_global(acc, gs, start, t.cell[k], k, P);
+    }
+    Pass2Out o;
+    acc.finish(P, PREC == P32K ? gp[id] : 0.0, o);
+    epilogue2<EPI>(P, e, id, o);
+  }
+}
+
+// =============================================================================
+// All pairs (rho > L/4): thread per particle, j = 0..N-1, stage positions not
+// wrapped (ChimeraSystem.cpp:131-167, :184-254)
+// =============================================================================
+
+template <int PREC, int SYS>
+__device__ __forceinline__ void allpairs_pass1(const KParams &P, int i, const double *gx, const double *gy,
+                                               const double *gp, Pass1Out &o) {
+  const double xi = gx[i], yi = gy[i], pi = gp[i];
+  if (SYS == SPC2_CHIMERA) {
+    if (PREC == P64) {
+      double A = 0.0;
+      int n = 0;
+      for (int j = 0; j < P.n; ++j) {
+        if (j == i) continue;
+        double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+        double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+        double r2 = dx * dx + dy * dy;
+        if (r2 <= P.rho2) {
+          A += sin(gp[j] - pi - P.alpha);
+          ++n;
+        }
+      }
+      o.n = n;
+      o.written = n != 0;
+      o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
+    } else {
+      // factored form; the FP32 variant accumulates in Kahan FP32 over FP64 geometry
+      double S = 0.0, C = 0.0;
+      Kahan Sf, Cf;
+      Sf.init();
+      Cf.init();
+      int n = 0;
+      for (int j = 0; j < P.n; ++j) {
+        if (j == i) continue;
+        double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+        double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+        double r2 = dx * dx + dy * dy;
+        if (r2 <= P.rho2) {
+          double sj, cj;
+          sincos(gp[j], &sj, &cj);
+          if (PREC == P32K) {
+            Sf.add((float)sj);
+            Cf.add((float)cj);
+          } else {
+            S += sj;
+            C += cj;
+          }
+          ++n;
+        }
+      }
+      if (PREC == P32K) {
+        S = Sf.value();
+        C = Cf.value();
+      }
+      double sa, ca;
+      sincos(pi + P.alpha, &sa, &ca);
+      o.n = n;
+      o.written = n != 0;
+      o.aphi = n != 0 ? P.sigma * (ca * S - sa * C) / (double)n : 0.0;
+    }
+    o.n2 = 0;
+    o.vortex_add = 0;
+  } else {
+    double A = 0.0, B = 0.0, Cc = 0.0;
+    int n0 = 0, n1 = 0, n2 = 0;
+    for (int j = 0; j < P.n; ++j) {
+      if (j == i) continue;
+      double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+      double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+      double dr = sqrt(dx * dx + dy * dy);
+      if (dr <= P.xi_a) {
+        A += vortex_w(P, dr) * sin(gp[j] - pi);
+        ++n0;
+      } else if (dr <= P.rho) {
+        B += vortex_w(P, dr) * sin(gp[j] - pi);
+        ++n1;
+      }
+      if (dr <= P.xi_r) {
+        Cc += -P.kappa * sin(atan2(dy, dx) - pi);
+        ++n2;
+      }
+    }
+    o.n = n0 + n1;
+    o.n2 = n2;
+    if (n0 + n1 > 0) {
+      o.aphi = (A + B) / ((double)n0 + (double)n1);
+      if (n2 > 0) o.aphi += Cc / (double)n2;
+      o.written = 1;
+      o.vortex_add = 0;
+    } else {
+      o.aphi = 0.0;
+      o.written = 0;
+      o.vortex_add = n2 > 0;
+      o.coh = n2 > 0 ? Cc / (double)n2 : 0.0;
+    }
+  }
+  o.phase = pi;
+}
+
+template <int PREC, int SYS, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass1_allpairs(KParams P, const double *gx, const double *gy,
+                                                           const double *gp, EpiCtx e, unsigned long long *counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < P.n;
+  Pass1Out o{};
+  if (active) {
+    allpairs_pass1<PREC, SYS>(P, i, gx, gy, gp, o);
+    epilogue1<EPI>(P, e, i, i, o);
+  }
+  if (counts) count_add(counts, active ? P.n - 1 : 0, active ? o.n : 0, active ? o.n2 : 0);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass2_allpairs(KParams P, const double *gx, const double *gy,
+                                                           const double *gp, EpiCtx e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const double xi = gx[i], yi = gy[i], pi = gp[i];
+  const double ai = e.aphi_slot[i], n = e.n_slot[i];
+  double a0 = 0.0, a1 = 0.0;
+  if (EPI == EPI_ITO2) {
+    a0 = e.a0[i];
+    a1 = e.a1[i];
+  }
+  // ChimeraSystem.cpp:223-254 (FP64 form for every precision: small-N path)
+  for (int j = 0; j < P.n; ++j) {
+    if (j == i) continue;
+    double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+    double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if ((r2 <= P.rho2) && (n != 0)) {
+      double cph = cos(gp[j] - pi - P.alpha);
+      a0 += cph / n;
+      a1 += (e.aphi_slot[j] - ai) * cph / n;
+    }
+  }
+  Pass2Out o;
+  o.phase = pi;
+  o.ai = ai;
+  o.a0 = a0 * -sqrt(2.0 * P.D);
+  o.a1 = a1 + -2.0 * P.D * ai;
+  epilogue2<EPI>(P, e, i, o);
+}
+
+// =============================================================================
+// dispatch
+// =============================================================================
+
+EpiCtx make_ctx(Sim &s, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
+                double *const k2[3], double *const k3[3]) {
+  EpiCtx e{};
+  e.ox = ox;
+  e.oy = oy;
+  e.op = op;
+  if (k1) {
+    e.k1x = k1[0];
+    e.k1y = k1[1];
+    e.k1p = k1[2];
+    e.k2x = k2[0];
+    e.k2y = k2[1];
+    e.k2p = k2[2];
+    e.k3x = k3[0];
+    e.k3y = k3[1];
+    e.k3p = k3[2];
+  }
+  e.sx = s.sx;
+  e.sy = s.sy;
+  e.sp = s.sp;
+  e.aphi_slot = s.aphi_slot;
+  e.n_slot = s.n_slot;
+  e.aphi_slot_f = s.aphi_slot_f;
+  e.a0 = s.add[0];
+  e.a1 = s.add[1];
+  e.sa = sa;
+  return e;
+}
+
+template <int PREC, int PASS>
+size_t lds_bytes(const Sim &s) {
+  return (size_t)s.kp.lds_cap * LdsTile<PREC, PASS>::bytes;
+}
+
+template <int PREC, int SYS, int EPI>
+void pass1_typed(Sim &s, const EpiCtx &e) {
+  unsigned long long *cnt = s.counting ? s.counts : nullptr;
+  if (s.linked) {
+    const dim3 grid(s.ny * s.kp.segs), blk(kBlock);
+    const size_t sh = lds_bytes<PREC, 1>(s);
+    if (SYS == SPC2_CHIMERA)
+      hipLaunchKernelGGL((k_pass1_lds<ChimAcc<PREC>, PREC, EPI>), grid, blk, sh, s.stream, s.kp, s.start, s.t,
+                         s.sorted, s.gp, e, cnt);
+    else
+      hipLaunchKernelGGL((k_pass1_lds<VortAcc<PREC>, PREC, EPI>), grid, blk, sh, s.stream, s.kp, s.start, s.t,
+                         s.sorted, s.gp, e, cnt);
+  } else {
+    hipLaunchKernelGGL((k_pass1_allpairs<PREC, SYS, EPI>), dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp,
+                       s.gx, s.gy, s.gp, e, cnt);
+  }
+}
+
+template <int PREC, int SYS>
+void pass1_epi(Sim &s, int epi, const EpiCtx &e) {
+  switch (epi) {
+    case EPI_K: pass1_typed<PREC, SYS, EPI_K>(s, e); break;
+    case EPI_ITO1: pass1_typed<PREC, SYS, EPI_ITO1>(s, e); break;
+    case EPI_SE: pass1_typed<PREC, SYS, EPI_SE>(s, e); break;
+    case EPI_RK4_FINAL: pass1_typed<PREC, SYS, EPI_RK4_FINAL>(s, e); break;
+  }
+}
+
+template <int PREC>
+void pass1_sys(Sim &s, int epi, const EpiCtx &e) {
+  if (s.P.system == SPC2_CHIMERA)
+    pass1_epi<PREC, SPC2_CHIMERA>(s, epi, e);
+  else
+    pass1_epi<PREC, SPC2_VORTEX>(s, epi, e);
+}
+
+template <int PREC, int EPI>
+void pass2_typed(Sim &s, const EpiCtx &e) {
+  if (s.linked) {
+    const size_t sh = lds_bytes<PREC, 2>(s);
+    hipLaunchKernelGGL((k_pass2_lds<PREC, EPI>), dim3(s.ny * s.kp.segs), dim3(kBlock), sh, s.stream, s.kp, s.start,
+                       s.t, s.sorted, s.gp, e);
+  } else
+    hipLaunchKernelGGL((k_pass2_allpairs<EPI>), dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, s.gx, s.gy,
+                       s.gp, e);
+}
+
+template <int PREC>
+void pass2_epi(Sim &s, int epi, const EpiCtx &e) {
+  if (epi == EPI_ITO2)
+    pass2_typed<PREC, EPI_ITO2>(s, e);
+  else
+    pass2_typed<PREC, EPI_SRK>(s, e);
+}
+
+// HIP events around each pair-kernel launch on the handle's stream; the
+// elapsed times are collected lazily (spc2_get_profile) so that the host does
+// not wait on the device per launch.
+struct ProfScope {
+  Sim &s;
+  hipEvent_t b = nullptr, en = nullptr;
+  explicit ProfScope(Sim &sim) : s(sim) {
+    if (!s.profiling) return;
+    s.prof_take(b, en);
+    (void)hipEventRecord(b, s.stream);
+  }
+  ~ProfScope() {
+    if (!s.profiling) return;
+    (void)hipEventRecord(en, s.stream);
+    s.prof_pending.emplace_back(b, en);
+  }
+};
+
+}  // namespace
+
+void configure_segments(Sim &s) {
+  KParams &k = s.kp;
+  if (!s.linked) {
+    k.seg_w = k.segs = 1;
+    k.lds_cap = 0;
+    return;
+  }
+  const double per_cell = (double)s.n / (double)s.nc;
+  int w = (int)std::floor(218.0 / std::max(per_cell, 1e-9));
+  w = std::max(1, std::min({w, kMaxSegW, s.nx}));
+  k.seg_w = w;
+  k.segs = (s.nx + w - 1) / w;
+  const double halo = 3.0 * (w + 2) * per_cell;
+  int cap = (int)std::ceil((1.6 * halo + 160.0) / 64.0) * 64;
+  int elem = 0;
+  switch (s.prec) {
+    case P32K: elem = LdsTile<P32K, 2>::bytes; break;
+    case P64: elem = LdsTile<P64, 2>::bytes; break;
+    default: elem = LdsTile<P64F, 2>::bytes;
+  }
+  cap = std::min(cap, (60 * 1024) / elem);
+  if (const char *env = std::getenv("SPC2_LDS_CAP")) cap = std::max(0, std::min(cap, std::atoi(env)));
+  k.lds_cap = cap;
+}
+
+void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
+                  double *const k2[3], double *const k3[3]) {
+  EpiCtx e = make_ctx(s, ox, oy, op, sa, k1, k2, k3);
+  ProfScope prof(s);
+  switch (s.prec) {
+    case P64: pass1_sys<P64>(s, epi, e); break;
+    case P64F: pass1_sys<P64F>(s, epi, e); break;
+    default: pass1_sys<P32K>(s, epi, e);
+  }
+}
+
+void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa) {
+  EpiCtx e = make_ctx(s, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
+  e.a0 = a0;
+  e.a1 = a1;
+  ProfScope prof(s);
+  switch (s.prec) {
+    case P64: pass2_epi<P64>(s, epi, e); break;
+    case P64F: pass2_epi<P64F>(s, epi, e); break;
+    default: pass2_epi<P32K>(s, epi, e);
+  }
+}
+
+}  // namespace spc2

@@ -217,6 +217,32 @@ def test_vortex_rhs_k(gpu, prec, n, mode, kc):
     assert rc[2] > 0
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
+@pytest.mark.parametrize("cap", ["0", "300"])
+def test_lds_overflow_fallback_matches(gpu, prec, cap, monkeypatch):
+    """Segments whose halo exceeds the LDS capacity read the same tiles from
+    global memory; results must not depend on which segments were staged."""
+    n, rho = 40000, 0.05
+    L = math.sqrt(n / 4000.0)
+    st = O.make_state(7, 1, n, L, rho)
+    kp = O.uniform(70, 3 * n, -1.0, 1.0)
+    kn = O.uniform(71, 3 * n, 5.0, 6.0)
+    outs = []
+    for c in (None, cap):
+        if c is None:
+            monkeypatch.delenv("SPC2_LDS_CAP", raising=False)
+        else:
+            monkeypatch.setenv("SPC2_LDS_CAP", c)
+        with spc2.Sim(vort_params(n, L, precision=prec)) as s:
+            s.set_state(st)
+            outs.append(s.eval_rhs(kp, 0.5, 0.01, kn))
+        with spc2.Sim(chim_params(n, L, 0.1, 0.3, 1.2, 1.4, 0.05, prec)) as s:
+            s.set_state(st)
+            outs.append(np.concatenate(s.eval_rhs_ito()))
+    monkeypatch.delenv("SPC2_LDS_CAP", raising=False)
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
+
+
 # ---- steps with the reference's own noise ----------------------------------------
 
 @pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
