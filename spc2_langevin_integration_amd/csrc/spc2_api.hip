@@ -1,10 +1,12 @@
 // spc2 C ABI: handle lifetime, host <-> device copies and the per-method
-// pipelines that chain the kernels of spc2_kernels.hip on the handle's stream.
+// pipelines that chain the kernels of spc2_cells.hip, spc2_pairs.hip and
+// spc2_misc.hip on the handle's stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -57,7 +59,10 @@ void free_all(Sim &s) {
   for (auto &p : s.der) f(p);
   for (auto &p : s.add) f(p);
   f(s.gx); f(s.gy); f(s.gp);
-  f(s.cell_of); f(s.count); f(s.local); f(s.start); f(s.tmp); f(s.sorted);
+  f(s.cell_of); f(s.count); f(s.local); f(s.start); f(s.tmp); f(s.tmpid); f(s.slot_h); f(s.slot_id);
+  f(s.hid); f(s.hpos); f(s.spare_i);
+  for (auto &p : s.spare) f(p);
+  for (auto &p : s.io) f(p);
   f(s.t.x); f(s.t.y); f(s.t.p); f(s.t.c); f(s.t.s); f(s.t.f4); f(s.t.cell);
   f(s.aphi_slot); f(s.n_slot); f(s.aphi_slot_f);
   f(s.noise); f(s.counts); f(s.red); f(s.bsum);
@@ -116,7 +121,11 @@ spc2_status alloc_all(Sim &s) {
   for (auto &p : s.der) A(p, n);
   for (auto &p : s.add) A(p, n);
   A(s.gx, n); A(s.gy, n); A(s.gp, n);
-  A(s.cell_of, n); A(s.count, nc); A(s.local, n); A(s.start, nc + 1); A(s.tmp, n); A(s.sorted, n);
+  A(s.cell_of, n); A(s.count, nc); A(s.local, n); A(s.start, nc + 1); A(s.tmp, n); A(s.tmpid, n);
+  A(s.slot_h, n); A(s.slot_id, n); A(s.hid, n); A(s.hpos, n); A(s.spare_i, n);
+  for (auto &p : s.io) A(p, n);
+  if (s.linked)
+    for (auto &p : s.spare) A(p, n);
   A(s.t.cell, n);
   if (s.prec == P32K) {
     A(s.t.f4, n);
@@ -128,7 +137,7 @@ spc2_status alloc_all(Sim &s) {
     }
   }
   A(s.aphi_slot, n); A(s.n_slot, n); A(s.aphi_slot_f, n);
-  A(s.counts, 3); A(s.red, 2 * 1024 + 8); A(s.bsum, (size_t)scan_blocks(std::max(s.nc, 1)) + 1);
+  A(s.counts, 3 * kCountSlots); A(s.red, 2 * 1024 + 8); A(s.bsum, (size_t)scan_blocks(std::max(s.nc, 1)) + 1);
 #undef A
   HIP_TRY(hipMemset(s.count, 0, sizeof(int) * nc));
   HIP_TRY(hipMemset(s.omega, 0, sizeof(double) * n));
@@ -136,19 +145,21 @@ spc2_status alloc_all(Sim &s) {
     for (auto &p : kk) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
   for (auto &p : s.der) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
   for (auto &p : s.add) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
-  HIP_TRY(hipMemset(s.counts, 0, sizeof(unsigned long long) * 3));
+  HIP_TRY(hipMemset(s.counts, 0, sizeof(unsigned long long) * 3 * kCountSlots));
   HIP_TRY(hipMemset(s.sx, 0, sizeof(double) * n));
   HIP_TRY(hipMemset(s.sy, 0, sizeof(double) * n));
   HIP_TRY(hipMemset(s.sp, 0, sizeof(double) * n));
   return SPC2_OK;
 }
 
-// AoS [x, y, phi] host array <-> three device arrays
+// AoS host array by particle id (stride `stride`, `comps` components) <->
+// home-ordered device arrays, through the id-ordered transfer buffers.
 spc2_status upload_aos(Sim &s, const double *aos, int stride, double *const *dst, int comps) {
   std::vector<double> buf((size_t)s.n);
   for (int c = 0; c < comps; ++c) {
     for (int i = 0; i < s.n; ++i) buf[(size_t)i] = aos[(size_t)i * stride + c];
-    HIP_TRY(hipMemcpyAsync(dst[c], buf.data(), sizeof(double) * (size_t)s.n, hipMemcpyHostToDevice, s.stream));
+    HIP_TRY(hipMemcpyAsync(s.io[c], buf.data(), sizeof(double) * (size_t)s.n, hipMemcpyHostToDevice, s.stream));
+    launch_to_home(s, s.io[c], dst[c]);
     HIP_TRY(hipStreamSynchronize(s.stream));
   }
   return SPC2_OK;
@@ -157,7 +168,8 @@ spc2_status upload_aos(Sim &s, const double *aos, int stride, double *const *dst
 spc2_status download_aos(Sim &s, double *aos, int stride, double *const *src, int comps) {
   std::vector<double> buf((size_t)s.n);
   for (int c = 0; c < comps; ++c) {
-    HIP_TRY(hipMemcpyAsync(buf.data(), src[c], sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+    launch_from_home(s, src[c], s.io[c]);
+    HIP_TRY(hipMemcpyAsync(buf.data(), s.io[c], sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipStreamSynchronize(s.stream));
     for (int i = 0; i < s.n; ++i) aos[(size_t)i * stride + c] = buf[(size_t)i];
   }
@@ -174,7 +186,30 @@ void build(Sim &s, double *const *kprev, double kcdt) {
     launch_scan(s);
     launch_place(s);
     launch_rank_tiles(s);
+    s.built = true;
   }
+}
+
+// Moves the particle arrays to the cell order of the current state (used
+// when a new state arrives).
+spc2_status rehome_now(Sim &s) {
+  if (!s.linked) return SPC2_OK;
+  build(s, nullptr, 0.0);
+  launch_rehome(s);
+  s.built = false;
+  s.steps_since_rehome = 0;
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+// After every rehome_every steps, moves the particle arrays to the cell order
+// of the last build.
+void maybe_rehome(Sim &s) {
+  if (!s.linked) return;
+  if (++s.steps_since_rehome < s.rehome_every || !s.built) return;
+  launch_rehome(s);
+  s.built = false;
+  s.steps_since_rehome = 0;
 }
 
 StepArgs step_args(const Sim &s, double dt, uint64_t step) {
@@ -262,6 +297,7 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
   }
   fill_kparams(s);
   configure_segments(s);
+  if (const char *env = std::getenv("SPC2_REHOME_EVERY")) s.rehome_every = std::max(1, std::atoi(env));
   hipError_t e = hipGetDevice(&s.device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -270,6 +306,10 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
     return fail(SPC2_ERR_HIP, m);
   }
   spc2_status st = alloc_all(s);
+  if (st == SPC2_OK) {
+    launch_identity_homes(s);
+    if (hipStreamSynchronize(s.stream) != hipSuccess) st = fail(SPC2_ERR_HIP, "initialising homes failed");
+  }
   if (st != SPC2_OK) {
     std::string m = g_err;
     free_all(s);
@@ -300,7 +340,8 @@ spc2_status spc2_set_state(spc2_sim_t h, const double *state) {
   if (check_handle(h) || !state) return fail(SPC2_ERR_INVALID, "null argument");
   Sim &s = h->s;
   double *d[3] = {s.sx, s.sy, s.sp};
-  return upload_aos(s, state, 3, d, 3);
+  if (spc2_status st = upload_aos(s, state, 3, d, 3)) return st;
+  return rehome_now(s);
 }
 
 spc2_status spc2_get_state(spc2_sim_t h, double *state) {
@@ -317,13 +358,14 @@ spc2_status spc2_init_random_state(spc2_sim_t h, double omega_std) {
   launch_init_state(s, omega_std);
   if (spc2_status st = check_launch("init_state")) return st;
   HIP_TRY(hipStreamSynchronize(s.stream));
-  return SPC2_OK;
+  return rehome_now(s);
 }
 
 spc2_status spc2_set_omega(spc2_sim_t h, const double *omega) {
   if (check_handle(h) || !omega) return fail(SPC2_ERR_INVALID, "null argument");
   Sim &s = h->s;
-  HIP_TRY(hipMemcpyAsync(s.omega, omega, sizeof(double) * (size_t)s.n, hipMemcpyHostToDevice, s.stream));
+  HIP_TRY(hipMemcpyAsync(s.io[0], omega, sizeof(double) * (size_t)s.n, hipMemcpyHostToDevice, s.stream));
+  launch_to_home(s, s.io[0], s.omega);
   HIP_TRY(hipStreamSynchronize(s.stream));
   return SPC2_OK;
 }
@@ -400,6 +442,7 @@ spc2_status spc2_step(spc2_sim_t h, int32_t method, double dt, int64_t step0, in
       build(s, k3, 1.0 * dt);
       launch_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
     }
+    maybe_rehome(s);
     if (spc2_status st = check_launch("step")) return st;
   }
   HIP_TRY(hipStreamSynchronize(s.stream));
@@ -445,9 +488,10 @@ spc2_status spc2_get_cell_list(spc2_sim_t h, int32_t *cell_of, int32_t *cell_sta
   if (!s.linked) return fail(SPC2_ERR_INVALID, "no cell list on the all-pairs path");
   build(s, nullptr, 0.0);
   if (spc2_status st = check_launch("cell list")) return st;
-  HIP_TRY(hipMemcpyAsync(cell_of, s.cell_of, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+  launch_from_home_int(s, s.cell_of, s.spare_i);
+  HIP_TRY(hipMemcpyAsync(cell_of, s.spare_i, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipMemcpyAsync(cell_start, s.start, sizeof(int) * ((size_t)s.nc + 1), hipMemcpyDeviceToHost, s.stream));
-  HIP_TRY(hipMemcpyAsync(sorted, s.sorted, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipMemcpyAsync(sorted, s.slot_id, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
   return SPC2_OK;
 }
@@ -463,7 +507,7 @@ spc2_status spc2_set_counting(spc2_sim_t h, int32_t enable) {
   if (check_handle(h)) return SPC2_ERR_INVALID;
   Sim &s = h->s;
   s.counting = enable != 0;
-  HIP_TRY(hipMemsetAsync(s.counts, 0, sizeof(unsigned long long) * 3, s.stream));
+  HIP_TRY(hipMemsetAsync(s.counts, 0, sizeof(unsigned long long) * 3 * kCountSlots, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
   return SPC2_OK;
 }
@@ -471,10 +515,11 @@ spc2_status spc2_set_counting(spc2_sim_t h, int32_t enable) {
 spc2_status spc2_get_counts(spc2_sim_t h, int64_t *counts3) {
   if (check_handle(h) || !counts3) return fail(SPC2_ERR_INVALID, "null argument");
   Sim &s = h->s;
-  unsigned long long c[3];
-  HIP_TRY(hipMemcpyAsync(c, s.counts, sizeof(c), hipMemcpyDeviceToHost, s.stream));
+  std::vector<unsigned long long> c(3 * (size_t)kCountSlots);
+  HIP_TRY(hipMemcpyAsync(c.data(), s.counts, sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
-  for (int i = 0; i < 3; ++i) counts3[i] = (int64_t)c[i];
+  for (int i = 0; i < 3; ++i) counts3[i] = 0;
+  for (size_t q = 0; q < c.size(); ++q) counts3[q % 3] += (int64_t)c[q];
   return SPC2_OK;
 }
 

@@ -1,6 +1,11 @@
 // Linked-cell build on gfx950: stage + binning + histogram, a three-phase
 // wavefront scan of the histogram, and a stable counting sort that writes the
 // cell-sorted tiles the pair kernels read.  Compiled with -ffp-contract=off.
+//
+// Particle data live in "home" order h (a permutation of the particle ids kept
+// close to cell order, see spc2_api.hip); hid[h] is the particle id at home h.
+// The cell list itself is the reference's: cells in row-major order, members
+// in ascending particle id.
 #include <hip/hip_runtime.h>
 
 #include "spc2_device.h"
@@ -11,39 +16,70 @@ namespace spc2 {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kScanItems = 4;                     // counts per thread
-constexpr int kScanTile = kBlock * kScanItems;    // counts per workgroup
+constexpr int kScanItems = 4;                   // counts per thread
+constexpr int kScanTile = kBlock * kScanItems;  // counts per workgroup
 
 inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// Histogram increment with one atomic per distinct cell per wavefront: lanes
+// holding the same cell take consecutive arrival ranks from a single
+// atomicAdd issued by the lowest such lane.  Every lane of the wavefront takes
+// part (valid == false for lanes past the end).
+__device__ __forceinline__ int wave_aggregated_rank(int *count, int c, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  bool done = !valid;
+  int rank = 0;
+  while (true) {
+    unsigned long long pending = __ballot(!done);
+    if (pending == 0ull) break;
+    const int leader = __builtin_ctzll(pending);
+    const int lc = __shfl(c, leader, 64);
+    const bool mine = !done && c == lc;
+    const unsigned long long same = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[lc], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (mine) {
+      rank = base + __popcll(same & below);
+      done = true;
+    }
+  }
+  return rank;
+}
 
 // K1: stage s + (k_coef * dt) * k (ChimeraSystem.cpp:274-276, evaluated as
 // (k_coef * dt) * k_prev like the reference); on the linked-cell path the
 // position is wrapped if outside the box and binned (ChimeraSystem.cpp:291-309).
 // `local` is the arrival rank inside the cell, used only to place the particle
-// before the stable re-ranking.
+// before the stable re-ranking by particle id.
 __global__ __launch_bounds__(kBlock) void k_stage_bin(int n, int linked, Box b, int nx, int ny, const double *sx,
                                                       const double *sy, const double *sp, const double *kx,
                                                       const double *ky, const double *kp, double kcdt, double *gx,
                                                       double *gy, double *gp, int *cell_of, int *count, int *local) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double x = sx[i], y = sy[i], p = sp[i];
-  if (kx) {
-    x = x + kcdt * kx[i];
-    y = y + kcdt * ky[i];
-    p = p + kcdt * kp[i];
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = h < n;
+  int c = 0;
+  if (valid) {
+    double x = sx[h], y = sy[h], p = sp[h];
+    if (kx) {
+      x = x + kcdt * kx[h];
+      y = y + kcdt * ky[h];
+      p = p + kcdt * kp[h];
+    }
+    if (linked) {
+      cond_wrap(b, x, y);
+      c = cell_coord(y, b.Ly, ny) * nx + cell_coord(x, b.Lx, nx);
+      cell_of[h] = c;
+    }
+    gx[h] = x;
+    gy[h] = y;
+    gp[h] = p;
   }
   if (linked) {
-    cond_wrap(b, x, y);
-    int cx = cell_coord(x, b.Lx, nx);
-    int cy = cell_coord(y, b.Ly, ny);
-    int c = cy * nx + cx;
-    cell_of[i] = c;
-    local[i] = atomicAdd(&count[c], 1);
+    int r = wave_aggregated_rank(count, c, valid);
+    if (valid) local[h] = r;
   }
-  gx[i] = x;
-  gy[i] = y;
-  gp[i] = p;
 }
 
 // Inclusive scan of one int per thread across a 256-thread workgroup:
@@ -118,33 +154,37 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(int *count, int nc, const
   }
 }
 
-// K3a: place each particle at cell start + arrival rank.
+// K3a: place each particle at cell start + arrival rank (home and id).
 __global__ __launch_bounds__(kBlock) void k_place(int n, const int *cell_of, const int *local, const int *start,
-                                                  int *tmp) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  tmp[start[cell_of[i]] + local[i]] = i;
+                                                  const int *hid, int *tmp, int *tmpid) {
+  int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= n) return;
+  int pos = start[cell_of[h]] + local[h];
+  tmp[pos] = h;
+  tmpid[pos] = hid[h];
 }
 
-// K3b: stable counting sort.  The final position of particle i inside its cell
-// is the number of cell members with a smaller index, which reproduces the
-// reference's ascending push_back order (ChimeraSystem.cpp:335-338).  The same
-// thread gathers i's stage data into the cell-sorted tile arrays.
+// K3b: stable counting sort.  The final position of a particle inside its cell
+// is the number of cell members with a smaller particle id, which reproduces
+// the reference's ascending push_back order (ChimeraSystem.cpp:335-338).  The
+// same thread gathers the particle's stage data into the cell-sorted tiles.
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_rank_tiles(int n, int nx, double hx, double hy, const int *tmp,
-                                                       const int *cell_of, const int *start, const double *gx,
-                                                       const double *gy, const double *gp, int *sorted, Tiles t) {
+                                                       const int *tmpid, const int *cell_of, const int *start,
+                                                       const double *gx, const double *gy, const double *gp,
+                                                       int *slot_h, int *slot_id, Tiles t) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  int i = tmp[k];
-  int c = cell_of[i];
-  int lo = start[c], hi = start[c + 1];
+  const int h = tmp[k], id = tmpid[k];
+  const int c = cell_of[h];
+  const int lo = start[c], hi = start[c + 1];
   int r = 0;
-  for (int m = lo; m < hi; ++m) r += tmp[m] < i;
-  int dst = lo + r;
-  sorted[dst] = i;
+  for (int m = lo; m < hi; ++m) r += tmpid[m] < id;
+  const int dst = lo + r;
+  slot_h[dst] = h;
+  slot_id[dst] = id;
   t.cell[dst] = c;
-  double x = gx[i], y = gy[i], p = gp[i];
+  double x = gx[h], y = gy[h], p = gp[h];
   if (PREC == P32K) {
     int cx = c % nx, cy = c / nx;
     double sn, cs;
@@ -184,23 +224,23 @@ void launch_scan(Sim &s) {
 
 void launch_place(Sim &s) {
   hipLaunchKernelGGL(k_place, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.cell_of, s.local, s.start,
-                     s.tmp);
+                     s.hid, s.tmp, s.tmpid);
 }
 
 void launch_rank_tiles(Sim &s) {
   dim3 g(blocks_for(s.n)), b(kBlock);
   switch (s.prec) {
     case P64:
-      hipLaunchKernelGGL(k_rank_tiles<P64>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.cell_of, s.start,
-                         s.gx, s.gy, s.gp, s.sorted, s.t);
+      hipLaunchKernelGGL(k_rank_tiles<P64>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.tmpid,
+                         s.cell_of, s.start, s.gx, s.gy, s.gp, s.slot_h, s.slot_id, s.t);
       break;
     case P64F:
-      hipLaunchKernelGGL(k_rank_tiles<P64F>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.cell_of,
-                         s.start, s.gx, s.gy, s.gp, s.sorted, s.t);
+      hipLaunchKernelGGL(k_rank_tiles<P64F>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.tmpid,
+                         s.cell_of, s.start, s.gx, s.gy, s.gp, s.slot_h, s.slot_id, s.t);
       break;
     default:
-      hipLaunchKernelGGL(k_rank_tiles<P32K>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.cell_of,
-                         s.start, s.gx, s.gy, s.gp, s.sorted, s.t);
+      hipLaunchKernelGGL(k_rank_tiles<P32K>, g, b, 0, s.stream, s.n, s.nx, s.kp.hx, s.kp.hy, s.tmp, s.tmpid,
+                         s.cell_of, s.start, s.gx, s.gy, s.gp, s.slot_h, s.slot_id, s.t);
   }
 }
 

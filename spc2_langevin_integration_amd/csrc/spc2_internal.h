@@ -15,6 +15,11 @@ namespace spc2 {
 
 enum Prec : int { P64 = 0, P32K = 1, P64F = 2 };
 
+// persistent per-particle double arrays moved by re-homing: x, y, phi, omega, k_1..k_4
+constexpr int kPermuteArrays = 16;
+// slot triples of the pair counters (candidates, pairs within rho, cohesion)
+constexpr int kCountSlots = 1024;
+
 // What a force kernel does with the per-particle result (its "epilogue").
 enum Epi : int {
   EPI_K = 0,          // k-form: write k_next (phi only when a partner was found)
@@ -76,9 +81,22 @@ struct Sim {
   double *der[3] = {}, *add[2] = {};
   // stage positions (wrapped for binning when linked), by particle
   double *gx = nullptr, *gy = nullptr, *gp = nullptr;
-  // cell list
+  // Particle arrays (state, omega, k, derivative, stage) are stored in "home"
+  // order h: hid[h] is the particle id at home h, hpos[id] its home.  Homes are
+  // re-assigned every rehome_every steps to the current cell-sorted order so
+  // that per-particle reads and writes stay coalesced.
+  int *hid = nullptr, *hpos = nullptr;
+  // cell list (per home: cell_of, local; per slot: tmp/tmpid while placing,
+  // then slot_h / slot_id = home / id of the particle in each cell-sorted slot)
   int *cell_of = nullptr, *count = nullptr, *local = nullptr, *start = nullptr, *tmp = nullptr,
-      *sorted = nullptr;
+      *tmpid = nullptr, *slot_h = nullptr, *slot_id = nullptr;
+  bool built = false;  // slot_h holds the cell order of the current homes
+  int rehome_every = 16, steps_since_rehome = 0;
+  // re-homing double buffers (state x, y, phi, omega, k_1..k_4) and id-order
+  // transfer scratch for the ABI's host arrays
+  double *spare[16] = {};
+  int *spare_i = nullptr;
+  double *io[3] = {};
   Tiles t{};
   // pass-1 results kept per slot for pass 2
   double *aphi_slot = nullptr, *n_slot = nullptr;
@@ -135,6 +153,11 @@ void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const Ste
                   double *const k2[3], double *const k3[3]);
 void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
 // spc2_misc.hip
+void launch_to_home(Sim &s, const double *src_id, double *dst_home);
+void launch_from_home(Sim &s, const double *src_home, double *dst_id);
+void launch_from_home_int(Sim &s, const int *src_home, int *dst_id);
+void launch_identity_homes(Sim &s);
+void launch_rehome(Sim &s);
 void launch_order_param(Sim &s, double *out2);
 void launch_init_state(Sim &s, double omega_std);
 void launch_noise(Sim &s, int per_particle, uint64_t step, double *out);

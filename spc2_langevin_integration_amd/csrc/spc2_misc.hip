@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 
 #include "spc2_device.h"
 #include "spc2_internal.h"
@@ -71,18 +72,62 @@ __global__ __launch_bounds__(kBlock) void k_order_final(int nb, const double *re
 // x, y ~ U[0, L), phi ~ U[0, 2 pi), omega ~ N(0, omega_std) from Philox
 // streams 1, 2, 3 at step 0 (replaces SimulationEnginePtr.cpp:31-50).
 __global__ __launch_bounds__(kBlock) void k_init_state(int n, double Lx, double Ly, uint64_t seed, double omega_std,
-                                                       double *sx, double *sy, double *sp, double *omega) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                                                       const int *hid, double *sx, double *sy, double *sp,
+                                                       double *omega) {
+  int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= n) return;
+  const uint32_t i = (uint32_t)hid[h];
   double ux, uy, up, unused;
-  uniform_pair(seed, 0, kStreamInitPos, (uint32_t)i, ux, uy);
-  uniform_pair(seed, 0, kStreamInitPhase, (uint32_t)i, up, unused);
-  sx[i] = ux * Lx;
-  sy[i] = uy * Ly;
-  sp[i] = up * kTwoPi;
+  uniform_pair(seed, 0, kStreamInitPos, i, ux, uy);
+  uniform_pair(seed, 0, kStreamInitPhase, i, up, unused);
+  sx[h] = ux * Lx;
+  sy[h] = uy * Ly;
+  sp[h] = up * kTwoPi;
   double z0, z1;
-  normal_pair(seed, 0, kStreamOmega, (uint32_t)i, z0, z1);
-  omega[i] = z0 * omega_std;
+  normal_pair(seed, 0, kStreamOmega, i, z0, z1);
+  omega[h] = z0 * omega_std;
+}
+
+// id-ordered host arrays <-> home-ordered device arrays
+__global__ __launch_bounds__(kBlock) void k_to_home(int n, const int *hpos, const double *src_id, double *dst_home) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < n) dst_home[hpos[id]] = src_id[id];
+}
+
+__global__ __launch_bounds__(kBlock) void k_from_home(int n, const int *hpos, const double *src_home, double *dst_id) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < n) dst_id[id] = src_home[hpos[id]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_from_home_int(int n, const int *hpos, const int *src_home, int *dst_id) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < n) dst_id[id] = src_home[hpos[id]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(int n, int *a, int *b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = i;
+    b[i] = i;
+  }
+}
+
+// Re-homing: the particle in cell-sorted slot k moves to home k.
+struct PermuteSet {
+  const double *src[kPermuteArrays];
+  double *dst[kPermuteArrays];
+};
+
+__global__ __launch_bounds__(kBlock) void k_permute(int n, const int *slot_h, PermuteSet ps, const int *hid_in,
+                                                    int *hid_out, int *hpos) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int h = slot_h[k];
+#pragma unroll
+  for (int a = 0; a < kPermuteArrays; ++a) ps.dst[a][k] = ps.src[a][h];
+  const int id = hid_in[h];
+  hid_out[k] = id;
+  hpos[id] = k;
 }
 
 __global__ __launch_bounds__(kBlock) void k_noise(int n, int per_particle, uint64_t seed, uint64_t step,
@@ -114,7 +159,41 @@ void launch_order_param(Sim &s, double *out2) {
 
 void launch_init_state(Sim &s, double omega_std) {
   hipLaunchKernelGGL(k_init_state, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.P.Lx, s.P.Ly, s.P.seed,
-                     omega_std, s.sx, s.sy, s.sp, s.omega);
+                     omega_std, s.hid, s.sx, s.sy, s.sp, s.omega);
+}
+
+void launch_to_home(Sim &s, const double *src_id, double *dst_home) {
+  hipLaunchKernelGGL(k_to_home, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_id, dst_home);
+}
+
+void launch_from_home(Sim &s, const double *src_home, double *dst_id) {
+  hipLaunchKernelGGL(k_from_home, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_home, dst_id);
+}
+
+void launch_from_home_int(Sim &s, const int *src_home, int *dst_id) {
+  hipLaunchKernelGGL(k_from_home_int, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_home,
+                     dst_id);
+}
+
+void launch_identity_homes(Sim &s) {
+  hipLaunchKernelGGL(k_iota, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hid, s.hpos);
+}
+
+// Moves every persistent per-particle array to the cell order of the last
+// build (slot k becomes home k) and swaps in the permuted buffers.
+void launch_rehome(Sim &s) {
+  double **arrs[kPermuteArrays] = {&s.sx, &s.sy, &s.sp, &s.omega, &s.k[0][0], &s.k[0][1], &s.k[0][2],
+                                   &s.k[1][0], &s.k[1][1], &s.k[1][2], &s.k[2][0], &s.k[2][1], &s.k[2][2],
+                                   &s.k[3][0], &s.k[3][1], &s.k[3][2]};
+  PermuteSet ps;
+  for (int a = 0; a < kPermuteArrays; ++a) {
+    ps.src[a] = *arrs[a];
+    ps.dst[a] = s.spare[a];
+  }
+  hipLaunchKernelGGL(k_permute, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.slot_h, ps, s.hid,
+                     s.spare_i, s.hpos);
+  for (int a = 0; a < kPermuteArrays; ++a) std::swap(*arrs[a], s.spare[a]);
+  std::swap(s.hid, s.spare_i);
 }
 
 void launch_noise(Sim &s, int per_particle, uint64_t step, double *out) {

@@ -11,6 +11,7 @@
 // particle index inside a cell).  A segment whose halo does not fit the LDS
 // capacity falls back to reading the same tiles from global memory with the
 // same arithmetic and order.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -590,15 +591,15 @@ __device__ __forceinline__ double phi_out(const Pass1Out &o, double old) {
 }
 
 template <int EPI>
-__device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int id, int slot, const Pass1Out &o) {
+__device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int h, int id, int slot, const Pass1Out &o) {
   double sn, cs;
   sincos(o.phase, &sn, &cs);
   double ax = P.v0 * cs, ay = P.v0 * sn;
   if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
-    double ap = phi_out(o, e.sa.zero_in ? 0.0 : e.op[id]);
-    e.ox[id] = ax;
-    e.oy[id] = ay;
-    e.op[id] = ap;
+    double ap = phi_out(o, e.sa.zero_in ? 0.0 : e.op[h]);
+    e.ox[h] = ax;
+    e.oy[h] = ay;
+    e.op[h] = ap;
     if (EPI == EPI_ITO1) {
       e.aphi_slot[slot] = ap;
       e.aphi_slot_f[slot] = (float)ap;
@@ -608,14 +609,14 @@ __device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int
       // RungeKutta4StepperPtr.cpp:72-77: (k1 + 2 k2 + 2 k3 + k4) * dt / 6 for
       // x, y and (...) / 6 * dt for phi, evaluated left to right
       const double dt = e.sa.dt;
-      double x = e.sx[id], y = e.sy[id], p = e.sp[id];
-      x += (e.k1x[id] + 2.0 * e.k2x[id] + 2.0 * e.k3x[id] + ax) * dt / 6.0;
-      y += (e.k1y[id] + 2.0 * e.k2y[id] + 2.0 * e.k3y[id] + ay) * dt / 6.0;
-      p += (e.k1p[id] + 2.0 * e.k2p[id] + 2.0 * e.k3p[id] + ap) / 6.0 * dt;
+      double x = e.sx[h], y = e.sy[h], p = e.sp[h];
+      x += (e.k1x[h] + 2.0 * e.k2x[h] + 2.0 * e.k3x[h] + ax) * dt / 6.0;
+      y += (e.k1y[h] + 2.0 * e.k2y[h] + 2.0 * e.k3y[h] + ay) * dt / 6.0;
+      p += (e.k1p[h] + 2.0 * e.k2p[h] + 2.0 * e.k3p[h] + ap) / 6.0 * dt;
       wrap_state(P, x, y);
-      e.sx[id] = x;
-      e.sy[id] = y;
-      e.sp[id] = p;
+      e.sx[h] = x;
+      e.sy[h] = y;
+      e.sp[h] = p;
     }
   } else if (EPI == EPI_SE) {
     // StochasticEulerStepperPtr.hpp:44-53 with a zero-filled derivative and the
@@ -623,30 +624,30 @@ __device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int
     double ap = phi_out(o, 0.0);
     double xi;
     if (e.sa.noise) {
-      xi = e.sa.noise[id];
+      xi = e.sa.noise[id];  // materialised noise is indexed by particle id
     } else {
       double z1;
       normal_pair(e.sa.seed, e.sa.step, kStreamStep, (uint32_t)id, xi, z1);
     }
     const double dt = e.sa.dt;
     double dW = e.sa.sqdt * xi;
-    double w = e.sa.omega ? e.sa.omega[id] : 0.0;
-    double x = e.sx[id], y = e.sy[id], p = e.sp[id];
+    double w = e.sa.omega ? e.sa.omega[h] : 0.0;
+    double x = e.sx[h], y = e.sy[h], p = e.sp[h];
     x += ax * dt;
     y += ay * dt;
     p += (w + ap) * dt + e.sa.sq2D * dW;
     wrap_state(P, x, y);
-    e.sx[id] = x;
-    e.sy[id] = y;
-    e.sp[id] = p;
+    e.sx[h] = x;
+    e.sy[h] = y;
+    e.sp[h] = p;
   }
 }
 
 template <int EPI>
-__device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int id, const Pass2Out &o) {
+__device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int h, int id, const Pass2Out &o) {
   if (EPI == EPI_ITO2) {
-    e.a0[id] = o.a0;
-    e.a1[id] = o.a1;
+    e.a0[h] = o.a0;
+    e.a1[h] = o.a1;
   } else {
     // StochasticRungeKuttaStepperPtr.hpp:52-66
     double sn, cs;
@@ -662,30 +663,41 @@ __device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int
     const double dt = e.sa.dt, D = e.sa.D, sq2D = e.sa.sq2D;
     double delta_W = U_1 * e.sa.sqdt;
     double delta_Z = e.sa.h1 * (U_1 + U_2 / e.sa.sqrt3);
-    double w = e.sa.omega ? e.sa.omega[id] : 0.0;
-    double x = e.sx[id], y = e.sy[id], p = e.sp[id];
+    double w = e.sa.omega ? e.sa.omega[h] : 0.0;
+    double x = e.sx[h], y = e.sy[h], p = e.sp[h];
     x += ax * dt - sq2D * ay * delta_Z - 0.5 * (ap * ay + D * ax) * dt * dt;
     y += ay * dt + sq2D * ax * delta_Z + 0.5 * (ap * ax - D * ay) * dt * dt;
     p += (w + ap) * dt + sq2D * delta_W + o.a0 * delta_Z + 0.5 * o.a1 * dt * dt;
     wrap_state(P, x, y);
-    e.sx[id] = x;
-    e.sy[id] = y;
-    e.sp[id] = p;
+    e.sx[h] = x;
+    e.sy[h] = y;
+    e.sp[h] = p;
   }
 }
 
+// Pair counters: workgroup totals added to one of kCountSlots slot triples
+// (blockIdx modulo kCountSlots), so that the atomics of a launch spread over
+// many addresses; the host sums the slots.  Called by every thread.
 __device__ __forceinline__ void count_add(unsigned long long *counts, long long a, long long b, long long c) {
-  // wavefront reduction, one atomic per wavefront
+  __shared__ long long part[3][kBlock / 64];
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
     a += __shfl_xor(a, d, 64);
     b += __shfl_xor(b, d, 64);
     c += __shfl_xor(c, d, 64);
   }
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&counts[0], (unsigned long long)a);
-    atomicAdd(&counts[1], (unsigned long long)b);
-    atomicAdd(&counts[2], (unsigned long long)c);
+    part[0][w] = a;
+    part[1][w] = b;
+    part[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    long long v = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) v += part[threadIdx.x][q];
+    atomicAdd(&counts[3 * (blockIdx.x % kCountSlots) + threadIdx.x], (unsigned long long)v);
   }
 }
 
@@ -879,8 +891,8 @@ __device__ __forceinline__ int visit_global(Acc &acc, const S &src, const int *s
 
 template <class Acc, int PREC, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass1_lds(KParams P, const int *__restrict__ start, Tiles t,
-                                                      const int *__restrict__ sorted, const double *gp, EpiCtx e,
-                                                      unsigned long long *counts) {
+                                                      const int *__restrict__ slot_h, const int *__restrict__ slot_id,
+                                                      const double *gp, EpiCtx e, unsigned long long *counts) {
   __shared__ SegShared sh;
   extern __shared__ float4 dyn_lds[];
   Seg g = seg_setup(P, start, sh);
@@ -915,10 +927,10 @@ __global__ __launch_bounds__(kBlock) void k_pass1_lds(KParams P, const int *__re
       acc.init(gs, k, k, t, P);
       cand = visit_global(acc, gs, start, t.cell[k], k, P);
     }
-    const int id = sorted[k];
+    const int h = slot_h[k], id = slot_id[k];
     Pass1Out o{};
-    acc.finish(P, PREC == P32K ? gp[id] : 0.0, o);
-    epilogue1<EPI>(P, e, id, k, o);
+    acc.finish(P, PREC == P32K ? gp[h] : 0.0, o);
+    epilogue1<EPI>(P, e, h, id, k, o);
     cand_sum += cand;
     n_sum += o.n;
     n2_sum += o.n2;
@@ -928,7 +940,8 @@ __global__ __launch_bounds__(kBlock) void k_pass1_lds(KParams P, const int *__re
 
 template <int PREC, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__restrict__ start, Tiles t,
-                                                      const int *__restrict__ sorted, const double *gp, EpiCtx e) {
+                                                      const int *__restrict__ slot_h, const int *__restrict__ slot_id,
+                                                      const double *gp, EpiCtx e) {
   __shared__ SegShared sh;
   extern __shared__ float4 dyn_lds[];
   Seg g = seg_setup(P, start, sh);
@@ -946,11 +959,11 @@ __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__re
   const int k0 = sh.gst[g.ncol + 1];
   for (int p = threadIdx.x; p < ncen; p += kBlock) {
     const int k = k0 + p, ei = e0 + p;
-    const int id = sorted[k];
+    const int h = slot_h[k], id = slot_id[k];
     double in0 = 0.0, in1 = 0.0;
     if (EPI == EPI_ITO2) {
-      in0 = e.a0[id];
-      in1 = e.a1[id];
+      in0 = e.a0[h];
+      in1 = e.a1[h];
     }
     Chim2Acc<PREC> acc;
     if (staged) {
@@ -968,8 +981,8 @@ __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__re
       visit_global(acc, gs, start, t.cell[k], k, P);
     }
     Pass2Out o;
-    acc.finish(P, PREC == P32K ? gp[id] : 0.0, o);
-    epilogue2<EPI>(P, e, id, o);
+    acc.finish(P, PREC == P32K ? gp[h] : 0.0, o);
+    epilogue2<EPI>(P, e, h, id, o);
   }
 }
 
@@ -1081,7 +1094,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_allpairs(KParams P, const doub
   Pass1Out o{};
   if (active) {
     allpairs_pass1<PREC, SYS>(P, i, gx, gy, gp, o);
-    epilogue1<EPI>(P, e, i, i, o);
+    epilogue1<EPI>(P, e, i, i, i, o);  // all pairs: home order is the id order
   }
   if (counts) count_add(counts, active ? P.n - 1 : 0, active ? o.n : 0, active ? o.n2 : 0);
 }
@@ -1115,7 +1128,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2_allpairs(KParams P, const doub
   o.ai = ai;
   o.a0 = a0 * -sqrt(2.0 * P.D);
   o.a1 = a1 + -2.0 * P.D * ai;
-  epilogue2<EPI>(P, e, i, o);
+  epilogue2<EPI>(P, e, i, i, o);
 }
 
 // =============================================================================
@@ -1156,6 +1169,21 @@ size_t lds_bytes(const Sim &s) {
   return (size_t)s.kp.lds_cap * LdsTile<PREC, PASS>::bytes;
 }
 
+// Launches a pair kernel on the handle's stream; with profiling on, the HIP
+// events are bound to the kernel's own dispatch (hipExtLaunchKernelGGL), so
+// their elapsed time is the kernel's execution time.
+template <typename... KArgs, typename... Args>
+void launch_pair(Sim &s, void (*kern)(KArgs...), dim3 grid, dim3 blk, size_t sh, Args... args) {
+  if (s.profiling) {
+    hipEvent_t b, e;
+    s.prof_take(b, e);
+    hipExtLaunchKernelGGL(kern, grid, blk, sh, s.stream, b, e, 0, args...);
+    s.prof_pending.emplace_back(b, e);
+  } else {
+    hipLaunchKernelGGL(kern, grid, blk, sh, s.stream, args...);
+  }
+}
+
 template <int PREC, int SYS, int EPI>
 void pass1_typed(Sim &s, const EpiCtx &e) {
   unsigned long long *cnt = s.counting ? s.counts : nullptr;
@@ -1163,14 +1191,14 @@ void pass1_typed(Sim &s, const EpiCtx &e) {
     const dim3 grid(s.ny * s.kp.segs), blk(kBlock);
     const size_t sh = lds_bytes<PREC, 1>(s);
     if (SYS == SPC2_CHIMERA)
-      hipLaunchKernelGGL((k_pass1_lds<ChimAcc<PREC>, PREC, EPI>), grid, blk, sh, s.stream, s.kp, s.start, s.t,
-                         s.sorted, s.gp, e, cnt);
+      launch_pair(s, k_pass1_lds<ChimAcc<PREC>, PREC, EPI>, grid, blk, sh, s.kp, (const int *)s.start, s.t,
+                  (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e, cnt);
     else
-      hipLaunchKernelGGL((k_pass1_lds<VortAcc<PREC>, PREC, EPI>), grid, blk, sh, s.stream, s.kp, s.start, s.t,
-                         s.sorted, s.gp, e, cnt);
+      launch_pair(s, k_pass1_lds<VortAcc<PREC>, PREC, EPI>, grid, blk, sh, s.kp, (const int *)s.start, s.t,
+                  (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e, cnt);
   } else {
-    hipLaunchKernelGGL((k_pass1_allpairs<PREC, SYS, EPI>), dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp,
-                       s.gx, s.gy, s.gp, e, cnt);
+    launch_pair(s, k_pass1_allpairs<PREC, SYS, EPI>, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.kp,
+                (const double *)s.gx, (const double *)s.gy, (const double *)s.gp, e, cnt);
   }
 }
 
@@ -1196,11 +1224,12 @@ template <int PREC, int EPI>
 void pass2_typed(Sim &s, const EpiCtx &e) {
   if (s.linked) {
     const size_t sh = lds_bytes<PREC, 2>(s);
-    hipLaunchKernelGGL((k_pass2_lds<PREC, EPI>), dim3(s.ny * s.kp.segs), dim3(kBlock), sh, s.stream, s.kp, s.start,
-                       s.t, s.sorted, s.gp, e);
-  } else
-    hipLaunchKernelGGL((k_pass2_allpairs<EPI>), dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, s.gx, s.gy,
-                       s.gp, e);
+    launch_pair(s, k_pass2_lds<PREC, EPI>, dim3(s.ny * s.kp.segs), dim3(kBlock), sh, s.kp, (const int *)s.start, s.t,
+                (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e);
+  } else {
+    launch_pair(s, k_pass2_allpairs<EPI>, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.kp, (const double *)s.gx,
+                (const double *)s.gy, (const double *)s.gp, e);
+  }
 }
 
 template <int PREC>
@@ -1210,24 +1239,6 @@ void pass2_epi(Sim &s, int epi, const EpiCtx &e) {
   else
     pass2_typed<PREC, EPI_SRK>(s, e);
 }
-
-// HIP events around each pair-kernel launch on the handle's stream; the
-// elapsed times are collected lazily (spc2_get_profile) so that the host does
-// not wait on the device per launch.
-struct ProfScope {
-  Sim &s;
-  hipEvent_t b = nullptr, en = nullptr;
-  explicit ProfScope(Sim &sim) : s(sim) {
-    if (!s.profiling) return;
-    s.prof_take(b, en);
-    (void)hipEventRecord(b, s.stream);
-  }
-  ~ProfScope() {
-    if (!s.profiling) return;
-    (void)hipEventRecord(en, s.stream);
-    s.prof_pending.emplace_back(b, en);
-  }
-};
 
 }  // namespace
 
@@ -1259,7 +1270,6 @@ void configure_segments(Sim &s) {
 void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
                   double *const k2[3], double *const k3[3]) {
   EpiCtx e = make_ctx(s, ox, oy, op, sa, k1, k2, k3);
-  ProfScope prof(s);
   switch (s.prec) {
     case P64: pass1_sys<P64>(s, epi, e); break;
     case P64F: pass1_sys<P64F>(s, epi, e); break;
@@ -1271,7 +1281,6 @@ void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa) {
   EpiCtx e = make_ctx(s, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
   e.a0 = a0;
   e.a1 = a1;
-  ProfScope prof(s);
   switch (s.prec) {
     case P64: pass2_epi<P64>(s, epi, e); break;
     case P64F: pass2_epi<P64F>(s, epi, e); break;

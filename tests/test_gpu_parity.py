@@ -275,8 +275,35 @@ def test_chimera_steps_with_reference_noise(gpu, golden, method, name, prec):
             assert_state(s.get_state(), ref, before, L64, 1e2 * TOL[prec], f"{name}/{prec} step {k}")
 
 
-def test_rk4_stale_phase_semantics(gpu):
+@pytest.mark.parametrize("method", ["se", "srk", "rk4"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+def test_rehoming_is_invisible(gpu, method, prec, monkeypatch):
+    """Particle arrays are moved to cell order every few steps; the moves must
+    not change a single bit of the trajectory, the cell list or the k buffers."""
+    n, rho, dt = 30000, 0.1, 0.01
+    L = math.sqrt(n / 4000.0)
+    st = O.make_state(9, 0, n, L, rho)
+    outs = []
+    for every in ("1", "1000000"):
+        monkeypatch.setenv("SPC2_REHOME_EVERY", every)
+        with spc2.Sim(chim_params(n, L, rho, 0.05, 1.0, 1.45, 0.05, prec, seed=3)) as s:
+            s.set_state(st)
+            s.step(method, dt, 0, 3)
+            s.set_state(s.get_state())          # state round trip through the id order
+            s.step(method, dt, 3, 3)
+            co, cs, so = s.cell_list()
+            outs.append((s.get_state(), co, cs, so, s.eval_rhs(np.zeros(3 * n), 0.0, dt, np.zeros(3 * n))))
+    monkeypatch.delenv("SPC2_REHOME_EVERY", raising=False)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    rco, rcs, rso, _ = O.cell_list(O.chimera(n, L, rho), outs[0][0])
+    assert np.array_equal(outs[0][2], rcs) and np.array_equal(outs[0][3], rso) and np.array_equal(outs[0][1], rco)
+
+
+@pytest.mark.parametrize("every", ["1", "16"])
+def test_rk4_stale_phase_semantics(gpu, every, monkeypatch):
     # rho = 1 at density 2: isolated particles keep k_phi from earlier stages/steps
+    monkeypatch.setenv("SPC2_REHOME_EVERY", every)
     n, L, rho, dt = N8, L64, 1.0, 0.05
     sysp = O.chimera(n, L, rho, 0.5, 1.0, 1.45, 0.0, threads=8)
     st = O.make_state(15, 0, n, L, rho)
