@@ -123,6 +123,26 @@ spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *addition
  * (PeriodicBoundaryConditions.cpp:133-141). */
 spc2_status spc2_step(spc2_sim_t h, int32_t method, double dt, int64_t step0, int64_t nsteps, const double *noise);
 
+/* spc2_step with flags.  SPC2_STEP_NO_WRAP leaves the positions unwrapped, as
+ * the reference's DoStep does (StochasticRungeKuttaStepperPtr.hpp:41-68): the
+ * engine then applies ApplyPeriodicBoundaryConditions itself
+ * (SimulationEnginePtr.cpp:111).  Used by the drop-in stepper classes. */
+enum { SPC2_STEP_NO_WRAP = 1 };
+spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0, int64_t nsteps, const double *noise,
+                         int32_t flags);
+
+/* Asynchronous trajectory frame: the state converted to float AoS [x, y, phi]
+ * by particle id (3N floats, the payload of a BinaryObserverPtr::SaveSystemState
+ * frame, BinaryObserverPtr.cpp:163-168) copied into `dst` on the handle's stream.
+ * dst should come from spc2_host_alloc (pinned) for the copy to overlap.
+ * spc2_snapshot_wait blocks until the last snapshot has landed. */
+spc2_status spc2_snapshot_async(spc2_sim_t h, float *dst);
+spc2_status spc2_snapshot_wait(spc2_sim_t h);
+
+/* Pinned host memory for snapshot rings. */
+spc2_status spc2_host_alloc(uint64_t bytes, void **out);
+spc2_status spc2_host_free(void *p);
+
 /* The Philox normals spc2_step would draw at global step `step`: N (SE) or 2N
  * (SRK) doubles.  Replaces the stepper's std::normal_distribution draws
  * (StochasticRungeKuttaStepperPtr.hpp:49-56). */

@@ -65,7 +65,8 @@ void free_all(Sim &s) {
   for (auto &p : s.io) f(p);
   f(s.t.x); f(s.t.y); f(s.t.p); f(s.t.c); f(s.t.s); f(s.t.f4); f(s.t.cell);
   f(s.aphi_slot); f(s.n_slot); f(s.aphi_slot_f);
-  f(s.noise); f(s.counts); f(s.red); f(s.bsum);
+  f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
+  if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
 
 void fill_kparams(Sim &s) {
@@ -225,6 +226,7 @@ StepArgs step_args(const Sim &s, double dt, uint64_t step) {
   a.noise = nullptr;
   a.omega = s.omega;
   a.zero_in = 0;
+  a.wrap = 1;
   return a;
 }
 
@@ -400,7 +402,13 @@ spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *addition
 }
 
 spc2_status spc2_step(spc2_sim_t h, int32_t method, double dt, int64_t step0, int64_t nsteps, const double *noise) {
+  return spc2_step_ex(h, method, dt, step0, nsteps, noise, 0);
+}
+
+spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0, int64_t nsteps, const double *noise,
+                         int32_t flags) {
   if (check_handle(h)) return SPC2_ERR_INVALID;
+  if (flags & ~SPC2_STEP_NO_WRAP) return fail(SPC2_ERR_INVALID, "unknown step flags");
   Sim &s = h->s;
   if (method < SPC2_SE || method > SPC2_RK4) return fail(SPC2_ERR_INVALID, "unknown method");
   if (!(dt > 0.0) || nsteps < 0 || step0 < 0) return fail(SPC2_ERR_INVALID, "dt must be > 0, steps >= 0");
@@ -420,6 +428,7 @@ spc2_status spc2_step(spc2_sim_t h, int32_t method, double dt, int64_t step0, in
   }
   for (int64_t k = 0; k < nsteps; ++k) {
     StepArgs sa = step_args(s, dt, (uint64_t)(step0 + k));
+    sa.wrap = (flags & SPC2_STEP_NO_WRAP) ? 0 : 1;
     if (noise && method != SPC2_RK4) sa.noise = s.noise + (size_t)k * npp * (size_t)s.n;
     if (method == SPC2_SE) {
       build(s, nullptr, 0.0);
@@ -538,6 +547,41 @@ spc2_status spc2_get_profile(spc2_sim_t h, double *pair_kernel_ms, int64_t *laun
   h->s.prof_collect();
   *pair_kernel_ms = h->s.prof_ms;
   *launches = h->s.prof_launches;
+  return SPC2_OK;
+}
+
+spc2_status spc2_snapshot_async(spc2_sim_t h, float *dst) {
+  if (check_handle(h) || !dst) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  if (!s.snap) HIP_TRY(dalloc(&s.snap, 3 * (size_t)s.n));
+  if (!s.snap_ev) HIP_TRY(hipEventCreateWithFlags(&s.snap_ev, hipEventDisableTiming));
+  launch_snapshot(s, s.snap);
+  if (spc2_status st = check_launch("snapshot")) return st;
+  HIP_TRY(hipMemcpyAsync(dst, s.snap, sizeof(float) * 3 * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipEventRecord(s.snap_ev, s.stream));
+  s.snap_pending = true;
+  return SPC2_OK;
+}
+
+spc2_status spc2_snapshot_wait(spc2_sim_t h) {
+  if (check_handle(h)) return SPC2_ERR_INVALID;
+  Sim &s = h->s;
+  if (s.snap_pending) {
+    HIP_TRY(hipEventSynchronize(s.snap_ev));
+    s.snap_pending = false;
+  }
+  return SPC2_OK;
+}
+
+spc2_status spc2_host_alloc(uint64_t bytes, void **out) {
+  if (!out) return fail(SPC2_ERR_INVALID, "null argument");
+  *out = nullptr;
+  HIP_TRY(hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault));
+  return SPC2_OK;
+}
+
+spc2_status spc2_host_free(void *p) {
+  if (p) HIP_TRY(hipHostFree(p));
   return SPC2_OK;
 }
 

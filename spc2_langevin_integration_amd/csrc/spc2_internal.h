@@ -62,6 +62,7 @@ struct StepArgs {
   const double *noise;  // optional materialised noise for this step (by particle)
   const double *omega;  // natural frequencies (chimera) or null
   int zero_in;          // 1: the phi output starts from 0 (zero-filled derivative), else in/out
+  int wrap;             // 1: apply the engine's periodic wrap after the update
 };
 
 struct Sim {
@@ -107,6 +108,10 @@ struct Sim {
   // counts: candidates, interactions, cohesion pairs
   unsigned long long *counts = nullptr;
   bool counting = false;
+  // float frame buffer for asynchronous snapshots and its completion event
+  float *snap = nullptr;
+  hipEvent_t snap_ev = nullptr;
+  bool snap_pending = false;
   // scan workgroup totals, reduction scratch
   int *bsum = nullptr;
   double *red = nullptr;
@@ -157,6 +162,7 @@ void launch_to_home(Sim &s, const double *src_id, double *dst_home);
 void launch_from_home(Sim &s, const double *src_home, double *dst_id);
 void launch_from_home_int(Sim &s, const int *src_home, int *dst_id);
 void launch_identity_homes(Sim &s);
+void launch_snapshot(Sim &s, float *out);
 void launch_rehome(Sim &s);
 void launch_order_param(Sim &s, double *out2);
 void launch_init_state(Sim &s, double omega_std);

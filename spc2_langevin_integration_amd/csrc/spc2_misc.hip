@@ -104,6 +104,18 @@ __global__ __launch_bounds__(kBlock) void k_from_home_int(int n, const int *hpos
   if (id < n) dst_id[id] = src_home[hpos[id]];
 }
 
+// Trajectory frame payload: float AoS [x, y, phi] by particle id
+// (BinaryObserverPtr.cpp:163-168 writes the float copy of the double state).
+__global__ __launch_bounds__(kBlock) void k_snapshot(int n, const int *hpos, const double *sx, const double *sy,
+                                                     const double *sp, float *out) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int h = hpos[id];
+  out[3 * (size_t)id] = (float)sx[h];
+  out[3 * (size_t)id + 1] = (float)sy[h];
+  out[3 * (size_t)id + 2] = (float)sp[h];
+}
+
 __global__ __launch_bounds__(kBlock) void k_iota(int n, int *a, int *b) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -173,6 +185,11 @@ void launch_from_home(Sim &s, const double *src_home, double *dst_id) {
 void launch_from_home_int(Sim &s, const int *src_home, int *dst_id) {
   hipLaunchKernelGGL(k_from_home_int, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_home,
                      dst_id);
+}
+
+void launch_snapshot(Sim &s, float *out) {
+  hipLaunchKernelGGL(k_snapshot, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, s.sx, s.sy, s.sp,
+                     out);
 }
 
 void launch_identity_homes(Sim &s) {

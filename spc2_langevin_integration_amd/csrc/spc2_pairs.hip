@@ -613,7 +613,7 @@ __device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int
       x += (e.k1x[h] + 2.0 * e.k2x[h] + 2.0 * e.k3x[h] + ax) * dt / 6.0;
       y += (e.k1y[h] + 2.0 * e.k2y[h] + 2.0 * e.k3y[h] + ay) * dt / 6.0;
       p += (e.k1p[h] + 2.0 * e.k2p[h] + 2.0 * e.k3p[h] + ap) / 6.0 * dt;
-      wrap_state(P, x, y);
+      if (e.sa.wrap) wrap_state(P, x, y);
       e.sx[h] = x;
       e.sy[h] = y;
       e.sp[h] = p;
@@ -636,7 +636,7 @@ __device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int
     x += ax * dt;
     y += ay * dt;
     p += (w + ap) * dt + e.sa.sq2D * dW;
-    wrap_state(P, x, y);
+    if (e.sa.wrap) wrap_state(P, x, y);
     e.sx[h] = x;
     e.sy[h] = y;
     e.sp[h] = p;
@@ -668,7 +668,7 @@ __device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int
     x += ax * dt - sq2D * ay * delta_Z - 0.5 * (ap * ay + D * ax) * dt * dt;
     y += ay * dt + sq2D * ax * delta_Z + 0.5 * (ap * ax - D * ay) * dt * dt;
     p += (w + ap) * dt + sq2D * delta_W + o.a0 * delta_Z + 0.5 * o.a1 * dt * dt;
-    wrap_state(P, x, y);
+    if (e.sa.wrap) wrap_state(P, x, y);
     e.sx[h] = x;
     e.sy[h] = y;
     e.sp[h] = p;
