@@ -107,8 +107,8 @@ spc2_status spc2_eval_rhs(spc2_sim_t h, const double *k_prev, double k_coef, dou
 
 /* Ito-form right-hand side: derivative (3N, in/out) and additional (2N, in/out,
  * [i][0], [i][1] per particle).  Replaces ChimeraSystemPtr::EvaluateRhs(state,
- * derivative, additional_derivative, dt) (ChimeraSystemPtr.cpp:131-144).  The
- * vortex-arrays Ito form returns SPC2_ERR_UNSUPPORTED in this build. */
+ * derivative, additional_derivative, dt) (ChimeraSystemPtr.cpp:131-144) and
+ * VortexArraysSystemPtr::EvaluateRhs (VortexArraysSystemPtr.cpp:122-135). */
 spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *additional);
 
 /* Advances the device-resident state nsteps steps of `method` with time step

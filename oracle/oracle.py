@@ -7,6 +7,7 @@ never by the product package.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -68,6 +69,7 @@ def lib():
         L.orc_init_state.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_double, dp, dp]
         L.orc_mt_state_size.restype = C.c_size_t
         L.orc_mt_seed.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_mt_uniform.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_double, dp]
         L.orc_mt_normals.argtypes = [C.c_void_p, C.c_int64, dp]
         for f in ("orc_cell_list", "orc_rhs_k", "orc_rhs_ito", "orc_step_se", "orc_step_srk", "orc_step_rk4"):
             getattr(L, f).restype = C.c_int
@@ -234,3 +236,17 @@ class MT19937Normals:
         out = np.empty(count, np.float64)
         lib().orc_mt_normals(self._buf, count, out)
         return out
+
+    def uniform(self, count, lo, hi):
+        out = np.empty(count, np.float64)
+        lib().orc_mt_uniform(self._buf, count, C.c_double(lo), C.c_double(hi), out)
+        return out
+
+    def reference_initial_state(self, n, L):
+        """x, y ~ U[0, L), phi ~ U[0, 2 pi) drawn per particle in that order
+        (SimulationEnginePtr.cpp:36-45)."""
+        st = np.empty(3 * n, np.float64)
+        for i in range(n):
+            st[3 * i:3 * i + 2] = self.uniform(2, 0.0, L)
+            st[3 * i + 2] = self.uniform(1, 0.0, 2.0 * math.pi)[0]
+        return st

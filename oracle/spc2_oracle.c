@@ -884,6 +884,14 @@ static double mt_canonical(mt_state *m) {
   return r;
 }
 
+/* std::uniform_real_distribution<double>(lo, hi): canonical * (hi - lo) + lo,
+ * the draws of SimulationEnginePtr::InitializeRandomSystemState
+ * (SimulationEnginePtr.cpp:36-45). */
+void orc_mt_uniform(void *vp, int64_t count, double lo, double hi, double *out) {
+  mt_state *m = (mt_state *)vp;
+  for (int64_t k = 0; k < count; ++k) out[k] = mt_canonical(m) * (hi - lo) + lo;
+}
+
 /* `count` draws from a fresh distribution (empty pair cache). */
 void orc_mt_normals(void *vp, int64_t count, double *out) {
   mt_state *m = (mt_state *)vp;

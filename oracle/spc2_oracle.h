@@ -68,6 +68,7 @@ void orc_init_state(uint64_t seed, int64_t n, double Lx, double Ly, double omega
 size_t orc_mt_state_size(void);
 void orc_mt_seed(void *st, uint32_t seed);
 void orc_mt_normals(void *st, int64_t count, double *out);
+void orc_mt_uniform(void *st, int64_t count, double lo, double hi, double *out);
 
 #ifdef __cplusplus
 }

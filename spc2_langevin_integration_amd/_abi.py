@@ -18,6 +18,7 @@ HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "spc2_hip.h")
 CHIMERA, VORTEX = 0, 1
 FP64, FP32_KAHAN, FP64_FACTORED = 0, 1, 2
 SE, SRK, RK4 = 0, 1, 2
+STEP_NO_WRAP = 1
 NEIGH_AUTO, NEIGH_CELLS, NEIGH_ALL_PAIRS = 0, 1, 2
 
 PRECISIONS = {"fp64": FP64, "fp32k": FP32_KAHAN, "fp64f": FP64_FACTORED}
@@ -65,6 +66,11 @@ def load():
         "spc2_eval_rhs": (st, [h, dp, C.c_double, C.c_double, dp]),
         "spc2_eval_rhs_ito": (st, [h, dp, dp]),
         "spc2_step": (st, [h, C.c_int32, C.c_double, C.c_int64, C.c_int64, dp]),
+        "spc2_step_ex": (st, [h, C.c_int32, C.c_double, C.c_int64, C.c_int64, dp, C.c_int32]),
+        "spc2_snapshot_async": (st, [h, C.c_void_p]),
+        "spc2_snapshot_wait": (st, [h]),
+        "spc2_host_alloc": (st, [C.c_uint64, C.POINTER(C.c_void_p)]),
+        "spc2_host_free": (st, [C.c_void_p]),
         "spc2_noise": (st, [h, C.c_int32, C.c_int64, dp]),
         "spc2_order_parameter": (st, [h, dp, dp]),
         "spc2_get_cell_list": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -175,12 +181,27 @@ class Sim:
         _check(load().spc2_eval_rhs_ito(self.h, _dp(d), _dp(a)))
         return d, a
 
-    def step(self, method, dt, step0=0, nsteps=1, noise=None):
+    def step(self, method, dt, step0=0, nsteps=1, noise=None, wrap=True):
         m = METHODS[method] if isinstance(method, str) else int(method)
         nz = None
         if noise is not None:
             nz = np.ascontiguousarray(noise, np.float64)
-        _check(load().spc2_step(self.h, m, dt, step0, nsteps, _dp(nz) if nz is not None else None))
+        flags = 0 if wrap else STEP_NO_WRAP
+        _check(load().spc2_step_ex(self.h, m, dt, step0, nsteps, _dp(nz) if nz is not None else None, flags))
+
+    def snapshot(self, buf=None):
+        """Float frame [x, y, phi] by particle id through the asynchronous
+        snapshot path; `buf` may be a PinnedFrame to reuse."""
+        own = buf is None
+        if own:
+            buf = PinnedFrame(3 * self.n)
+        try:
+            _check(load().spc2_snapshot_async(self.h, buf.ptr))
+            _check(load().spc2_snapshot_wait(self.h))
+            return buf.array().copy()
+        finally:
+            if own:
+                buf.free()
 
     def noise(self, method, step):
         m = METHODS[method] if isinstance(method, str) else int(method)
@@ -224,6 +245,24 @@ class Sim:
 
     def synchronize(self):
         _check(load().spc2_synchronize(self.h))
+
+
+class PinnedFrame:
+    """Pinned host buffer of `count` floats from spc2_host_alloc."""
+
+    def __init__(self, count):
+        self.count = int(count)
+        p = C.c_void_p()
+        _check(load().spc2_host_alloc(4 * self.count, C.byref(p)))
+        self.ptr = p
+
+    def array(self):
+        return np.ctypeslib.as_array(C.cast(self.ptr, C.POINTER(C.c_float)), shape=(self.count,))
+
+    def free(self):
+        if self.ptr:
+            load().spc2_host_free(self.ptr)
+            self.ptr = None
 
 
 def declared_symbols(header=HEADER_PATH):

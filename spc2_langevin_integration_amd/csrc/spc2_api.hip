@@ -64,7 +64,7 @@ void free_all(Sim &s) {
   for (auto &p : s.spare) f(p);
   for (auto &p : s.io) f(p);
   f(s.t.x); f(s.t.y); f(s.t.p); f(s.t.c); f(s.t.s); f(s.t.f4); f(s.t.cell);
-  f(s.aphi_slot); f(s.n_slot); f(s.aphi_slot_f);
+  f(s.aphi_slot); f(s.n_slot); f(s.n2_slot); f(s.aphi_slot_f);
   f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
@@ -107,10 +107,15 @@ void fill_kparams(Sim &s) {
   k.mu_p_over_xa2f = P.xi_a > 0 ? (float)(P.mu_plus / (P.xi_a * P.xi_a)) : 0.0f;
   k.c4f = P.rho > P.xi_a ? (float)(-P.mu_minus * 4.0 / ((P.rho - P.xi_a) * (P.rho - P.xi_a))) : 0.0f;
   k.kappaf = (float)P.kappa;
+  k.v0f = (float)P.v0;
+  k.Df = (float)P.D;
   // FP32 r^2 from cell-local coordinates is within a few 1e-7 relative of the
   // FP64 value; pairs closer than 1e-5 relative to a cutoff take the FP64 test.
   k.band_rho2f = (float)(1e-5 * k.rho2);
   k.band_xr2f = (float)(1e-5 * P.xi_r * P.xi_r);
+  k.band_xa2f = (float)(1e-5 * P.xi_a * P.xi_a);
+  const double close = 1e-2 * std::min(k.hx, k.hy);
+  k.close2f = (float)(close * close);
 }
 
 spc2_status alloc_all(Sim &s) {
@@ -137,7 +142,7 @@ spc2_status alloc_all(Sim &s) {
       A(s.t.c, n); A(s.t.s, n);
     }
   }
-  A(s.aphi_slot, n); A(s.n_slot, n); A(s.aphi_slot_f, n);
+  A(s.aphi_slot, n); A(s.n_slot, n); A(s.n2_slot, n); A(s.aphi_slot_f, n);
   A(s.counts, 3 * kCountSlots); A(s.red, 2 * 1024 + 8); A(s.bsum, (size_t)scan_blocks(std::max(s.nc, 1)) + 1);
 #undef A
   HIP_TRY(hipMemset(s.count, 0, sizeof(int) * nc));
@@ -224,7 +229,9 @@ StepArgs step_args(const Sim &s, double dt, uint64_t step) {
   a.seed = s.P.seed;
   a.step = step;
   a.noise = nullptr;
-  a.omega = s.omega;
+  // natural frequencies enter the chimera drift only (the vortex-arrays
+  // steppers have none, StochasticRungeKuttaStepperPtr.hpp:94)
+  a.omega = s.P.system == SPC2_CHIMERA ? s.omega : nullptr;
   a.zero_in = 0;
   a.wrap = 1;
   return a;
@@ -388,8 +395,6 @@ spc2_status spc2_eval_rhs(spc2_sim_t h, const double *k_prev, double k_coef, dou
 spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *additional) {
   if (check_handle(h) || !derivative || !additional) return fail(SPC2_ERR_INVALID, "null argument");
   Sim &s = h->s;
-  if (s.P.system != SPC2_CHIMERA)
-    return fail(SPC2_ERR_UNSUPPORTED, "the vortex-arrays Ito form (second pass) is not provided by this build");
   if (spc2_status st = upload_aos(s, derivative, 3, s.der, 3)) return st;
   if (spc2_status st = upload_aos(s, additional, 2, s.add, 2)) return st;
   build(s, nullptr, 0.0);
@@ -412,8 +417,6 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
   Sim &s = h->s;
   if (method < SPC2_SE || method > SPC2_RK4) return fail(SPC2_ERR_INVALID, "unknown method");
   if (!(dt > 0.0) || nsteps < 0 || step0 < 0) return fail(SPC2_ERR_INVALID, "dt must be > 0, steps >= 0");
-  if (method == SPC2_SRK && s.P.system != SPC2_CHIMERA)
-    return fail(SPC2_ERR_UNSUPPORTED, "SRK for the vortex-arrays system needs its Ito second pass (not provided)");
   const int npp = method == SPC2_SRK ? 2 : 1;
   if (noise && method != SPC2_RK4) {
     size_t need = (size_t)nsteps * npp * (size_t)s.n;

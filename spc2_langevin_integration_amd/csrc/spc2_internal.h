@@ -41,9 +41,11 @@ struct KParams {
   // vortex
   double mu_plus, mu_minus, xi_a, xi_r, kappa;
   double outer2;  // max(rho, xi_r)^2: widest test radius
-  float xa2f, xr2f, outer2f, xaf, rhof, mu_pf, mu_p_over_xa2f, c4f, kappaf;
+  float xa2f, xr2f, outer2f, xaf, rhof, mu_pf, mu_p_over_xa2f, c4f, kappaf, v0f, Df;
   // FP32 cutoff band: |r^2 - cutoff^2| <= band * cutoff^2 is re-decided in FP64
-  float band_rho2f, band_xr2f;
+  float band_rho2f, band_xr2f, band_xa2f;
+  // FP32 cohesion pairs closer than this (r^2) take FP64 geometry
+  float close2f;
   // pair-kernel decomposition: row segments of seg_w cells, segs per row,
   // LDS capacity in tile elements per workgroup
   int seg_w, segs, lds_cap;
@@ -100,7 +102,7 @@ struct Sim {
   double *io[3] = {};
   Tiles t{};
   // pass-1 results kept per slot for pass 2
-  double *aphi_slot = nullptr, *n_slot = nullptr;
+  double *aphi_slot = nullptr, *n_slot = nullptr, *n2_slot = nullptr;
   float *aphi_slot_f = nullptr;
   // materialised noise (optional)
   double *noise = nullptr;

@@ -59,6 +59,13 @@ __device__ __forceinline__ double vortex_w(const KParams &P, double r) {
   return 0.0;
 }
 
+// AlignmentStrengthDerivative, VortexArraysSystemPtr.cpp:751-763
+__device__ __forceinline__ double vortex_dw(const KParams &P, double r) {
+  if (r < P.xi_a) return P.mu_plus * (-2.0 * r / P.xi_a / P.xi_a);
+  if (r < P.rho) return -P.mu_minus * 4.0 * (-2.0 * r + P.rho + P.xi_a) / (P.rho - P.xi_a) / (P.rho - P.xi_a);
+  return 0.0;
+}
+
 // FP64 distance^2 of slots k and m with the reference min-image: decides the
 // rare FP32 pairs that fall inside a cutoff band.
 __device__ __forceinline__ double exact_r2(const double *tx, const double *ty, int k, int m, double Lx, double rLx,
@@ -66,6 +73,16 @@ __device__ __forceinline__ double exact_r2(const double *tx, const double *ty, i
   double dx = min_image(tx[m] - tx[k], Lx, rLx);
   double dy = min_image(ty[m] - ty[k], Ly, rLy);
   return dx * dx + dy * dy;
+}
+
+// FP64 min-image separation of slots k and m, for cohesion pairs closer than
+// a hundredth of a cell: there the 1/r and 1/r^2 factors would amplify the
+// FP32 rounding of the cell-local coordinates (about 6e-8 of a cell width)
+// beyond 1e-6 of the term.
+__device__ __forceinline__ void exact_d(const double *tx, const double *ty, int k, int m, const KParams &P, double &dx,
+                                        double &dy) {
+  dx = min_image(tx[m] - tx[k], P.Lx, P.rLx);
+  dy = min_image(ty[m] - ty[k], P.Ly, P.rLy);
 }
 
 // Per-particle result of pass 1.
@@ -383,12 +400,20 @@ struct VortAcc<P32K> {
     CW.add(w * v.z);
     n01 += inr;
     if (inc) {
-      if (r2 > 0.0f) {
+      if (r2 >= P.close2f) {
         float ir = __builtin_amdgcn_rsqf(r2);
         Tx.add(dx * ir);
         Ty.add(dy * ir);
       } else {
-        Tx.add(1.0f);  // atan2(+0, +0) = 0
+        double ddx, ddy;
+        exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
+        double dr = sqrt(ddx * ddx + ddy * ddy);
+        if (dr > 0.0) {
+          Tx.add((float)(ddx / dr));
+          Ty.add((float)(ddy / dr));
+        } else {
+          Tx.add(1.0f);  // atan2(+0, +0) = 0
+        }
       }
       ++n2;
     }
@@ -422,21 +447,26 @@ struct Pass2Out {
   double phase, a0, a1, ai;
 };
 
+// Per-particle pass-2 inputs: pass-1 phi derivative, partner counts (within
+// rho; cohesion), and the additional-derivative values before the call.
+struct Pass2In {
+  double ai, n, n2, in0, in1;
+};
+
 template <int PREC>
 struct Chim2Acc;
 
 template <>
 struct Chim2Acc<P64> {
   double xi, yi, pi, ai, n, a0, a1;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &, double a_i, double n_i, double in0,
-                       double in1) {
+  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &, const Pass2In &in) {
     xi = s.x[e];
     yi = s.y[e];
     pi = s.p[e];
-    ai = a_i;
-    n = n_i;
-    a0 = in0;
-    a1 = in1;
+    ai = in.ai;
+    n = in.n;
+    a0 = in.in0;
+    a1 = in.in1;
   }
   __device__ __forceinline__ void row(int, int, int, const KParams &) {}
   __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
@@ -460,15 +490,14 @@ struct Chim2Acc<P64> {
 template <>
 struct Chim2Acc<P64F> {
   double xi, yi, pi, ai, n, in0, in1, C, S, Pc, Qs;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &, double a_i, double n_i,
-                       double i0, double i1) {
+  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &, const Pass2In &in) {
     xi = s.x[e];
     yi = s.y[e];
     pi = t.p[k];
-    ai = a_i;
-    n = n_i;
-    in0 = i0;
-    in1 = i1;
+    ai = in.ai;
+    n = in.n;
+    in0 = in.in0;
+    in1 = in.in1;
     C = S = Pc = Qs = 0.0;
   }
   __device__ __forceinline__ void row(int, int, int, const KParams &) {}
@@ -508,16 +537,15 @@ struct Chim2Acc<P32K> {
   Kahan C, S, Pc, Qs;
   int k;
   const double *tx, *ty;
-  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P, double a_i, double n_i,
-                       double i0, double i1) {
+  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P, const Pass2In &in) {
     float4 v = s.f4[e];
     xl = v.x;
     yl = v.y;
     co.set(xl, P.hxf);
-    ai = a_i;
-    n = n_i;
-    in0 = i0;
-    in1 = i1;
+    ai = in.ai;
+    n = in.n;
+    in0 = in.in0;
+    in1 = in.in1;
     C.init();
     S.init();
     Pc.init();
@@ -562,6 +590,230 @@ struct Chim2Acc<P32K> {
 };
 
 // =============================================================================
+// Vortex-arrays pass 2 (Ito form, VortexArraysSystemPtr.cpp:662-733): over the
+// partners within rho, -W cos(dphi) / n01 and the drift derivative of
+// W(r) sin(dphi); over the partners within xi_r, the cohesion counterparts;
+// additional[0] is scaled by sqrt(2 D) at the end.  Velocities are
+// v (cos phi, sin phi) of the stage phases, as the pass-1 derivative holds them.
+// =============================================================================
+template <int PREC>
+struct Vort2Acc;
+
+// FP64 reference form
+template <>
+struct Vort2Acc<P64> {
+  double xi, yi, pi, vxi, vyi, ai, n1, n2, a0, a1;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &P, const Pass2In &in) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = s.p[e];
+    vxi = P.v0 * cos(pi);
+    vyi = P.v0 * sin(pi);
+    ai = in.ai;
+    n1 = in.n;
+    n2 = in.n2;
+    a0 = in.in0;
+    a1 = in.in1;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 > P.outer2 * 1.000000000001) return;
+    double dr = sqrt(r2);
+    const bool al = (dr <= P.rho) && (n1 > 0.0), co = (dr <= P.xi_r) && (n2 > 0.0);
+    if (!al && !co) return;
+    const double pj = s.p[m];
+    const double vxj = P.v0 * cos(pj), vyj = P.v0 * sin(pj);
+    if (al) {
+      double cph = cos(pj - pi);
+      double sph = sin(pj - pi);
+      double w = vortex_w(P, dr);
+      double dw = vortex_dw(P, dr);
+      a0 += -w * cph / n1;
+      a1 += ((vxj - vxi) * dw * dx / dr + (vyj - vyi) * dw * dy / dr) * sph / n1 + (s.aj[m] - ai) * w * cph / n1 -
+            2.0 * P.D * sph * w / n1;
+    }
+    if (co) {
+      double th = atan2(dy, dx);
+      double cpp = cos(th - pi);
+      double spp = sin(th - pi);
+      a0 += P.kappa * cpp / n2;
+      a1 += -P.kappa * ((vxj - vxi) * (-dy) / dr / dr + (vyj - vyi) * dx / dr / dr - ai) * cpp / n2 +
+            P.kappa * P.D * spp / n2;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
+    o.phase = pi;
+    o.ai = ai;
+    o.a0 = a0 * sqrt(2.0 * P.D);
+    o.a1 = a1;
+  }
+};
+
+// Factored FP64 form: cos, sin of (phi_j - phi_i) and of (atan2(dy, dx) - phi_i)
+// from the tile's cos phi, sin phi and dx / r, dy / r, no per-pair trig; the
+// sums are divided by the counts once.
+template <>
+struct Vort2Acc<P64F> {
+  double xi, yi, pi, ci, si, ai, n1, n2, in0, in1, A0, A1, C0, C1;
+  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &, const Pass2In &in) {
+    xi = s.x[e];
+    yi = s.y[e];
+    pi = t.p[k];
+    ci = s.c[e];
+    si = s.s[e];
+    ai = in.ai;
+    n1 = in.n;
+    n2 = in.n2;
+    in0 = in.in0;
+    in1 = in.in1;
+    A0 = A1 = C0 = C1 = 0.0;
+  }
+  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
+  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
+    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
+    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 > P.outer2 * 1.000000000001) return;
+    double dr = sqrt(r2);
+    const bool al = dr <= P.rho, co = dr <= P.xi_r;
+    if (!al && !co) return;
+    const double cj = s.c[m], sj = s.s[m];
+    const double dvx = P.v0 * cj - P.v0 * ci, dvy = P.v0 * sj - P.v0 * si;
+    if (al) {
+      double cph = cj * ci + sj * si;
+      double sph = sj * ci - cj * si;
+      double w = vortex_w(P, dr);
+      double g = vortex_dw(P, dr) / dr;
+      A0 += w * cph;
+      A1 += (dvx * dx + dvy * dy) * g * sph + (s.aj[m] - ai) * w * cph - 2.0 * P.D * sph * w;
+    }
+    if (co) {
+      // atan2(+0, +0) = 0 for a coincident partner
+      double cpp = dr > 0.0 ? (dx * ci + dy * si) / dr : ci;
+      double spp = dr > 0.0 ? (dy * ci - dx * si) / dr : -si;
+      C0 += cpp;
+      C1 += -P.kappa * ((dvy * dx - dvx * dy) / r2 - ai) * cpp + P.kappa * P.D * spp;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
+    double a0 = in0, a1 = in1;
+    if (n1 > 0.0) {
+      a0 += -A0 / n1;
+      a1 += A1 / n1;
+    }
+    if (n2 > 0.0) {
+      a0 += P.kappa * C0 / n2;
+      a1 += C1 / n2;
+    }
+    o.phase = pi;
+    o.ai = ai;
+    o.a0 = a0 * sqrt(2.0 * P.D);
+    o.a1 = a1;
+  }
+};
+
+// FP32 form over cell-local coordinates with Kahan sums; cutoff decisions
+// (rho, xi_r, and the xi_a branch of W') within the band are taken in FP64.
+template <>
+struct Vort2Acc<P32K> {
+  float xl, yl, cif, sif, aif;
+  ColOffsets co;
+  double ai, n1, n2, in0, in1, C0, C1;
+  Kahan A0, A1;
+  int k;
+  const double *tx, *ty;
+  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P, const Pass2In &in) {
+    float4 v = s.f4[e];
+    xl = v.x;
+    yl = v.y;
+    cif = v.z;
+    sif = v.w;
+    co.set(xl, P.hxf);
+    ai = in.ai;
+    aif = (float)in.ai;
+    n1 = in.n;
+    n2 = in.n2;
+    in0 = in.in0;
+    in1 = in.in1;
+    A0.init();
+    A1.init();
+    C0 = C1 = 0.0;
+    k = kk;
+    tx = t.x;
+    ty = t.y;
+  }
+  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
+    co.oy = (float)dyo * P.hyf - yl;
+    co.b1 = b1;
+    co.b2 = b2;
+  }
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    float4 v = s.f4[m];
+    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+    float r2 = __builtin_fmaf(dx, dx, dy * dy);
+    bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f, inner = r2 < P.xa2f;
+    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f || __builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f ||
+        __builtin_fabsf(r2 - P.xa2f) <= P.band_xa2f) {
+      double d = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy));
+      inr = d <= P.rho;
+      inc = d <= P.xi_r;
+      inner = d < P.xi_a;
+    }
+    if (!inr && !inc) return;
+    const float cj = v.z, sj = v.w;
+    const float dvx = P.v0f * cj - P.v0f * cif, dvy = P.v0f * sj - P.v0f * sif;
+    const float ir = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    if (inr) {
+      float r = r2 * ir;
+      float cph = cj * cif + sj * sif;
+      float sph = sj * cif - cj * sif;
+      float w = inner ? __builtin_fmaf(-P.mu_p_over_xa2f, r2, P.mu_pf) : P.c4f * (r - P.xaf) * (P.rhof - r);
+      // W'(r) / r
+      float g = inner ? -2.0f * P.mu_p_over_xa2f : P.c4f * (P.rhof + P.xaf - 2.0f * r) * ir;
+      float aj = s.aj[m];
+      A0.add(w * cph);
+      A1.add((dvx * dx + dvy * dy) * g * sph + (aj - aif) * w * cph - 2.0f * P.Df * sph * w);
+    }
+    if (inc) {
+      double cpp, spp, q;
+      if (r2 >= P.close2f) {
+        cpp = (dx * cif + dy * sif) * ir;
+        spp = (dy * cif - dx * sif) * ir;
+        q = (dvy * dx - dvx * dy) * ir * ir;
+      } else {
+        double ddx, ddy;
+        exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
+        const double dr2 = ddx * ddx + ddy * ddy, dr = sqrt(dr2);
+        const double cid = cif, sid = sif;
+        cpp = dr > 0.0 ? (ddx * cid + ddy * sid) / dr : cid;
+        spp = dr > 0.0 ? (ddy * cid - ddx * sid) / dr : -sid;
+        q = ((double)dvy * ddx - (double)dvx * ddy) / dr2;
+      }
+      C0 += cpp;
+      C1 += -P.kappa * (q - ai) * cpp + P.kappa * P.D * spp;
+    }
+  }
+  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass2Out &o) const {
+    double a0 = in0, a1 = in1;
+    if (n1 > 0.0) {
+      a0 += -A0.value() / n1;
+      a1 += A1.value() / n1;
+    }
+    if (n2 > 0.0) {
+      a0 += P.kappa * C0 / n2;
+      a1 += C1 / n2;
+    }
+    o.phase = phase;
+    o.ai = ai;
+    o.a0 = a0 * sqrt(2.0 * P.D);
+    o.a1 = a1;
+  }
+};
+
+// =============================================================================
 // Epilogues
 // =============================================================================
 struct EpiCtx {
@@ -571,6 +823,7 @@ struct EpiCtx {
   const double *k3x, *k3y, *k3p;
   double *sx, *sy, *sp;           // master state by particle
   double *aphi_slot, *n_slot;     // pass-1 results by slot (pass-2 inputs)
+  double *n2_slot;                // vortex cohesion partner counts by slot
   float *aphi_slot_f;
   double *a0, *a1;                // additional by particle (in/out)
   StepArgs sa;
@@ -604,6 +857,7 @@ __device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int
       e.aphi_slot[slot] = ap;
       e.aphi_slot_f[slot] = (float)ap;
       e.n_slot[slot] = (double)o.n;
+      e.n2_slot[slot] = (double)o.n2;
     }
     if (EPI == EPI_RK4_FINAL) {
       // RungeKutta4StepperPtr.cpp:72-77: (k1 + 2 k2 + 2 k3 + k4) * dt / 6 for
@@ -938,7 +1192,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_lds(KParams P, const int *__re
   if (counts) count_add(counts, cand_sum, n_sum, n2_sum);
 }
 
-template <int PREC, int EPI>
+template <class Acc, int PREC, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__restrict__ start, Tiles t,
                                                       const int *__restrict__ slot_h, const int *__restrict__ slot_id,
                                                       const double *gp, EpiCtx e) {
@@ -960,12 +1214,12 @@ __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__re
   for (int p = threadIdx.x; p < ncen; p += kBlock) {
     const int k = k0 + p, ei = e0 + p;
     const int h = slot_h[k], id = slot_id[k];
-    double in0 = 0.0, in1 = 0.0;
+    Pass2In in{e.aphi_slot[k], e.n_slot[k], e.n2_slot[k], 0.0, 0.0};
     if (EPI == EPI_ITO2) {
-      in0 = e.a0[h];
-      in1 = e.a1[h];
+      in.in0 = e.a0[h];
+      in.in1 = e.a1[h];
     }
-    Chim2Acc<PREC> acc;
+    Acc acc;
     if (staged) {
       int lo = 1, hi = g.W;
       while (lo < hi) {
@@ -973,11 +1227,11 @@ __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__re
         if (sh.off[g.ncol + mid] <= ei) lo = mid;
         else hi = mid - 1;
       }
-      acc.init(lt.src(), ei, k, t, P, e.aphi_slot[k], e.n_slot[k], in0, in1);
+      acc.init(lt.src(), ei, k, t, P, in);
       visit_lds(acc, lt.src(), sh, g.ncol, lo, ei, P);
     } else {
       auto gs = global_src<PREC>(t, e, 2);
-      acc.init(gs, k, k, t, P, e.aphi_slot[k], e.n_slot[k], in0, in1);
+      acc.init(gs, k, k, t, P, in);
       visit_global(acc, gs, start, t.cell[k], k, P);
     }
     Pass2Out o;
@@ -1099,7 +1353,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_allpairs(KParams P, const doub
   if (counts) count_add(counts, active ? P.n - 1 : 0, active ? o.n : 0, active ? o.n2 : 0);
 }
 
-template <int EPI>
+template <int SYS, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass2_allpairs(KParams P, const double *gx, const double *gy,
                                                            const double *gp, EpiCtx e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1110,6 +1364,45 @@ __global__ __launch_bounds__(kBlock) void k_pass2_allpairs(KParams P, const doub
   if (EPI == EPI_ITO2) {
     a0 = e.a0[i];
     a1 = e.a1[i];
+  }
+  if (SYS == SPC2_VORTEX) {
+    // VortexArraysSystemPtr.cpp:305-353 (FP64 form for every precision)
+    const double n2 = e.n2_slot[i];
+    const double vxi = P.v0 * cos(pi), vyi = P.v0 * sin(pi);
+    for (int j = 0; j < P.n; ++j) {
+      if (j == i) continue;
+      double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+      double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+      double dr = sqrt(dx * dx + dy * dy);
+      const bool al = (dr <= P.rho) && (n > 0.0), co = (dr <= P.xi_r) && (n2 > 0.0);
+      if (!al && !co) continue;
+      const double pj = gp[j];
+      const double vxj = P.v0 * cos(pj), vyj = P.v0 * sin(pj);
+      if (al) {
+        double cph = cos(pj - pi);
+        double sph = sin(pj - pi);
+        double w = vortex_w(P, dr);
+        double dw = vortex_dw(P, dr);
+        a0 += -w * cph / n;
+        a1 += ((vxj - vxi) * dw * dx / dr + (vyj - vyi) * dw * dy / dr) * sph / n + (e.aphi_slot[j] - ai) * w * cph / n -
+              2.0 * P.D * sph * w / n;
+      }
+      if (co) {
+        double th = atan2(dy, dx);
+        double cpp = cos(th - pi);
+        double spp = sin(th - pi);
+        a0 += P.kappa * cpp / n2;
+        a1 += -P.kappa * ((vxj - vxi) * (-dy) / dr / dr + (vyj - vyi) * dx / dr / dr - ai) * cpp / n2 +
+              P.kappa * P.D * spp / n2;
+      }
+    }
+    Pass2Out o;
+    o.phase = pi;
+    o.ai = ai;
+    o.a0 = a0 * sqrt(2.0 * P.D);
+    o.a1 = a1;
+    epilogue2<EPI>(P, e, i, i, o);
+    return;
   }
   // ChimeraSystem.cpp:223-254 (FP64 form for every precision: small-N path)
   for (int j = 0; j < P.n; ++j) {
@@ -1157,6 +1450,7 @@ EpiCtx make_ctx(Sim &s, double *ox, double *oy, double *op, const StepArgs &sa, 
   e.sp = s.sp;
   e.aphi_slot = s.aphi_slot;
   e.n_slot = s.n_slot;
+  e.n2_slot = s.n2_slot;
   e.aphi_slot_f = s.aphi_slot_f;
   e.a0 = s.add[0];
   e.a1 = s.add[1];
@@ -1220,24 +1514,37 @@ void pass1_sys(Sim &s, int epi, const EpiCtx &e) {
     pass1_epi<PREC, SPC2_VORTEX>(s, epi, e);
 }
 
-template <int PREC, int EPI>
+template <int PREC, int SYS, int EPI>
 void pass2_typed(Sim &s, const EpiCtx &e) {
   if (s.linked) {
     const size_t sh = lds_bytes<PREC, 2>(s);
-    launch_pair(s, k_pass2_lds<PREC, EPI>, dim3(s.ny * s.kp.segs), dim3(kBlock), sh, s.kp, (const int *)s.start, s.t,
-                (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e);
+    const dim3 grid(s.ny * s.kp.segs), blk(kBlock);
+    if (SYS == SPC2_CHIMERA)
+      launch_pair(s, k_pass2_lds<Chim2Acc<PREC>, PREC, EPI>, grid, blk, sh, s.kp, (const int *)s.start, s.t,
+                  (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e);
+    else
+      launch_pair(s, k_pass2_lds<Vort2Acc<PREC>, PREC, EPI>, grid, blk, sh, s.kp, (const int *)s.start, s.t,
+                  (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e);
   } else {
-    launch_pair(s, k_pass2_allpairs<EPI>, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.kp, (const double *)s.gx,
+    launch_pair(s, k_pass2_allpairs<SYS, EPI>, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.kp, (const double *)s.gx,
                 (const double *)s.gy, (const double *)s.gp, e);
   }
 }
 
-template <int PREC>
+template <int PREC, int SYS>
 void pass2_epi(Sim &s, int epi, const EpiCtx &e) {
   if (epi == EPI_ITO2)
-    pass2_typed<PREC, EPI_ITO2>(s, e);
+    pass2_typed<PREC, SYS, EPI_ITO2>(s, e);
   else
-    pass2_typed<PREC, EPI_SRK>(s, e);
+    pass2_typed<PREC, SYS, EPI_SRK>(s, e);
+}
+
+template <int PREC>
+void pass2_sys(Sim &s, int epi, const EpiCtx &e) {
+  if (s.P.system == SPC2_CHIMERA)
+    pass2_epi<PREC, SPC2_CHIMERA>(s, epi, e);
+  else
+    pass2_epi<PREC, SPC2_VORTEX>(s, epi, e);
 }
 
 }  // namespace
@@ -1282,9 +1589,9 @@ void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa) {
   e.a0 = a0;
   e.a1 = a1;
   switch (s.prec) {
-    case P64: pass2_epi<P64>(s, epi, e); break;
-    case P64F: pass2_epi<P64F>(s, epi, e); break;
-    default: pass2_epi<P32K>(s, epi, e);
+    case P64: pass2_sys<P64>(s, epi, e); break;
+    case P64F: pass2_sys<P64F>(s, epi, e); break;
+    default: pass2_sys<P32K>(s, epi, e);
   }
 }
 

@@ -218,6 +218,67 @@ def test_vortex_rhs_k(gpu, prec, n, mode, kc):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
+@pytest.mark.parametrize("n,mode,init", [(40000, 0, 0), (40000, 1, 1), (300, 0, 1)])
+def test_vortex_rhs_ito(gpu, prec, n, mode, init):
+    """Both passes of the vortex Ito form; bars relative to the largest
+    reference value of each output (the pass-2 sums cancel heavily)."""
+    rho, D = 0.05, 0.02
+    L = math.sqrt(n / 4000.0)   # n = 300: all pairs
+    st = O.make_state(41 + n, mode, n, L, rho)
+    d0 = O.uniform(90, 3 * n, -1.0, 1.0) if init else np.zeros(3 * n)
+    a0 = O.uniform(91, 2 * n, -1.0, 1.0) if init else np.zeros(2 * n)
+    rd, ra, rc = O.rhs_ito(orc_vortex(n, L, D=D), st, d0, a0)
+    with spc2.Sim(vort_params(n, L, precision=prec, D=D)) as s:
+        s.set_state(st)
+        s.set_counting(True)
+        gd, ga = s.eval_rhs_ito(d0, a0)
+        cnt = s.counts()
+    tol = TOL[prec]
+    assert_rhs(gd, rd, 0.025, 100.0 + 20.0, tol, f"vortex ito/{prec} derivative")
+    for c, name in ((0, "add0"), (1, "add1")):
+        scale = np.abs(ra[c::2]).max()
+        err = np.abs(ga[c::2] - ra[c::2]).max()
+        assert err <= tol * scale, f"vortex ito/{prec} {name}: {err:.3e} > {tol} x {scale:.3e}"
+    assert np.array_equal(cnt[1:], rc[1:]) and rc[2] > 0, (cnt, rc)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
+def test_vortex_srk_steps_with_given_noise(gpu, prec):
+    """StochasticRungeKuttaStepperPtr::DoStep on the vortex system
+    (StochasticRungeKuttaStepperPtr.hpp:70-97): one step from the oracle's state
+    each time, same normals."""
+    n, rho, D, dt = 40000, 0.05, 0.01, 0.01
+    L = math.sqrt(n / 4000.0)
+    sysp = orc_vortex(n, L, D=D, threads=8)
+    st = O.make_state(12, 0, n, L, rho)
+    mt = O.MT19937Normals(2024)
+    with spc2.Sim(vort_params(n, L, precision=prec, D=D)) as s:
+        ref = st
+        for k in range(3):
+            before = ref
+            xi = mt.normals(2 * n)
+            ref = O.step_srk(sysp, before, dt, xi)
+            s.set_state(before)
+            s.step("srk", dt, step0=k, nsteps=1, noise=xi)
+            assert_state(s.get_state(), ref, before, L, 1e2 * TOL[prec], f"vortex srk/{prec} step {k}")
+
+
+def test_vortex_srk_philox_trajectory_matches_oracle(gpu):
+    n, rho, D, dt, steps = 20000, 0.05, 0.01, 0.002, 5
+    L = math.sqrt(n / 4000.0)
+    sysp = orc_vortex(n, L, D=D, threads=8)
+    st, _ = O.init_state(7, n, L, L, 0.0)
+    ref = st
+    for k in range(steps):
+        ref = O.step_srk(sysp, ref, dt, O.philox_normals(7, k, 0, n, 2))
+    with spc2.Sim(vort_params(n, L, D=D, seed=7)) as s:
+        s.init_random_state(0.0)
+        s.step("srk", dt, 0, steps)
+        g = s.get_state()
+    assert np.abs(periodic_diff(g, ref, L)).max() < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
 @pytest.mark.parametrize("cap", ["0", "300"])
 def test_lds_overflow_fallback_matches(gpu, prec, cap, monkeypatch):
     """Segments whose halo exceeds the LDS capacity read the same tiles from

@@ -1,0 +1,60 @@
+"""The C++ drop-in host layer (libspc2_host.so, spc2_run) without a GPU: it is
+built, links only the C ABI, exports the reference's class interfaces and
+rejects bad configurations before touching the device."""
+import os
+import subprocess
+
+import pytest
+
+PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "spc2_langevin_integration_amd")
+HOST_LIB = os.path.join(PKG, "libspc2_host.so")
+RUNNER = os.path.join(PKG, "spc2_run")
+
+
+@pytest.fixture(scope="module")
+def built():
+    if not (os.path.exists(HOST_LIB) and os.path.exists(RUNNER)):
+        subprocess.run(["make", "-C", os.path.join(PKG, "csrc"), "-j8"], check=True, capture_output=True)
+    return True
+
+
+def test_host_library_exports_reference_interfaces(built):
+    out = subprocess.run(["nm", "-DC", "--defined-only", HOST_LIB], check=True, capture_output=True, text=True).stdout
+    for sym in [
+        "PeriodicBoundaryConditions::ClassAEffectiveParticleDistance(double, double, double, double, double&, double&) const",
+        "PeriodicBoundaryConditions::ApplyPeriodicBoundaryConditions(double*, long) const",
+        "ChimeraSystem::EvaluateRhs(double const*, std::vector<double, std::allocator<double> > const&, "
+        "std::vector<double, std::allocator<double> >&, double, double)",
+        "VortexArraysSystem::EvaluateRhs(double const*, std::vector<double, std::allocator<double> >&",
+        "StochasticEulerStepper::DoStep(ChimeraSystem&, double*, double, double)",
+        "StochasticRungeKuttaStepper::DoStep(VortexArraysSystem&, double*, double, double)",
+        "RungeKutta4Stepper::DoStep(VortexArraysSystem&, double*, double, double)",
+        "BinaryObserver::SaveSystemState(double*, long, double)",
+        "BinaryObserver::SaveSummaryStatistics(double const*, long, double)",
+        "SimulationEngine::RunChimeraSimulation()",
+        "SimulationEngine::RunVortexArraySimulation()",
+    ]:
+        assert sym in out, sym
+
+
+def test_host_library_links_only_the_c_abi(built):
+    """The host layer reaches the GPU only through libspc2_hip.so."""
+    out = subprocess.run(["nm", "-D", "--undefined-only", HOST_LIB], check=True, capture_output=True, text=True).stdout
+    names = [line.split()[-1].split("@")[0] for line in out.splitlines() if line.strip()]
+    assert "spc2_step_ex" in names and "spc2_snapshot_async" in names
+    assert not [n for n in names if n.startswith("hip")], "host layer calls the HIP runtime directly"
+
+
+def test_runner_help_and_config_errors(built, tmp_path):
+    r = subprocess.run([RUNNER, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "system method precision" in r.stdout
+    r = subprocess.run([RUNNER, "bogus=1"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unknown configuration key 'bogus'" in r.stderr
+    r = subprocess.run([RUNNER, "n"], capture_output=True, text=True)
+    assert r.returncode == 1 and "expected key=value" in r.stderr
+    cfg = tmp_path / "run.cfg"
+    cfg.write_text("# comment\nsystem=vortex   method=srk\nmethod=euler\n")
+    r = subprocess.run([RUNNER, f"config={cfg}"], capture_output=True, text=True)
+    assert r.returncode == 1 and "method must be se, srk or rk4" in r.stderr
+    r = subprocess.run([RUNNER, "loop=both"], capture_output=True, text=True)
+    assert r.returncode == 1 and "loop must be device or host" in r.stderr

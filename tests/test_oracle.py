@@ -317,6 +317,57 @@ def test_vortex_translation_invariance():
     np.testing.assert_allclose(a, b, atol=1e-10)
 
 
+def _phi_drift(s, st):
+    d, _, cnt = O.rhs_ito(s, st)
+    return d[2::3], cnt
+
+
+@pytest.mark.parametrize("rho", [0.2, 0.3])  # linked cells (rho <= L/4) and all pairs
+def test_vortex_ito_terms_are_generator_derivatives(rho):
+    """The vortex pass-2 terms are the Ito-Taylor terms of the pass-1 drift a_i
+    with the neighbour counts held fixed:
+        additional[i][0] = sqrt(2 D) da_i/dphi_i
+        additional[i][1] = sum_j grad_j a_i . f_j + D sum_j d^2 a_i / dphi_j^2
+    with f the pass-1 derivative (the drift of every coordinate).  Checked by
+    central differences of the oracle's own pass 1, which the brute-force and
+    closed-form tests above pin: a dropped term, a wrong factor (e.g. the 2 D
+    that collects the diffusion of both phases of a pair) or a sign fails."""
+    n, Lb, D = 90, 1.0, 0.05
+    s = O.vortex(n, Lb, rho=rho, velocity=0.1, mu_plus=100.0, mu_minus=80.0, xi_a=0.45 * rho, xi_r=0.3 * rho,
+                 kappa=20.0, D=D)
+    st = O.make_state(5, 0, n, Lb, rho)
+    d, add, cnt = O.rhs_ito(s, st)
+    a, _ = _phi_drift(s, st)
+    assert cnt[1] > 3 * n and cnt[2] > n // 4  # alignment and cohesion both exercised
+    scale0 = np.abs(add[0::2]).max()
+    scale1 = np.abs(add[1::2]).max()
+
+    # L^1 a_i: first differences in phi_i
+    h = 1e-6
+    dphi = np.empty(n)
+    for i in range(n):
+        sp, sm = st.copy(), st.copy()
+        sp[3 * i + 2] += h
+        sm[3 * i + 2] -= h
+        dphi[i] = (_phi_drift(s, sp)[0][i] - _phi_drift(s, sm)[0][i]) / (2 * h)
+    np.testing.assert_allclose(add[0::2], math.sqrt(2 * D) * dphi, rtol=0, atol=1e-7 * scale0)
+
+    # L^0 a_i: directional derivative along the drift + D x phase Laplacian
+    eps = 1e-6
+    ap, cp = _phi_drift(s, st + eps * d)
+    am, cm = _phi_drift(s, st - eps * d)
+    assert (cp == cnt).all() and (cm == cnt).all()  # neighbour sets unchanged
+    directional = (ap - am) / (2 * eps)
+    h2 = 1e-4
+    lap = np.zeros(n)
+    for j in range(n):
+        sp, sm = st.copy(), st.copy()
+        sp[3 * j + 2] += h2
+        sm[3 * j + 2] -= h2
+        lap += (_phi_drift(s, sp)[0] + _phi_drift(s, sm)[0] - 2 * a) / (h2 * h2)
+    np.testing.assert_allclose(add[1::2], directional + D * lap, rtol=0, atol=1e-6 * scale1)
+
+
 # ---- steppers: closed forms of the Ptr (active-build) operation order --------
 
 def test_se_ptr_form_closed_form():
@@ -396,6 +447,24 @@ def test_normal_transform_matches_libm_box_muller():
         r = math.sqrt(-2.0 * math.log(u1))
         ref[i] = (r * math.cos(2 * math.pi * u2), r * math.sin(2 * math.pi * u2))
     np.testing.assert_allclose(z, ref, rtol=0, atol=4e-15 * np.maximum(1, np.abs(ref)).max())
+
+
+def test_mt19937_uniform_draws_known_answers():
+    """std::uniform_real_distribution over std::mt19937, as the reference's
+    InitializeRandomSystemState draws (SimulationEnginePtr.cpp:36-45):
+    generate_canonical<double, 53> combines two 32-bit words as
+    (w0 + w1 2^32) / 2^64.  The raw words come from numpy's legacy
+    RandomState(seed), which seeds mt19937 as std::mt19937(seed) does; its
+    10000th word at the default seed 5489 is the value the C++ standard
+    requires of mt19937 (4123659995)."""
+    w = np.random.RandomState(5489).randint(0, 2 ** 32, size=10000, dtype=np.uint32)
+    assert int(w[-1]) == 4123659995
+    for seed in (5489, 42, 2 ** 31 + 7):
+        words = np.random.RandomState(seed).randint(0, 2 ** 32, size=600, dtype=np.uint32).astype(np.float64)
+        canon = (words[0::2] + words[1::2] * 2.0 ** 32) / 2.0 ** 64
+        lo, hi = -0.5, 3.25
+        got = O.MT19937Normals(seed).uniform(300, lo, hi)
+        assert np.array_equal(got, canon * (hi - lo) + lo)
 
 
 def test_normal_transform_statistics():
