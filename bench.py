@@ -5,11 +5,13 @@ form, with the pair kernel's roofline position and the CPU oracle baseline.
 Workload: vortex arrays, deterministic RK4 (D = 0), dt = 0.01, N = 1,048,576 in a
 periodic box at density 4000 (L = sqrt(N / 4000) = 16.19086), rho = 0.05,
 xi_a = 0.01, xi_r = 0.005, v = 0.025, mu+ = mu- = 100, kappa = 20; uniform
-random positions and phases (Philox, seed 42).  Linked cells rebuilt at every
-RHS evaluation, as the reference does.  Inputs are generated on the device and
-stay resident; a step is one full RK4 step (4 RHS evaluations + combine + wrap).
+random positions and phases (Philox, seed 42).  Neighbours from a Verlet list
+with skin 0.2 rho, rebuilt before any particle can have moved skin / 2
+(--neighbour cells: the reference's linked cells rebuilt at every RHS).  Inputs
+are generated on the device and stay resident; a step is one full RK4 step
+(4 RHS evaluations + combine + wrap, plus the list rebuilds that fall in it).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--neighbour verlet|cells]
 """
 from __future__ import annotations
 
@@ -39,8 +41,8 @@ def workload(n=1048576):
                 kappa=20.0, D=0.0, dt=0.01, seed=42)
 
 
-def make_sim(spc2, w, precision):
-    p = spc2.params("vortex", precision, n=w["n"], L=w["L"], rho=w["rho"], velocity=w["velocity"],
+def make_sim(spc2, w, precision, neighbour="verlet"):
+    p = spc2.params("vortex", precision, neighbour, n=w["n"], L=w["L"], rho=w["rho"], velocity=w["velocity"],
                     mu_plus=w["mu_plus"], mu_minus=w["mu_minus"], xi_a=w["xi_a"], xi_r=w["xi_r"],
                     kappa=w["kappa"], D=w["D"], seed=w["seed"])
     s = spc2.Sim(p)
@@ -48,23 +50,27 @@ def make_sim(spc2, w, precision):
     return s
 
 
-def time_precision(spc2, w, precision, steps, warmup):
-    s = make_sim(spc2, w, precision)
+def time_precision(spc2, w, precision, steps, warmup, neighbour):
+    s = make_sim(spc2, w, precision, neighbour)
     s.step("rk4", w["dt"], 0, warmup)
     s.synchronize()
+    b0 = s.verlet_stats()[0] if neighbour == "verlet" else 0
     t0 = time.perf_counter()
     s.step("rk4", w["dt"], warmup, steps)
     s.synchronize()
     t1 = time.perf_counter()
+    builds = (s.verlet_stats()[0] - b0) if neighbour == "verlet" else None
     R, _ = s.order_parameter()
+    phases = s.get_state()[2::3]
     s.close()
     sec = t1 - t0
-    return {"particle_steps_per_s": w["n"] * steps / sec, "ms_per_step": 1e3 * sec / steps, "R_final": R}
+    return {"particle_steps_per_s": w["n"] * steps / sec, "ms_per_step": 1e3 * sec / steps, "R_final": R,
+            "phases": phases, "list_builds": builds}
 
 
-def profile_pair_kernel(spc2, w, precision, steps):
+def profile_pair_kernel(spc2, w, precision, steps, neighbour):
     """HIP-event timing of the pair kernels on the handle's stream, with pair counts."""
-    s = make_sim(spc2, w, precision)
+    s = make_sim(spc2, w, precision, neighbour)
     s.step("rk4", w["dt"], 0, 2)
     s.set_profiling(True)            # timing pass: events bound to the pair-kernel dispatches
     s.step("rk4", w["dt"], 2, steps)
@@ -77,14 +83,40 @@ def profile_pair_kernel(spc2, w, precision, steps):
     per = cnt / launches
     flops = (FLOP_CANDIDATE * per[0] + FLOP_VORTEX_PAIR * per[1] + FLOP_COHESION * per[2] + FLOP_EPILOGUE * w["n"])
     avg_s = ms / launches / 1e3
-    # minimal HBM bytes per launch: FP32 tile read (16 B) + slot->particle and
-    # cell (8 B) + FP64 phase (8 B) + k written (24 B); the last stage also reads
-    # k1..k3 (72 B) and reads and writes the state (48 B)
-    bytes_per = w["n"] * (56 + (72 + 48) / 4.0) if precision == "fp32k" else w["n"] * (72 + 24 + 8 + (120 / 4.0))
+    # algorithmic HBM bytes per launch, per particle: own tile (FP32 16 B; FP64
+    # 24 B) + FP64 stage phase 8 B + home->id and counts 8 B + k written 24 B;
+    # the last RK4 stage also reads k1..k3 (72 B) and reads and writes the state
+    # (48 B), a quarter of the launches; Verlet: 4 B per list entry (neighbour
+    # tiles are re-read from cache, not HBM)
+    own = 16 if precision == "fp32k" else 24
+    list_b = 4.0 * per[0] if neighbour == "verlet" else 0.0
+    bytes_per = w["n"] * (own + 8 + 8 + 24 + (72 + 48) / 4.0) + list_b
     return {"avg_ms": ms / launches, "launches": launches, "flops_per_launch": flops,
             "tflops": flops / avg_s / 1e12, "bytes_per_launch": bytes_per, "gbs": bytes_per / avg_s / 1e9,
             "candidates_per_particle": per[0] / w["n"], "neighbours_per_particle": per[1] / w["n"],
             "cohesion_per_particle": per[2] / w["n"]}
+
+
+def error_horizon(spc2, w, nb, horizon):
+    """FP32-compensated vs FP64 from the same initial state after `horizon`
+    steps, and the FP64 run against FP64 with the other neighbour structure
+    (same arithmetic, another summation order): the divergence floor of this
+    chaotic system at that horizon."""
+    import numpy as np
+    ph = {}
+    for prec, neigh in (("fp64", nb), ("fp32k", nb), ("fp64", "cells" if nb == "verlet" else "verlet")):
+        s = make_sim(spc2, w, prec, neigh)
+        s.step("rk4", w["dt"], 0, horizon)
+        ph[(prec, neigh)] = s.get_state()[2::3]
+        s.close()
+    ref = ph[("fp64", nb)]
+    R = lambda p: float(np.abs(np.exp(1j * p).mean()))
+    out = {}
+    for key, name in (((("fp32k", nb)), "fp32"), (next(k for k in ph if k[1] != nb), "fp64_reordered")):
+        d = np.abs(ph[key] - ref)
+        out[name] = {"phase_err_max": float(d.max()), "phase_err_rms": float(np.sqrt((d ** 2).mean())),
+                     "order_param_abs_err": abs(R(ph[key]) - R(ref))}
+    return out
 
 
 def cpu_baseline(w, steps=2):
@@ -117,6 +149,9 @@ def main():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precisions", default="fp32k,fp64,fp64f",
                     help="forms to time; the first is the headline value")
+    ap.add_argument("--neighbour", default="verlet", choices=["verlet", "cells"])
+    ap.add_argument("--error-horizon", type=int, default=100,
+                    help="steps after which FP32 and FP64 runs are compared before chaotic decorrelation")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every GPU box here has one MI355X and this build has no multi-GPU path "
@@ -127,17 +162,37 @@ def main():
     w = workload(args.n)
     res = {}
     precs = args.precisions.split(",")
+    nb = args.neighbour
     for prec in precs:
-        res[prec] = time_precision(spc2, w, prec, args.steps, args.warmup)
-        print(f"# {prec}: {res[prec]['particle_steps_per_s']:.4e} particle-steps/s, "
-              f"{res[prec]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
-    prof = profile_pair_kernel(spc2, w, "fp32k", args.profile_steps)
-    prof64 = profile_pair_kernel(spc2, w, "fp64", max(4, args.profile_steps // 4)) if "fp64" in res else prof
-    print(f"# pair kernel fp32k: {prof['avg_ms']:.4f} ms/launch, {prof['tflops']:.2f} TFLOP/s; "
-          f"fp64: {prof64['avg_ms']:.4f} ms/launch", file=sys.stderr, flush=True)
+        res[prec] = time_precision(spc2, w, prec, args.steps, args.warmup, nb)
+        print(f"# {prec} ({nb}): {res[prec]['particle_steps_per_s']:.4e} particle-steps/s, "
+              f"{res[prec]['ms_per_step']:.3f} ms/step, list builds {res[prec]['list_builds']}",
+              file=sys.stderr, flush=True)
+    prof = profile_pair_kernel(spc2, w, "fp32k", args.profile_steps, nb)
+    prof64 = profile_pair_kernel(spc2, w, "fp64", max(4, args.profile_steps // 4), nb) if "fp64" in res else None
+    print(f"# pair kernel fp32k: {prof['avg_ms']:.4f} ms/launch, {prof['tflops']:.2f} TFLOP/s, "
+          f"{prof['gbs']:.0f} GB/s" + (f"; fp64: {prof64['avg_ms']:.4f} ms/launch" if prof64 else ""),
+          file=sys.stderr, flush=True)
+    hz = error_horizon(spc2, w, nb, args.error_horizon) if "fp64" in res and args.error_horizon > 0 else None
+    if hz:
+        print(f"# after {args.error_horizon} steps: fp32 vs fp64 {hz['fp32']}; fp64 reordered {hz['fp64_reordered']}",
+              file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(w)
 
     v = res["fp32k"]
+    err = {}
+    if "fp64" in res:
+        import numpy as np
+        dphi = np.abs(v["phases"] - res["fp64"]["phases"])
+        err = {"errors_fp32_vs_fp64": {
+            f"after_{args.warmup + args.steps}_steps": {
+                "phase_err_max": float(dphi.max()), "phase_err_rms": float(np.sqrt((dphi ** 2).mean())),
+                "order_param_abs_err": abs(v["R_final"] - res["fp64"]["R_final"])},
+            **({f"after_{args.error_horizon}_steps": hz["fp32"],
+                f"fp64_reordered_after_{args.error_horizon}_steps": hz["fp64_reordered"]} if hz else {})}}
+    hbm = nb == "verlet"
+    kname = ("k_pass1_verlet<VortAcc<P32K>>" if hbm else "k_pass1_lds<VortAcc<P32K>>") + \
+        " (vortex pair sums, FP32 Kahan)"
     line = {
         "metric": "particle-steps/s at N=1M (FP32-comp vs FP64, same GPU)",
         "value": v["particle_steps_per_s"],
@@ -152,21 +207,28 @@ def main():
         "dtype": "f32 pair sums (Kahan) + f64 state",
         "data": "synthetic (Philox uniform positions and phases, seed 42)",
         "config": {"workload": "vortex-arrays RK4, N=1048576, density 4000 (L=16.19086), rho=0.05, xi_a=0.01, "
-                               "xi_r=0.005, v=0.025, mu+-=100, kappa=20, D=0, dt=0.01, linked cells per RHS",
-                   "n": w["n"], "method": "rk4", "neighbour": "cells"},
+                               "xi_r=0.005, v=0.025, mu+-=100, kappa=20, D=0, dt=0.01, "
+                               + ("Verlet list, skin 0.2 rho, rebuilt at skin/2 displacement" if hbm
+                                  else "linked cells per RHS"),
+                   "n": w["n"], "method": "rk4", "neighbour": nb,
+                   "list_builds_in_timed_steps": v["list_builds"]},
         "fp64_value": res["fp64"]["particle_steps_per_s"] if "fp64" in res else None,
         "fp64_factored_value": res["fp64f"]["particle_steps_per_s"] if "fp64f" in res else None,
         "fp32_over_fp64": (v["particle_steps_per_s"] / res["fp64"]["particle_steps_per_s"]) if "fp64" in res else None,
-        "order_param_abs_err_fp32_vs_fp64": abs(v["R_final"] - res["fp64"]["R_final"]) if "fp64" in res else None,
+        **err,
         "roofline": {
-            "bound": "valu", "kernel": "k_pass1_lds<VortAcc<P32K>> (vortex pair sums, FP32 Kahan)",
-            "achieved": prof["tflops"], "peak": PEAK["fp32k"], "unit": "TFLOP/s",
-            "frac": prof["tflops"] / PEAK["fp32k"], "traffic": None,
-            "avg_launch_ms": prof["avg_ms"], "flops_per_launch": prof["flops_per_launch"],
-            "hbm_achieved_GBs": prof["gbs"], "hbm_frac": prof["gbs"] / HBM_PEAK_GBS,
+            "bound": "hbm" if hbm else "valu", "kernel": kname,
+            "achieved": prof["gbs"] if hbm else prof["tflops"],
+            "peak": HBM_PEAK_GBS if hbm else PEAK["fp32k"],
+            "unit": "GB/s" if hbm else "TFLOP/s",
+            "frac": prof["gbs"] / HBM_PEAK_GBS if hbm else prof["tflops"] / PEAK["fp32k"],
+            "traffic": None,
+            "avg_launch_ms": prof["avg_ms"], "bytes_per_launch": prof["bytes_per_launch"],
+            "flops_per_launch": prof["flops_per_launch"], "valu_tflops": prof["tflops"],
+            "valu_frac": prof["tflops"] / PEAK["fp32k"],
             "candidates_per_particle": prof["candidates_per_particle"],
             "neighbours_per_particle": prof["neighbours_per_particle"],
-            "fp64_pair_kernel_avg_ms": prof64["avg_ms"], "fp64_pair_kernel_tflops": prof64["tflops"],
+            "fp64_pair_kernel_avg_ms": prof64["avg_ms"] if prof64 else None,
         },
         "cpu_baseline": cpu,
     }

@@ -44,8 +44,17 @@ typedef enum spc2_method { SPC2_SE = 0, SPC2_SRK = 1, SPC2_RK4 = 2 } spc2_method
 
 /* Neighbour search.  AUTO follows the reference dispatch: linked cells iff
  * rho <= L/4, all pairs otherwise (ChimeraSystem.cpp:90-98,
- * VortexArraysSystemPtr.cpp:113-119). */
-typedef enum spc2_neighbour { SPC2_NEIGH_AUTO = 0, SPC2_NEIGH_CELLS = 1, SPC2_NEIGH_ALL_PAIRS = 2 } spc2_neighbour;
+ * VortexArraysSystemPtr.cpp:113-119).  VERLET keeps a list of the pairs within
+ * max(rho, xi_r) + verlet_skin of the positions at its build and rebuilds it
+ * before any particle can have moved more than verlet_skin / 2 (the
+ * reference's dormant path, ChimeraSystemPtr.cpp:650-927); the neighbour sets
+ * at every evaluation are those of the linked cells. */
+typedef enum spc2_neighbour {
+  SPC2_NEIGH_AUTO = 0,
+  SPC2_NEIGH_CELLS = 1,
+  SPC2_NEIGH_ALL_PAIRS = 2,
+  SPC2_NEIGH_VERLET = 3
+} spc2_neighbour;
 
 /* Model and run parameters.  The reference hard-codes these in the engine
  * (SimulationEnginePtr.cpp:82-91, :131-139) and in Definitions.hpp:20-22;
@@ -63,6 +72,7 @@ typedef struct spc2_params {
   /* vortex arrays (VortexArraysSystemPtr.hpp:16-25) */
   double mu_plus, mu_minus, xi_a, xi_r, kappa;
   uint64_t seed;     /* Philox key for initial state, natural frequencies and noise */
+  double verlet_skin; /* SPC2_NEIGH_VERLET list skin; 0 selects 0.2 * rho */
 } spc2_params;
 
 typedef struct spc2_sim *spc2_sim_t;
@@ -168,6 +178,10 @@ spc2_status spc2_get_counts(spc2_sim_t h, int64_t *counts3);
 /* HIP-event timing of the pair kernels on the handle's stream. */
 spc2_status spc2_set_profiling(spc2_sim_t h, int32_t enable);
 spc2_status spc2_get_profile(spc2_sim_t h, double *pair_kernel_ms, int64_t *launches);
+
+/* Verlet-list builds since creation and the entries of the current list
+ * (SPC2_ERR_INVALID unless the handle uses SPC2_NEIGH_VERLET). */
+spc2_status spc2_verlet_stats(spc2_sim_t h, int64_t *builds, int64_t *entries);
 
 spc2_status spc2_synchronize(spc2_sim_t h);
 

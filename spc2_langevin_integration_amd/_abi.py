@@ -19,7 +19,7 @@ CHIMERA, VORTEX = 0, 1
 FP64, FP32_KAHAN, FP64_FACTORED = 0, 1, 2
 SE, SRK, RK4 = 0, 1, 2
 STEP_NO_WRAP = 1
-NEIGH_AUTO, NEIGH_CELLS, NEIGH_ALL_PAIRS = 0, 1, 2
+NEIGH_AUTO, NEIGH_CELLS, NEIGH_ALL_PAIRS, NEIGH_VERLET = 0, 1, 2, 3
 
 PRECISIONS = {"fp64": FP64, "fp32k": FP32_KAHAN, "fp64f": FP64_FACTORED}
 METHODS = {"se": SE, "srk": SRK, "rk4": RK4}
@@ -31,7 +31,7 @@ class Params(C.Structure):
         ("n", C.c_int64), ("Lx", C.c_double), ("Ly", C.c_double), ("rho", C.c_double),
         ("v0", C.c_double), ("sigma", C.c_double), ("alpha", C.c_double), ("D", C.c_double),
         ("mu_plus", C.c_double), ("mu_minus", C.c_double), ("xi_a", C.c_double), ("xi_r", C.c_double),
-        ("kappa", C.c_double), ("seed", C.c_uint64),
+        ("kappa", C.c_double), ("seed", C.c_uint64), ("verlet_skin", C.c_double),
     ]
 
 
@@ -80,6 +80,7 @@ def load():
         "spc2_set_profiling": (st, [h, C.c_int32]),
         "spc2_get_profile": (st, [h, dp, C.POINTER(C.c_int64)]),
         "spc2_synchronize": (st, [h]),
+        "spc2_verlet_stats": (st, [h, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -113,7 +114,8 @@ def params(system="chimera", precision="fp64", neighbour="auto", **kw) -> Params
     p = default_params()
     p.system = CHIMERA if system == "chimera" else VORTEX
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-    p.neighbour = {"auto": NEIGH_AUTO, "cells": NEIGH_CELLS, "all_pairs": NEIGH_ALL_PAIRS}[neighbour]
+    p.neighbour = {"auto": NEIGH_AUTO, "cells": NEIGH_CELLS, "all_pairs": NEIGH_ALL_PAIRS,
+                   "verlet": NEIGH_VERLET}[neighbour]
     if "L" in kw:
         L = kw.pop("L")
         p.Lx = p.Ly = L
@@ -245,6 +247,12 @@ class Sim:
 
     def synchronize(self):
         _check(load().spc2_synchronize(self.h))
+
+    def verlet_stats(self):
+        """(list builds so far, entries of the current list)."""
+        b, e = C.c_int64(), C.c_int64()
+        _check(load().spc2_verlet_stats(self.h, C.byref(b), C.byref(e)))
+        return b.value, e.value
 
 
 class PinnedFrame:

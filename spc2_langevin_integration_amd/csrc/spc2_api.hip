@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <utility>
@@ -66,6 +67,7 @@ void free_all(Sim &s) {
   f(s.t.x); f(s.t.y); f(s.t.p); f(s.t.c); f(s.t.s); f(s.t.f4); f(s.t.cell);
   f(s.aphi_slot); f(s.n_slot); f(s.n2_slot); f(s.aphi_slot_f);
   f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
+  f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb);
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
 
@@ -143,7 +145,11 @@ spc2_status alloc_all(Sim &s) {
     }
   }
   A(s.aphi_slot, n); A(s.n_slot, n); A(s.n2_slot, n); A(s.aphi_slot_f, n);
-  A(s.counts, 3 * kCountSlots); A(s.red, 2 * 1024 + 8); A(s.bsum, (size_t)scan_blocks(std::max(s.nc, 1)) + 1);
+  A(s.counts, 3 * kCountSlots); A(s.red, 2 * 1024 + 8);
+  A(s.bsum, (size_t)scan_blocks(std::max({s.nc, s.nw, 1})) + 1);
+  if (s.verlet) {
+    A(s.vcnt, n); A(s.wmax, (size_t)s.nw + 1); A(s.wrow, (size_t)s.nw + 1); A(s.xb, n); A(s.yb, n);
+  }
 #undef A
   HIP_TRY(hipMemset(s.count, 0, sizeof(int) * nc));
   HIP_TRY(hipMemset(s.omega, 0, sizeof(double) * n));
@@ -182,8 +188,79 @@ spc2_status download_aos(Sim &s, double *aos, int stride, double *const *src, in
   return SPC2_OK;
 }
 
-// Stage + cell list + tiles for one RHS evaluation.
+// Verlet list from the positions s + kcdt * kprev (the state when kprev is
+// null): bin them on the build grid, move the particle arrays into the build's
+// cell order, then count, scan and fill the wavefront-interleaved list.
+spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt) {
+  if (kprev)
+    launch_stage_bin(s, kprev[0], kprev[1], kprev[2], kcdt);
+  else
+    launch_stage_bin(s, nullptr, nullptr, nullptr, 0.0);
+  launch_scan(s);
+  launch_place(s);
+  launch_rank_tiles(s);
+  launch_rehome(s);  // slot k becomes home k: tiles, cells and start[] are now by home
+  HIP_TRY(hipMemcpyAsync(s.xb, s.t.x, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
+  HIP_TRY(hipMemcpyAsync(s.yb, s.t.y, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
+  launch_verlet_count(s);
+  launch_scan_array(s, s.wmax, s.nw, s.wrow);
+  int rows = 0;
+  HIP_TRY(hipMemcpyAsync(&rows, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  const size_t need = (size_t)rows * 64;
+  if (need > s.vlist_cap || !s.vlist) {
+    if (s.vlist) (void)hipFree(s.vlist);
+    s.vlist = nullptr;
+    s.vlist_cap = 0;
+    const size_t cap = need + need / 8 + 64 * 64;
+    HIP_TRY(dalloc(&s.vlist, cap));
+    s.vlist_cap = cap;
+  }
+  launch_verlet_fill(s);
+  if (spc2_status st = check_launch("verlet build")) return st;
+  s.vbuilt = true;
+  s.built = false;
+  s.steps_since_rehome = 0;
+  s.drift = kprev ? std::numeric_limits<double>::infinity() : 0.0;
+  s.vbuilds += 1;
+  s.ventries = -1;
+  return SPC2_OK;
+}
+
+// Largest displacement of the state since the Verlet build (synchronous).
+spc2_status verlet_measure(Sim &s) {
+  double d2 = 0.0;
+  launch_drift(s, s.red + 2048);
+  HIP_TRY(hipMemcpyAsync(&d2, s.red + 2048, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  s.drift = std::sqrt(d2) * (1.0 + 1e-12);
+  return SPC2_OK;
+}
+
+// Makes the list valid for evaluations at positions within `reach` of the
+// state: stage positions are within drift + reach of the build positions, and
+// every pair within rho at the stage was within rho + skin at the build while
+// that stays <= skin / 2.
+spc2_status verlet_ensure(Sim &s, double reach) {
+  const double half = 0.5 * s.skin * (1.0 - 1e-9);
+  if (s.vbuilt && s.drift + reach <= half) return SPC2_OK;
+  if (s.vbuilt && std::isinf(s.drift)) {
+    if (spc2_status st = verlet_measure(s)) return st;
+    if (s.drift + reach <= half) return SPC2_OK;
+  }
+  return verlet_build(s, nullptr, 0.0);
+}
+
+// Stage + cell list + tiles for one RHS evaluation (Verlet: stage and tiles
+// by home only; the list is kept valid by verlet_ensure).
 void build(Sim &s, double *const *kprev, double kcdt) {
+  if (s.verlet) {
+    if (kprev)
+      launch_verlet_stage(s, kprev[0], kprev[1], kprev[2], kcdt);
+    else
+      launch_verlet_stage(s, nullptr, nullptr, nullptr, 0.0);
+    return;
+  }
   if (kprev)
     launch_stage_bin(s, kprev[0], kprev[1], kprev[2], kcdt);
   else
@@ -199,6 +276,7 @@ void build(Sim &s, double *const *kprev, double kcdt) {
 // Moves the particle arrays to the cell order of the current state (used
 // when a new state arrives).
 spc2_status rehome_now(Sim &s) {
+  if (s.verlet) return verlet_build(s, nullptr, 0.0);
   if (!s.linked) return SPC2_OK;
   build(s, nullptr, 0.0);
   launch_rehome(s);
@@ -211,11 +289,27 @@ spc2_status rehome_now(Sim &s) {
 // After every rehome_every steps, moves the particle arrays to the cell order
 // of the last build.
 void maybe_rehome(Sim &s) {
-  if (!s.linked) return;
+  if (!s.linked || s.verlet) return;  // Verlet homes change only at list builds
   if (++s.steps_since_rehome < s.rehome_every || !s.built) return;
   launch_rehome(s);
   s.built = false;
   s.steps_since_rehome = 0;
+}
+
+// RHS passes on the handle's neighbour structure.
+void rhs_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
+               double *const k2[3], double *const k3[3]) {
+  if (s.verlet)
+    launch_verlet_pass1(s, epi, ox, oy, op, sa, k1, k2, k3);
+  else
+    launch_pass1(s, epi, ox, oy, op, sa, k1, k2, k3);
+}
+
+void rhs_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa) {
+  if (s.verlet)
+    launch_verlet_pass2(s, epi, a0, a1, sa);
+  else
+    launch_pass2(s, epi, a0, a1, sa);
 }
 
 StepArgs step_args(const Sim &s, double dt, uint64_t step) {
@@ -277,8 +371,9 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
   if (p->system != SPC2_CHIMERA && p->system != SPC2_VORTEX) return fail(SPC2_ERR_INVALID, "unknown system");
   if (p->precision < SPC2_FP64 || p->precision > SPC2_FP64_FACTORED)
     return fail(SPC2_ERR_INVALID, "unknown precision");
-  if (p->neighbour < SPC2_NEIGH_AUTO || p->neighbour > SPC2_NEIGH_ALL_PAIRS)
+  if (p->neighbour < SPC2_NEIGH_AUTO || p->neighbour > SPC2_NEIGH_VERLET)
     return fail(SPC2_ERR_INVALID, "unknown neighbour mode");
+  if (!(p->verlet_skin >= 0.0)) return fail(SPC2_ERR_INVALID, "verlet_skin must be >= 0");
   if (!(p->D >= 0.0)) return fail(SPC2_ERR_INVALID, "D must be >= 0");
   spc2_sim *h = new (std::nothrow) spc2_sim();
   if (!h) return fail(SPC2_ERR_NOMEM, "host allocation failed");
@@ -289,8 +384,25 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
   if (p->neighbour == SPC2_NEIGH_AUTO)
     s.linked = p->rho <= 0.25 * p->Lx;  // ChimeraSystem.cpp:90, VortexArraysSystemPtr.cpp:113
   else
-    s.linked = p->neighbour == SPC2_NEIGH_CELLS;
-  if (s.linked) {
+    s.linked = p->neighbour == SPC2_NEIGH_CELLS || p->neighbour == SPC2_NEIGH_VERLET;
+  s.verlet = p->neighbour == SPC2_NEIGH_VERLET;
+  s.nw = (s.n + 63) / 64;
+  if (s.verlet) {
+    if (p->n >= (1LL << 27)) {  // list entries pack the neighbour's home above 4 offset bits
+      delete h;
+      return fail(SPC2_ERR_INVALID, "Verlet lists support n < 2^27");
+    }
+    s.skin = p->verlet_skin > 0.0 ? p->verlet_skin : 0.2 * p->rho;
+    s.rlist = (p->system == SPC2_VORTEX ? std::max(p->rho, p->xi_r) : p->rho) + s.skin;
+    double fx = p->Lx / s.rlist, fy = p->Ly / s.rlist;
+    if (fx > 2.0e9 || fy > 2.0e9 || fx * fy > 1.0e9 || (int)fx < 3 || (int)fy < 3) {
+      delete h;
+      return fail(SPC2_ERR_INVALID, "Verlet lists need 3 to 1e9 build cells (L / (rho + skin) >= 3)");
+    }
+    s.nx = (int)fx;  // build cells of side L / nx >= rho + skin
+    s.ny = (int)fy;
+    s.nc = s.nx * s.ny;
+  } else if (s.linked) {
     double fx = p->Lx / p->rho, fy = p->Ly / p->rho;
     if (fx > 2.0e9 || fy > 2.0e9 || fx * fy > 1.0e9) {
       delete h;
@@ -385,9 +497,18 @@ spc2_status spc2_eval_rhs(spc2_sim_t h, const double *k_prev, double k_coef, dou
   // k_prev -> k[1] (scratch), k_next (in/out) -> k[0]
   if (spc2_status st = upload_aos(s, k_prev, 3, s.k[1], 3)) return st;
   if (spc2_status st = upload_aos(s, k_next, 3, s.k[0], 3)) return st;
+  if (s.verlet) {
+    // reach of the stage: |k_coef dt| max_i |(k_x, k_y)_i|
+    double kmax = 0.0;
+    for (int64_t i = 0; i < s.n; ++i) kmax = std::max(kmax, std::hypot(k_prev[3 * i], k_prev[3 * i + 1]));
+    const double reach = std::fabs(k_coef * dt) * kmax * (1.0 + 1e-12);
+    spc2_status st = reach <= 0.5 * s.skin * (1.0 - 1e-9) ? verlet_ensure(s, reach)
+                                                           : verlet_build(s, s.k[1], k_coef * dt);
+    if (st) return st;
+  }
   build(s, s.k[1], k_coef * dt);
   StepArgs sa = step_args(s, dt, 0);
-  launch_pass1(s, EPI_K, s.k[0][0], s.k[0][1], s.k[0][2], sa, nullptr, nullptr, nullptr);
+  rhs_pass1(s, EPI_K, s.k[0][0], s.k[0][1], s.k[0][2], sa, nullptr, nullptr, nullptr);
   if (spc2_status st = check_launch("eval_rhs")) return st;
   return download_aos(s, k_next, 3, s.k[0], 3);
 }
@@ -397,10 +518,12 @@ spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *addition
   Sim &s = h->s;
   if (spc2_status st = upload_aos(s, derivative, 3, s.der, 3)) return st;
   if (spc2_status st = upload_aos(s, additional, 2, s.add, 2)) return st;
+  if (s.verlet)
+    if (spc2_status st = verlet_ensure(s, 0.0)) return st;
   build(s, nullptr, 0.0);
   StepArgs sa = step_args(s, 0.0, 0);
-  launch_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
-  launch_pass2(s, EPI_ITO2, s.add[0], s.add[1], sa);
+  rhs_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
+  rhs_pass2(s, EPI_ITO2, s.add[0], s.add[1], sa);
   if (spc2_status st = check_launch("eval_rhs_ito")) return st;
   if (spc2_status st = download_aos(s, derivative, 3, s.der, 3)) return st;
   return download_aos(s, additional, 2, s.add, 2);
@@ -430,29 +553,44 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
     HIP_TRY(hipMemcpyAsync(s.noise, noise, sizeof(double) * need, hipMemcpyHostToDevice, s.stream));
   }
   for (int64_t k = 0; k < nsteps; ++k) {
+    if (s.verlet) {
+      // RK4 stages lie within v0 dt of the state (|k_x, k_y| = v0), SE and SRK
+      // evaluate at the state.  A rebuild re-homes the particle arrays, so the
+      // step's arguments are taken after it.
+      if (spc2_status st = verlet_ensure(s, method == SPC2_RK4 ? std::fabs(s.P.v0) * dt * (1.0 + 1e-12) : 0.0))
+        return st;
+    }
     StepArgs sa = step_args(s, dt, (uint64_t)(step0 + k));
     sa.wrap = (flags & SPC2_STEP_NO_WRAP) ? 0 : 1;
     if (noise && method != SPC2_RK4) sa.noise = s.noise + (size_t)k * npp * (size_t)s.n;
     if (method == SPC2_SE) {
       build(s, nullptr, 0.0);
-      launch_pass1(s, EPI_SE, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
+      rhs_pass1(s, EPI_SE, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
     } else if (method == SPC2_SRK) {
       build(s, nullptr, 0.0);
       sa.zero_in = 1;  // derivative zero-filled each step (StochasticRungeKuttaStepperPtr.hpp:43-46)
-      launch_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
-      launch_pass2(s, EPI_SRK, nullptr, nullptr, sa);
+      rhs_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
+      rhs_pass2(s, EPI_SRK, nullptr, nullptr, sa);
     } else {
       // RungeKutta4StepperPtr.cpp:25-43: k1 = f(s, k1, 0), k2 = f(s, k1, 1/2),
       // k3 = f(s, k2, 1/2), k4 = f(s, k3, 1), stage = s + (c * dt) * k
       double *const *k1 = s.k[0], *const *k2 = s.k[1], *const *k3 = s.k[2], *const *k4 = s.k[3];
       build(s, k1, 0.0 * dt);
-      launch_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);
+      rhs_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);
       build(s, k1, 0.5 * dt);
-      launch_pass1(s, EPI_K, k2[0], k2[1], k2[2], sa, nullptr, nullptr, nullptr);
+      rhs_pass1(s, EPI_K, k2[0], k2[1], k2[2], sa, nullptr, nullptr, nullptr);
       build(s, k2, 0.5 * dt);
-      launch_pass1(s, EPI_K, k3[0], k3[1], k3[2], sa, nullptr, nullptr, nullptr);
+      rhs_pass1(s, EPI_K, k3[0], k3[1], k3[2], sa, nullptr, nullptr, nullptr);
       build(s, k3, 1.0 * dt);
-      launch_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
+      rhs_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
+    }
+    if (s.verlet) {
+      // SE and RK4 move a particle by at most v0 dt per step; SRK adds noise to
+      // the positions, so its displacement is measured before the next use
+      if (method == SPC2_SRK)
+        s.drift = std::numeric_limits<double>::infinity();
+      else
+        s.drift += std::fabs(s.P.v0) * dt * (1.0 + 1e-12);
     }
     maybe_rehome(s);
     if (spc2_status st = check_launch("step")) return st;
@@ -498,6 +636,7 @@ spc2_status spc2_get_cell_list(spc2_sim_t h, int32_t *cell_of, int32_t *cell_sta
   if (check_handle(h) || !cell_of || !cell_start || !sorted) return fail(SPC2_ERR_INVALID, "null argument");
   Sim &s = h->s;
   if (!s.linked) return fail(SPC2_ERR_INVALID, "no cell list on the all-pairs path");
+  if (s.verlet) return fail(SPC2_ERR_INVALID, "no per-RHS cell list on the Verlet path");
   build(s, nullptr, 0.0);
   if (spc2_status st = check_launch("cell list")) return st;
   launch_from_home_int(s, s.cell_of, s.spare_i);
@@ -505,6 +644,23 @@ spc2_status spc2_get_cell_list(spc2_sim_t h, int32_t *cell_of, int32_t *cell_sta
   HIP_TRY(hipMemcpyAsync(cell_start, s.start, sizeof(int) * ((size_t)s.nc + 1), hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipMemcpyAsync(sorted, s.slot_id, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+spc2_status spc2_verlet_stats(spc2_sim_t h, int64_t *builds, int64_t *entries) {
+  if (check_handle(h) || !builds || !entries) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  if (!s.verlet) return fail(SPC2_ERR_INVALID, "the handle does not use Verlet lists");
+  if (s.ventries < 0 && s.vbuilt) {
+    std::vector<int> c((size_t)s.n);
+    HIP_TRY(hipMemcpyAsync(c.data(), s.vcnt, sizeof(int) * c.size(), hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    long long t = 0;
+    for (int v : c) t += v;
+    s.ventries = t;
+  }
+  *builds = s.vbuilds;
+  *entries = s.vbuilt ? s.ventries : 0;
   return SPC2_OK;
 }
 

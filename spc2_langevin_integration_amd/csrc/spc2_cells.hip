@@ -215,12 +215,15 @@ void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *
 
 int scan_blocks(int nc) { return (nc + kScanTile - 1) / kScanTile; }
 
-void launch_scan(Sim &s) {
-  const int nb = scan_blocks(s.nc);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s.stream, s.count, s.nc, s.bsum);
-  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kBlock), 0, s.stream, s.bsum, nb, s.start, s.nc);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s.stream, s.count, s.nc, s.bsum, s.start);
+// Exclusive scan of count[0..m) into out[0..m], out[m] = total; clears count.
+void launch_scan_array(Sim &s, int *count, int m, int *out) {
+  const int nb = scan_blocks(m);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s.stream, count, m, s.bsum);
+  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kBlock), 0, s.stream, s.bsum, nb, out, m);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s.stream, count, m, s.bsum, out);
 }
+
+void launch_scan(Sim &s) { launch_scan_array(s, s.count, s.nc, s.start); }
 
 void launch_place(Sim &s) {
   hipLaunchKernelGGL(k_place, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.cell_of, s.local, s.start,

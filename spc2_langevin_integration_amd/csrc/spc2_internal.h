@@ -110,6 +110,16 @@ struct Sim {
   // counts: candidates, interactions, cohesion pairs
   unsigned long long *counts = nullptr;
   bool counting = false;
+  // Verlet lists: the build grid is nx x ny (cells of side >= rlist); homes are
+  // re-assigned to the build's cell order at each build
+  bool verlet = false, vbuilt = false;
+  double skin = 0.0, rlist = 0.0;
+  double drift = 0.0;  // bound on the largest |state - build position| (inf: unknown)
+  int nw = 0;          // wavefronts of 64 homes
+  int *vcnt = nullptr, *wmax = nullptr, *wrow = nullptr, *vlist = nullptr;
+  size_t vlist_cap = 0;
+  double *xb = nullptr, *yb = nullptr;  // build positions by home
+  long long vbuilds = 0, ventries = 0;
   // float frame buffer for asynchronous snapshots and its completion event
   float *snap = nullptr;
   hipEvent_t snap_ev = nullptr;
@@ -152,6 +162,7 @@ struct Sim {
 void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
 int scan_blocks(int nc);
 void launch_scan(Sim &s);
+void launch_scan_array(Sim &s, int *count, int m, int *out);
 void launch_place(Sim &s);
 void launch_rank_tiles(Sim &s);
 // spc2_pairs.hip
@@ -159,6 +170,14 @@ void configure_segments(Sim &s);
 void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
                   double *const k2[3], double *const k3[3]);
 void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
+// spc2_verlet.hip
+void launch_verlet_count(Sim &s);
+void launch_verlet_fill(Sim &s);
+void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
+void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa,
+                         double *const k1[3], double *const k2[3], double *const k3[3]);
+void launch_verlet_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
+void launch_drift(Sim &s, double *out);
 // spc2_misc.hip
 void launch_to_home(Sim &s, const double *src_id, double *dst_home);
 void launch_from_home(Sim &s, const double *src_home, double *dst_id);

@@ -19,6 +19,7 @@
 
 #include "spc2_device.h"
 #include "spc2_internal.h"
+#include "spc2_pair_accumulators.h"
 
 namespace spc2 {
 
@@ -29,931 +30,6 @@ constexpr int kMaxSegW = 64;
 constexpr int kMaxSlots = 3 * (kMaxSegW + 2);
 
 inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
-
-// ---- tile sources -------------------------------------------------------------
-// Src32 reads the FP32 tile (cell-local x, y and cos, sin of the phase); `slot`
-// maps an LDS index to its global slot (null: the index is the slot).
-struct Src32 {
-  const float4 *f4;
-  const int *slot;
-  const float *aj;
-  __device__ __forceinline__ int gslot(int m) const { return slot ? slot[m] : m; }
-};
-struct Src64 {
-  const double *x, *y, *p, *c, *s, *aj;
-};
-
-template <int PREC>
-struct SrcOf {
-  using type = Src64;
-};
-template <>
-struct SrcOf<P32K> {
-  using type = Src32;
-};
-
-// AlignmentStrength, VortexArraysSystemPtr.cpp:737-749
-__device__ __forceinline__ double vortex_w(const KParams &P, double r) {
-  if (r < P.xi_a) return P.mu_plus * (1.0 - r * r / (P.xi_a * P.xi_a));
-  if (r < P.rho) return -P.mu_minus * 4.0 * (r - P.xi_a) * (P.rho - r) / (P.rho - P.xi_a) / (P.rho - P.xi_a);
-  return 0.0;
-}
-
-// AlignmentStrengthDerivative, VortexArraysSystemPtr.cpp:751-763
-__device__ __forceinline__ double vortex_dw(const KParams &P, double r) {
-  if (r < P.xi_a) return P.mu_plus * (-2.0 * r / P.xi_a / P.xi_a);
-  if (r < P.rho) return -P.mu_minus * 4.0 * (-2.0 * r + P.rho + P.xi_a) / (P.rho - P.xi_a) / (P.rho - P.xi_a);
-  return 0.0;
-}
-
-// FP64 distance^2 of slots k and m with the reference min-image: decides the
-// rare FP32 pairs that fall inside a cutoff band.
-__device__ __forceinline__ double exact_r2(const double *tx, const double *ty, int k, int m, double Lx, double rLx,
-                                           double Ly, double rLy) {
-  double dx = min_image(tx[m] - tx[k], Lx, rLx);
-  double dy = min_image(ty[m] - ty[k], Ly, rLy);
-  return dx * dx + dy * dy;
-}
-
-// FP64 min-image separation of slots k and m, for cohesion pairs closer than
-// a hundredth of a cell: there the 1/r and 1/r^2 factors would amplify the
-// FP32 rounding of the cell-local coordinates (about 6e-8 of a cell width)
-// beyond 1e-6 of the term.
-__device__ __forceinline__ void exact_d(const double *tx, const double *ty, int k, int m, const KParams &P, double &dx,
-                                        double &dy) {
-  dx = min_image(tx[m] - tx[k], P.Lx, P.rLx);
-  dy = min_image(ty[m] - ty[k], P.Ly, P.rLy);
-}
-
-// Per-particle result of pass 1.
-struct Pass1Out {
-  double phase;    // FP64 stage phase of i
-  double aphi;     // phi derivative when `written`
-  int written;     // 0: phi entry left as is (chimera n_i = 0; vortex n0+n1 = 0 and n2 = 0)
-  int vortex_add;  // vortex: 1 if only the cohesion term applies (added to the old value)
-  double coh;      // vortex cohesion / n2 for vortex_add
-  int n;           // partners within rho
-  int n2;          // cohesion partners
-};
-
-// FP32 x offset of the candidate's cell column relative to i's: m < b1 is the
-// left stencil column, b1 <= m < b2 the middle one, m >= b2 the right one.
-// Written as -x_i + sel * h with sel in {-1, 0, 1} (exact), so that the
-// compiler keeps every member in registers (a select between members can be
-// turned into a select between their addresses, which pushes the whole
-// accumulator to scratch memory).
-struct ColOffsets {
-  float oM, hxf, oy;
-  int b1, b2;
-  __device__ __forceinline__ void set(float xl, float h) {
-    oM = -xl;
-    hxf = h;
-  }
-  __device__ __forceinline__ float ox(int m) const {
-    float sel = (float)((m >= b2) - (m < b1));
-    return __builtin_fmaf(sel, hxf, oM);
-  }
-};
-
-// =============================================================================
-// Chimera pass 1 (ChimeraSystem.cpp:391-424)
-// =============================================================================
-template <int PREC>
-struct ChimAcc;
-
-// FP64 reference form: A += sin(phi_j - phi_i - alpha) over |r_ij| <= rho
-template <>
-struct ChimAcc<P64> {
-  double xi, yi, pi, A;
-  int n;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = s.p[e];
-    A = 0.0;
-    n = 0;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if (r2 <= P.rho2) {
-      A += sin(s.p[m] - pi - P.alpha);
-      ++n;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double, Pass1Out &o) const {
-    o.phase = pi;
-    o.n = n;
-    o.n2 = 0;
-    o.written = n != 0;
-    o.vortex_add = 0;
-    o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
-  }
-};
-
-// FP64 factored: sum sin(phi_j - phi_i - alpha)
-//   = cos(phi_i + alpha) sum sin phi_j - sin(phi_i + alpha) sum cos phi_j
-template <>
-struct ChimAcc<P64F> {
-  double xi, yi, pi, S, C;
-  int n;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = t.p[k];
-    S = 0.0;
-    C = 0.0;
-    n = 0;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if (r2 <= P.rho2) {
-      S += s.s[m];
-      C += s.c[m];
-      ++n;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double, Pass1Out &o) const {
-    double sa, ca;
-    sincos(pi + P.alpha, &sa, &ca);
-    double A = ca * S - sa * C;
-    o.phase = pi;
-    o.n = n;
-    o.n2 = 0;
-    o.written = n != 0;
-    o.vortex_add = 0;
-    o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
-  }
-};
-
-// FP32 + Kahan over cell-local coordinates; cutoff band re-decided in FP64
-template <>
-struct ChimAcc<P32K> {
-  float xl, yl;
-  ColOffsets co;
-  Kahan S, C;
-  int n, k;
-  const double *tx, *ty;
-  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P) {
-    float4 v = s.f4[e];
-    xl = v.x;
-    yl = v.y;
-    co.set(xl, P.hxf);
-    S.init();
-    C.init();
-    n = 0;
-    k = kk;
-    tx = t.x;
-    ty = t.y;
-  }
-  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
-    co.oy = (float)dyo * P.hyf - yl;
-    co.b1 = b1;
-    co.b2 = b2;
-  }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
-    float4 v = s.f4[m];
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
-    float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool in = r2 <= P.rho2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
-      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
-    S.add(in ? v.w : 0.0f);
-    C.add(in ? v.z : 0.0f);
-    n += in;
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
-    double sa, ca;
-    sincos(phase + P.alpha, &sa, &ca);
-    double A = ca * S.value() - sa * C.value();
-    o.phase = phase;
-    o.n = n;
-    o.n2 = 0;
-    o.written = n != 0;
-    o.vortex_add = 0;
-    o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
-  }
-};
-
-// =============================================================================
-// Vortex arrays pass 1 (VortexArraysSystemPtr.cpp:459-504)
-// =============================================================================
-template <int PREC>
-struct VortAcc;
-
-template <>
-struct VortAcc<P64> {
-  double xi, yi, pi, A, B, Cc;
-  int n0, n1, n2;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = s.p[e];
-    A = B = Cc = 0.0;
-    n0 = n1 = n2 = 0;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if (r2 > P.outer2 * 1.000000000001) return;  // cannot pass the sqrt tests below
-    double dr = sqrt(r2);
-    double pj = s.p[m];
-    if (dr <= P.xi_a) {
-      A += vortex_w(P, dr) * sin(pj - pi);
-      ++n0;
-    } else if (dr <= P.rho) {
-      B += vortex_w(P, dr) * sin(pj - pi);
-      ++n1;
-    }
-    if (dr <= P.xi_r) {
-      Cc += -P.kappa * sin(atan2(dy, dx) - pi);
-      ++n2;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &, double, Pass1Out &o) const {
-    o.phase = pi;
-    o.n = n0 + n1;
-    o.n2 = n2;
-    double n01 = (double)n0 + (double)n1;
-    if (n0 + n1 > 0) {
-      o.aphi = (A + B) / n01;
-      if (n2 > 0) o.aphi += Cc / (double)n2;
-      o.written = 1;
-      o.vortex_add = 0;
-    } else {
-      o.aphi = 0.0;
-      o.written = 0;
-      o.vortex_add = n2 > 0;
-      o.coh = n2 > 0 ? Cc / (double)n2 : 0.0;
-    }
-  }
-};
-
-// factored: sum W sin(phi_j - phi_i) = cos phi_i sum W sin phi_j
-//   - sin phi_i sum W cos phi_j;  sin(atan2(dy, dx) - phi_i) = (dy cos phi_i - dx sin phi_i) / r
-template <>
-struct VortAcc<P64F> {
-  double xi, yi, pi, SW, CW, Tx, Ty;
-  int n01, n2;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = t.p[k];
-    SW = CW = Tx = Ty = 0.0;
-    n01 = n2 = 0;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if (r2 > P.outer2 * 1.000000000001) return;
-    double dr = sqrt(r2);
-    if (dr <= P.rho) {
-      double w = vortex_w(P, dr);
-      SW += w * s.s[m];
-      CW += w * s.c[m];
-      ++n01;
-    }
-    if (dr <= P.xi_r) {
-      if (dr > 0.0) {
-        Tx += dx / dr;
-        Ty += dy / dr;
-      } else {
-        Tx += 1.0;  // atan2(+0, +0) = 0
-      }
-      ++n2;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double, Pass1Out &o) const {
-    double sp, cp;
-    sincos(pi, &sp, &cp);
-    o.phase = pi;
-    o.n = n01;
-    o.n2 = n2;
-    double coh = n2 > 0 ? -P.kappa * (cp * Ty - sp * Tx) / (double)n2 : 0.0;
-    if (n01 > 0) {
-      o.aphi = (cp * SW - sp * CW) / (double)n01 + coh;
-      o.written = 1;
-      o.vortex_add = 0;
-    } else {
-      o.aphi = 0.0;
-      o.written = 0;
-      o.vortex_add = n2 > 0;
-      o.coh = coh;
-    }
-  }
-};
-
-template <>
-struct VortAcc<P32K> {
-  float xl, yl;
-  ColOffsets co;
-  Kahan SW, CW, Tx, Ty;
-  int n01, n2, k;
-  const double *tx, *ty;
-  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P) {
-    float4 v = s.f4[e];
-    xl = v.x;
-    yl = v.y;
-    co.set(xl, P.hxf);
-    SW.init();
-    CW.init();
-    Tx.init();
-    Ty.init();
-    n01 = n2 = 0;
-    k = kk;
-    tx = t.x;
-    ty = t.y;
-  }
-  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
-    co.oy = (float)dyo * P.hyf - yl;
-    co.b1 = b1;
-    co.b2 = b2;
-  }
-  // Branch-free except for the rare cases (cutoff band, cohesion range): every
-  // lane evaluates W and adds W sin, W cos masked to zero outside rho.
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
-    float4 v = s.f4[m];
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
-    float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f || __builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f) {
-      // the reference tests sqrt(r^2) <= cutoff in FP64 (VortexArraysSystemPtr.cpp:473-488)
-      double d = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy));
-      inr = d <= P.rho;
-      inc = d <= P.xi_r;
-    }
-    float r = __builtin_amdgcn_sqrtf(r2);
-    float w_in = __builtin_fmaf(-P.mu_p_over_xa2f, r2, P.mu_pf);
-    float w_an = P.c4f * (r - P.xaf) * (P.rhof - r);
-    float w = r2 <= P.xa2f ? w_in : w_an;
-    w = inr ? w : 0.0f;
-    SW.add(w * v.w);
-    CW.add(w * v.z);
-    n01 += inr;
-    if (inc) {
-      if (r2 >= P.close2f) {
-        float ir = __builtin_amdgcn_rsqf(r2);
-        Tx.add(dx * ir);
-        Ty.add(dy * ir);
-      } else {
-        double ddx, ddy;
-        exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
-        double dr = sqrt(ddx * ddx + ddy * ddy);
-        if (dr > 0.0) {
-          Tx.add((float)(ddx / dr));
-          Ty.add((float)(ddy / dr));
-        } else {
-          Tx.add(1.0f);  // atan2(+0, +0) = 0
-        }
-      }
-      ++n2;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
-    double sp, cp;
-    sincos(phase, &sp, &cp);
-    o.phase = phase;
-    o.n = n01;
-    o.n2 = n2;
-    double coh = n2 > 0 ? -P.kappa * (cp * Ty.value() - sp * Tx.value()) / (double)n2 : 0.0;
-    if (n01 > 0) {
-      o.aphi = (cp * SW.value() - sp * CW.value()) / (double)n01 + coh;
-      o.written = 1;
-      o.vortex_add = 0;
-    } else {
-      o.aphi = 0.0;
-      o.written = 0;
-      o.vortex_add = n2 > 0;
-      o.coh = coh;
-    }
-  }
-};
-
-// =============================================================================
-// Chimera pass 2 (Ito form, ChimeraSystem.cpp:589-644): over the same partners,
-// cos(phi_j - phi_i - alpha) / n_i and (a_j - a_i) cos(...) / n_i (no sigma in
-// these terms, as in the reference)
-// =============================================================================
-struct Pass2Out {
-  double phase, a0, a1, ai;
-};
-
-// Per-particle pass-2 inputs: pass-1 phi derivative, partner counts (within
-// rho; cohesion), and the additional-derivative values before the call.
-struct Pass2In {
-  double ai, n, n2, in0, in1;
-};
-
-template <int PREC>
-struct Chim2Acc;
-
-template <>
-struct Chim2Acc<P64> {
-  double xi, yi, pi, ai, n, a0, a1;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &, const Pass2In &in) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = s.p[e];
-    ai = in.ai;
-    n = in.n;
-    a0 = in.in0;
-    a1 = in.in1;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if ((r2 <= P.rho2) && (n != 0)) {
-      double cph = cos(s.p[m] - pi - P.alpha);
-      a0 += cph / n;
-      a1 += (s.aj[m] - ai) * cph / n;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
-    o.phase = pi;
-    o.ai = ai;
-    o.a0 = a0 * -sqrt(2.0 * P.D);
-    o.a1 = a1 + -2.0 * P.D * ai;
-  }
-};
-
-template <>
-struct Chim2Acc<P64F> {
-  double xi, yi, pi, ai, n, in0, in1, C, S, Pc, Qs;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &, const Pass2In &in) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = t.p[k];
-    ai = in.ai;
-    n = in.n;
-    in0 = in.in0;
-    in1 = in.in1;
-    C = S = Pc = Qs = 0.0;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if (r2 <= P.rho2) {
-      double aj = s.aj[m];
-      C += s.c[m];
-      S += s.s[m];
-      Pc += aj * s.c[m];
-      Qs += aj * s.s[m];
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
-    double sa, ca;
-    sincos(pi + P.alpha, &sa, &ca);
-    double a0 = in0, a1 = in1;
-    if (n != 0) {
-      double sc = ca * C + sa * S;
-      a0 += sc / n;
-      a1 += (ca * Pc + sa * Qs - ai * sc) / n;
-    }
-    o.phase = pi;
-    o.ai = ai;
-    o.a0 = a0 * -sqrt(2.0 * P.D);
-    o.a1 = a1 + -2.0 * P.D * ai;
-  }
-};
-
-template <>
-struct Chim2Acc<P32K> {
-  float xl, yl;
-  ColOffsets co;
-  double ai, n, in0, in1;
-  Kahan C, S, Pc, Qs;
-  int k;
-  const double *tx, *ty;
-  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P, const Pass2In &in) {
-    float4 v = s.f4[e];
-    xl = v.x;
-    yl = v.y;
-    co.set(xl, P.hxf);
-    ai = in.ai;
-    n = in.n;
-    in0 = in.in0;
-    in1 = in.in1;
-    C.init();
-    S.init();
-    Pc.init();
-    Qs.init();
-    k = kk;
-    tx = t.x;
-    ty = t.y;
-  }
-  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
-    co.oy = (float)dyo * P.hyf - yl;
-    co.b1 = b1;
-    co.b2 = b2;
-  }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
-    float4 v = s.f4[m];
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
-    float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool in = r2 <= P.rho2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
-      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
-    float aj = in ? s.aj[m] : 0.0f;
-    float cz = in ? v.z : 0.0f, sw = in ? v.w : 0.0f;
-    C.add(cz);
-    S.add(sw);
-    Pc.add(aj * cz);
-    Qs.add(aj * sw);
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass2Out &o) const {
-    double sa, ca;
-    sincos(phase + P.alpha, &sa, &ca);
-    double a0 = in0, a1 = in1;
-    if (n != 0) {
-      double sc = ca * C.value() + sa * S.value();
-      a0 += sc / n;
-      a1 += (ca * Pc.value() + sa * Qs.value() - ai * sc) / n;
-    }
-    o.phase = phase;
-    o.ai = ai;
-    o.a0 = a0 * -sqrt(2.0 * P.D);
-    o.a1 = a1 + -2.0 * P.D * ai;
-  }
-};
-
-// =============================================================================
-// Vortex-arrays pass 2 (Ito form, VortexArraysSystemPtr.cpp:662-733): over the
-// partners within rho, -W cos(dphi) / n01 and the drift derivative of
-// W(r) sin(dphi); over the partners within xi_r, the cohesion counterparts;
-// additional[0] is scaled by sqrt(2 D) at the end.  Velocities are
-// v (cos phi, sin phi) of the stage phases, as the pass-1 derivative holds them.
-// =============================================================================
-template <int PREC>
-struct Vort2Acc;
-
-// FP64 reference form
-template <>
-struct Vort2Acc<P64> {
-  double xi, yi, pi, vxi, vyi, ai, n1, n2, a0, a1;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int, const Tiles &, const KParams &P, const Pass2In &in) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = s.p[e];
-    vxi = P.v0 * cos(pi);
-    vyi = P.v0 * sin(pi);
-    ai = in.ai;
-    n1 = in.n;
-    n2 = in.n2;
-    a0 = in.in0;
-    a1 = in.in1;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if (r2 > P.outer2 * 1.000000000001) return;
-    double dr = sqrt(r2);
-    const bool al = (dr <= P.rho) && (n1 > 0.0), co = (dr <= P.xi_r) && (n2 > 0.0);
-    if (!al && !co) return;
-    const double pj = s.p[m];
-    const double vxj = P.v0 * cos(pj), vyj = P.v0 * sin(pj);
-    if (al) {
-      double cph = cos(pj - pi);
-      double sph = sin(pj - pi);
-      double w = vortex_w(P, dr);
-      double dw = vortex_dw(P, dr);
-      a0 += -w * cph / n1;
-      a1 += ((vxj - vxi) * dw * dx / dr + (vyj - vyi) * dw * dy / dr) * sph / n1 + (s.aj[m] - ai) * w * cph / n1 -
-            2.0 * P.D * sph * w / n1;
-    }
-    if (co) {
-      double th = atan2(dy, dx);
-      double cpp = cos(th - pi);
-      double spp = sin(th - pi);
-      a0 += P.kappa * cpp / n2;
-      a1 += -P.kappa * ((vxj - vxi) * (-dy) / dr / dr + (vyj - vyi) * dx / dr / dr - ai) * cpp / n2 +
-            P.kappa * P.D * spp / n2;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
-    o.phase = pi;
-    o.ai = ai;
-    o.a0 = a0 * sqrt(2.0 * P.D);
-    o.a1 = a1;
-  }
-};
-
-// Factored FP64 form: cos, sin of (phi_j - phi_i) and of (atan2(dy, dx) - phi_i)
-// from the tile's cos phi, sin phi and dx / r, dy / r, no per-pair trig; the
-// sums are divided by the counts once.
-template <>
-struct Vort2Acc<P64F> {
-  double xi, yi, pi, ci, si, ai, n1, n2, in0, in1, A0, A1, C0, C1;
-  __device__ __forceinline__ void init(const Src64 &s, int e, int k, const Tiles &t, const KParams &, const Pass2In &in) {
-    xi = s.x[e];
-    yi = s.y[e];
-    pi = t.p[k];
-    ci = s.c[e];
-    si = s.s[e];
-    ai = in.ai;
-    n1 = in.n;
-    n2 = in.n2;
-    in0 = in.in0;
-    in1 = in.in1;
-    A0 = A1 = C0 = C1 = 0.0;
-  }
-  __device__ __forceinline__ void row(int, int, int, const KParams &) {}
-  __device__ __forceinline__ void pair(const Src64 &s, int m, const KParams &P) {
-    double dx = min_image(s.x[m] - xi, P.Lx, P.rLx);
-    double dy = min_image(s.y[m] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if (r2 > P.outer2 * 1.000000000001) return;
-    double dr = sqrt(r2);
-    const bool al = dr <= P.rho, co = dr <= P.xi_r;
-    if (!al && !co) return;
-    const double cj = s.c[m], sj = s.s[m];
-    const double dvx = P.v0 * cj - P.v0 * ci, dvy = P.v0 * sj - P.v0 * si;
-    if (al) {
-      double cph = cj * ci + sj * si;
-      double sph = sj * ci - cj * si;
-      double w = vortex_w(P, dr);
-      double g = vortex_dw(P, dr) / dr;
-      A0 += w * cph;
-      A1 += (dvx * dx + dvy * dy) * g * sph + (s.aj[m] - ai) * w * cph - 2.0 * P.D * sph * w;
-    }
-    if (co) {
-      // atan2(+0, +0) = 0 for a coincident partner
-      double cpp = dr > 0.0 ? (dx * ci + dy * si) / dr : ci;
-      double spp = dr > 0.0 ? (dy * ci - dx * si) / dr : -si;
-      C0 += cpp;
-      C1 += -P.kappa * ((dvy * dx - dvx * dy) / r2 - ai) * cpp + P.kappa * P.D * spp;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double, Pass2Out &o) const {
-    double a0 = in0, a1 = in1;
-    if (n1 > 0.0) {
-      a0 += -A0 / n1;
-      a1 += A1 / n1;
-    }
-    if (n2 > 0.0) {
-      a0 += P.kappa * C0 / n2;
-      a1 += C1 / n2;
-    }
-    o.phase = pi;
-    o.ai = ai;
-    o.a0 = a0 * sqrt(2.0 * P.D);
-    o.a1 = a1;
-  }
-};
-
-// FP32 form over cell-local coordinates with Kahan sums; cutoff decisions
-// (rho, xi_r, and the xi_a branch of W') within the band are taken in FP64.
-template <>
-struct Vort2Acc<P32K> {
-  float xl, yl, cif, sif, aif;
-  ColOffsets co;
-  double ai, n1, n2, in0, in1, C0, C1;
-  Kahan A0, A1;
-  int k;
-  const double *tx, *ty;
-  __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P, const Pass2In &in) {
-    float4 v = s.f4[e];
-    xl = v.x;
-    yl = v.y;
-    cif = v.z;
-    sif = v.w;
-    co.set(xl, P.hxf);
-    ai = in.ai;
-    aif = (float)in.ai;
-    n1 = in.n;
-    n2 = in.n2;
-    in0 = in.in0;
-    in1 = in.in1;
-    A0.init();
-    A1.init();
-    C0 = C1 = 0.0;
-    k = kk;
-    tx = t.x;
-    ty = t.y;
-  }
-  __device__ __forceinline__ void row(int dyo, int b1, int b2, const KParams &P) {
-    co.oy = (float)dyo * P.hyf - yl;
-    co.b1 = b1;
-    co.b2 = b2;
-  }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
-    float4 v = s.f4[m];
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
-    float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f, inner = r2 < P.xa2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f || __builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f ||
-        __builtin_fabsf(r2 - P.xa2f) <= P.band_xa2f) {
-      double d = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy));
-      inr = d <= P.rho;
-      inc = d <= P.xi_r;
-      inner = d < P.xi_a;
-    }
-    if (!inr && !inc) return;
-    const float cj = v.z, sj = v.w;
-    const float dvx = P.v0f * cj - P.v0f * cif, dvy = P.v0f * sj - P.v0f * sif;
-    const float ir = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-    if (inr) {
-      float r = r2 * ir;
-      float cph = cj * cif + sj * sif;
-      float sph = sj * cif - cj * sif;
-      float w = inner ? __builtin_fmaf(-P.mu_p_over_xa2f, r2, P.mu_pf) : P.c4f * (r - P.xaf) * (P.rhof - r);
-      // W'(r) / r
-      float g = inner ? -2.0f * P.mu_p_over_xa2f : P.c4f * (P.rhof + P.xaf - 2.0f * r) * ir;
-      float aj = s.aj[m];
-      A0.add(w * cph);
-      A1.add((dvx * dx + dvy * dy) * g * sph + (aj - aif) * w * cph - 2.0f * P.Df * sph * w);
-    }
-    if (inc) {
-      double cpp, spp, q;
-      if (r2 >= P.close2f) {
-        cpp = (dx * cif + dy * sif) * ir;
-        spp = (dy * cif - dx * sif) * ir;
-        q = (dvy * dx - dvx * dy) * ir * ir;
-      } else {
-        double ddx, ddy;
-        exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
-        const double dr2 = ddx * ddx + ddy * ddy, dr = sqrt(dr2);
-        const double cid = cif, sid = sif;
-        cpp = dr > 0.0 ? (ddx * cid + ddy * sid) / dr : cid;
-        spp = dr > 0.0 ? (ddy * cid - ddx * sid) / dr : -sid;
-        q = ((double)dvy * ddx - (double)dvx * ddy) / dr2;
-      }
-      C0 += cpp;
-      C1 += -P.kappa * (q - ai) * cpp + P.kappa * P.D * spp;
-    }
-  }
-  __device__ __forceinline__ void finish(const KParams &P, double phase, Pass2Out &o) const {
-    double a0 = in0, a1 = in1;
-    if (n1 > 0.0) {
-      a0 += -A0.value() / n1;
-      a1 += A1.value() / n1;
-    }
-    if (n2 > 0.0) {
-      a0 += P.kappa * C0 / n2;
-      a1 += C1 / n2;
-    }
-    o.phase = phase;
-    o.ai = ai;
-    o.a0 = a0 * sqrt(2.0 * P.D);
-    o.a1 = a1;
-  }
-};
-
-// =============================================================================
-// Epilogues
-// =============================================================================
-struct EpiCtx {
-  double *ox, *oy, *op;           // k_next / derivative / k_4 by particle (phi in/out)
-  const double *k1x, *k1y, *k1p;  // RK4 combine inputs by particle
-  const double *k2x, *k2y, *k2p;
-  const double *k3x, *k3y, *k3p;
-  double *sx, *sy, *sp;           // master state by particle
-  double *aphi_slot, *n_slot;     // pass-1 results by slot (pass-2 inputs)
-  double *n2_slot;                // vortex cohesion partner counts by slot
-  float *aphi_slot_f;
-  double *a0, *a1;                // additional by particle (in/out)
-  StepArgs sa;
-};
-
-__device__ __forceinline__ void wrap_state(const KParams &P, double &x, double &y) {
-  // the array form of the wrap, applied to every particle (PeriodicBoundaryConditions.cpp:133-141)
-  x = x - floor(x * P.rLx) * P.Lx;
-  y = y - floor(y * P.rLy) * P.Ly;
-}
-
-// Final phi derivative given the in/out value `old` (k_next or derivative entry
-// before the call): the reference writes it only when partners exist.
-__device__ __forceinline__ double phi_out(const Pass1Out &o, double old) {
-  if (o.written) return o.aphi;
-  if (o.vortex_add) return old + o.coh;
-  return old;
-}
-
-template <int EPI>
-__device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int h, int id, int slot, const Pass1Out &o) {
-  double sn, cs;
-  sincos(o.phase, &sn, &cs);
-  double ax = P.v0 * cs, ay = P.v0 * sn;
-  if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
-    double ap = phi_out(o, e.sa.zero_in ? 0.0 : e.op[h]);
-    e.ox[h] = ax;
-    e.oy[h] = ay;
-    e.op[h] = ap;
-    if (EPI == EPI_ITO1) {
-      e.aphi_slot[slot] = ap;
-      e.aphi_slot_f[slot] = (float)ap;
-      e.n_slot[slot] = (double)o.n;
-      e.n2_slot[slot] = (double)o.n2;
-    }
-    if (EPI == EPI_RK4_FINAL) {
-      // RungeKutta4StepperPtr.cpp:72-77: (k1 + 2 k2 + 2 k3 + k4) * dt / 6 for
-      // x, y and (...) / 6 * dt for phi, evaluated left to right
-      const double dt = e.sa.dt;
-      double x = e.sx[h], y = e.sy[h], p = e.sp[h];
-      x += (e.k1x[h] + 2.0 * e.k2x[h] + 2.0 * e.k3x[h] + ax) * dt / 6.0;
-      y += (e.k1y[h] + 2.0 * e.k2y[h] + 2.0 * e.k3y[h] + ay) * dt / 6.0;
-      p += (e.k1p[h] + 2.0 * e.k2p[h] + 2.0 * e.k3p[h] + ap) / 6.0 * dt;
-      if (e.sa.wrap) wrap_state(P, x, y);
-      e.sx[h] = x;
-      e.sy[h] = y;
-      e.sp[h] = p;
-    }
-  } else if (EPI == EPI_SE) {
-    // StochasticEulerStepperPtr.hpp:44-53 with a zero-filled derivative and the
-    // natural frequency in the drift as (omega + a_phi) dt
-    double ap = phi_out(o, 0.0);
-    double xi;
-    if (e.sa.noise) {
-      xi = e.sa.noise[id];  // materialised noise is indexed by particle id
-    } else {
-      double z1;
-      normal_pair(e.sa.seed, e.sa.step, kStreamStep, (uint32_t)id, xi, z1);
-    }
-    const double dt = e.sa.dt;
-    double dW = e.sa.sqdt * xi;
-    double w = e.sa.omega ? e.sa.omega[h] : 0.0;
-    double x = e.sx[h], y = e.sy[h], p = e.sp[h];
-    x += ax * dt;
-    y += ay * dt;
-    p += (w + ap) * dt + e.sa.sq2D * dW;
-    if (e.sa.wrap) wrap_state(P, x, y);
-    e.sx[h] = x;
-    e.sy[h] = y;
-    e.sp[h] = p;
-  }
-}
-
-template <int EPI>
-__device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int h, int id, const Pass2Out &o) {
-  if (EPI == EPI_ITO2) {
-    e.a0[h] = o.a0;
-    e.a1[h] = o.a1;
-  } else {
-    // StochasticRungeKuttaStepperPtr.hpp:52-66
-    double sn, cs;
-    sincos(o.phase, &sn, &cs);
-    const double ax = P.v0 * cs, ay = P.v0 * sn, ap = o.ai;
-    double U_1, U_2;
-    if (e.sa.noise) {
-      U_1 = e.sa.noise[2 * id];
-      U_2 = e.sa.noise[2 * id + 1];
-    } else {
-      normal_pair(e.sa.seed, e.sa.step, kStreamStep, (uint32_t)id, U_1, U_2);
-    }
-    const double dt = e.sa.dt, D = e.sa.D, sq2D = e.sa.sq2D;
-    double delta_W = U_1 * e.sa.sqdt;
-    double delta_Z = e.sa.h1 * (U_1 + U_2 / e.sa.sqrt3);
-    double w = e.sa.omega ? e.sa.omega[h] : 0.0;
-    double x = e.sx[h], y = e.sy[h], p = e.sp[h];
-    x += ax * dt - sq2D * ay * delta_Z - 0.5 * (ap * ay + D * ax) * dt * dt;
-    y += ay * dt + sq2D * ax * delta_Z + 0.5 * (ap * ax - D * ay) * dt * dt;
-    p += (w + ap) * dt + sq2D * delta_W + o.a0 * delta_Z + 0.5 * o.a1 * dt * dt;
-    if (e.sa.wrap) wrap_state(P, x, y);
-    e.sx[h] = x;
-    e.sy[h] = y;
-    e.sp[h] = p;
-  }
-}
-
-// Pair counters: workgroup totals added to one of kCountSlots slot triples
-// (blockIdx modulo kCountSlots), so that the atomics of a launch spread over
-// many addresses; the host sums the slots.  Called by every thread.
-__device__ __forceinline__ void count_add(unsigned long long *counts, long long a, long long b, long long c) {
-  __shared__ long long part[3][kBlock / 64];
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    a += __shfl_xor(a, d, 64);
-    b += __shfl_xor(b, d, 64);
-    c += __shfl_xor(c, d, 64);
-  }
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    part[0][w] = a;
-    part[1][w] = b;
-    part[2][w] = c;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    long long v = 0;
-#pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) v += part[threadIdx.x][q];
-    atomicAdd(&counts[3 * (blockIdx.x % kCountSlots) + threadIdx.x], (unsigned long long)v);
-  }
-}
 
 // =============================================================================
 // Row-segment setup and LDS staging
@@ -980,13 +56,6 @@ struct SegShared {
   int gst[kMaxSlots];      // its first global slot
   int wtot[kBlock / 64];
 };
-
-// Blocks are dealt to the 8 XCDs round robin; map them so that each XCD takes
-// a contiguous run of row segments and its L2 sees neighbouring halo rows.
-__device__ __forceinline__ int xcd_segment(int b, int nb) {
-  const int per = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
-  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
 
 struct Seg {
   int cy, cx0, W, ncol, ns, total;
@@ -1428,54 +497,9 @@ __global__ __launch_bounds__(kBlock) void k_pass2_allpairs(KParams P, const doub
 // dispatch
 // =============================================================================
 
-EpiCtx make_ctx(Sim &s, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
-                double *const k2[3], double *const k3[3]) {
-  EpiCtx e{};
-  e.ox = ox;
-  e.oy = oy;
-  e.op = op;
-  if (k1) {
-    e.k1x = k1[0];
-    e.k1y = k1[1];
-    e.k1p = k1[2];
-    e.k2x = k2[0];
-    e.k2y = k2[1];
-    e.k2p = k2[2];
-    e.k3x = k3[0];
-    e.k3y = k3[1];
-    e.k3p = k3[2];
-  }
-  e.sx = s.sx;
-  e.sy = s.sy;
-  e.sp = s.sp;
-  e.aphi_slot = s.aphi_slot;
-  e.n_slot = s.n_slot;
-  e.n2_slot = s.n2_slot;
-  e.aphi_slot_f = s.aphi_slot_f;
-  e.a0 = s.add[0];
-  e.a1 = s.add[1];
-  e.sa = sa;
-  return e;
-}
-
 template <int PREC, int PASS>
 size_t lds_bytes(const Sim &s) {
   return (size_t)s.kp.lds_cap * LdsTile<PREC, PASS>::bytes;
-}
-
-// Launches a pair kernel on the handle's stream; with profiling on, the HIP
-// events are bound to the kernel's own dispatch (hipExtLaunchKernelGGL), so
-// their elapsed time is the kernel's execution time.
-template <typename... KArgs, typename... Args>
-void launch_pair(Sim &s, void (*kern)(KArgs...), dim3 grid, dim3 blk, size_t sh, Args... args) {
-  if (s.profiling) {
-    hipEvent_t b, e;
-    s.prof_take(b, e);
-    hipExtLaunchKernelGGL(kern, grid, blk, sh, s.stream, b, e, 0, args...);
-    s.prof_pending.emplace_back(b, e);
-  } else {
-    hipLaunchKernelGGL(kern, grid, blk, sh, s.stream, args...);
-  }
 }
 
 template <int PREC, int SYS, int EPI>

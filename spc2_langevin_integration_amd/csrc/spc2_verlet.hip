@@ -1,0 +1,350 @@
+// Verlet neighbour lists on gfx950 (the list the reference keeps in
+// ChimeraSystemPtr::EvaluateInteractionsWithVerletNeighborList,
+// ChimeraSystemPtr.cpp:650-927): every pair within rho_out + skin of the
+// positions at the build, found over a 3 x 3 stencil of cells of side
+// >= rho_out + skin.  The list stays valid while no particle has moved more
+// than skin / 2 since the build; the RHS kernels then visit ~ (1 + skin/rho)^2
+// pi / 9 of the cell-stencil candidates and skip the per-RHS cell build.
+//
+// Layout.  At a build the particle arrays are re-homed into the build's cell
+// order, so home h is also the tile index and homes of one cell are
+// contiguous.  The list is wavefront-interleaved ("wave-ELL"): the entries of
+// the 64 homes of wavefront w occupy rows wrow[w] .. wrow[w+1]-1, entry e of
+// lane l at (wrow[w] + e) * 64 + l, so a wavefront reads its e-th entries as
+// one 256-byte line.  Each entry packs the neighbour's home (bits 4..31) and
+// its stencil cell offset (dcx + 1) | (dcy + 1) << 2, which the FP32 form needs
+// to turn cell-local coordinates into a separation.
+//
+// Per RHS a stage kernel writes the tiles by home (FP64 wrapped stage position
+// for the cutoff decisions, and the precision's own tile), then one pass kernel
+// per RHS pass runs the same accumulators and epilogues as the cell-stencil
+// kernels.  Compiled with -ffp-contract=off.
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "spc2_device.h"
+#include "spc2_internal.h"
+#include "spc2_pair_accumulators.h"
+
+namespace spc2 {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// Visits the candidates of home i in a 3 x 3 stencil of build cells (cells of
+// side >= rlist), ascending home within each cell, in the reference's stencil
+// order; calls f(j, code) for every j != i within rlist (FP64, min image of
+// the wrapped build positions).
+template <class F>
+__device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const int *start, const Tiles &t, int i,
+                                                         double rlist2, F &&f) {
+  const int c = t.cell[i];
+  const int cx = c % P.nx, cy = c / P.nx;
+  const double xi = t.x[i], yi = t.y[i];
+  for (int dyo = -1; dyo <= 1; ++dyo) {
+    const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
+    for (int dxo = -1; dxo <= 1; ++dxo) {
+      const int jc = row + adjust_cell(cx + dxo, P.nx);
+      const int code = (dxo + 1) | ((dyo + 1) << 2);
+      for (int j = start[jc], e = start[jc + 1]; j < e; ++j) {
+        if (j == i) continue;
+        double dx = min_image(t.x[j] - xi, P.Lx, P.rLx);
+        double dy = min_image(t.y[j] - yi, P.Ly, P.rLy);
+        if (dx * dx + dy * dy <= rlist2) f(j, code);
+      }
+    }
+  }
+}
+
+// Build pass 1: entries per home and the largest count of each wavefront.
+__global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *start, Tiles t, double rlist2,
+                                                         int *vcnt, int *wmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int cnt = 0;
+  if (i < P.n) {
+    for_each_build_neighbour(P, start, t, i, rlist2, [&](int, int) { ++cnt; });
+    vcnt[i] = cnt;
+  }
+  int m = cnt;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+  const int w = i >> 6;
+  if ((threadIdx.x & 63) == 0 && w * 64 < P.n) wmax[w] = m;
+}
+
+// Build pass 2: the entries, in the same order as the count.
+__global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *start, Tiles t, double rlist2,
+                                                        const int *wrow, int *vlist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  int *dst = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+  int e = 0;
+  for_each_build_neighbour(P, start, t, i, rlist2, [&](int j, int code) {
+    dst[(size_t)e * 64] = (j << 4) | code;
+    ++e;
+  });
+}
+
+// Per-RHS stage: s + (k_coef dt) k by home, the FP64 tile (conditionally
+// wrapped, ChimeraSystem.cpp:291-309) and the precision's own tile.  FP32
+// coordinates are relative to the particle's build cell, taken at the nearest
+// periodic image (a particle is within skin / 2 of its build position).
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_verlet_stage(KParams P, const double *sx, const double *sy,
+                                                         const double *sp, const double *kx, const double *ky,
+                                                         const double *kp, double kcdt, Tiles t, double *gp) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= P.n) return;
+  double x = sx[h], y = sy[h], p = sp[h];
+  if (kx) {
+    x = x + kcdt * kx[h];
+    y = y + kcdt * ky[h];
+    p = p + kcdt * kp[h];
+  }
+  Box b{P.Lx, P.Ly, P.rLx, P.rLy};
+  cond_wrap(b, x, y);
+  t.x[h] = x;
+  t.y[h] = y;
+  gp[h] = p;
+  if (PREC == P32K) {
+    const int c = t.cell[h];
+    double xl = x - (double)(c % P.nx) * P.hx, yl = y - (double)(c / P.nx) * P.hy;
+    if (xl > 0.5 * P.Lx) xl -= P.Lx;
+    else if (xl < -0.5 * P.Lx) xl += P.Lx;
+    if (yl > 0.5 * P.Ly) yl -= P.Ly;
+    else if (yl < -0.5 * P.Ly) yl += P.Ly;
+    double sn, cs;
+    sincos(p, &sn, &cs);
+    t.f4[h] = make_float4((float)xl, (float)yl, (float)cs, (float)sn);
+  } else {
+    t.p[h] = p;
+    if (PREC == P64F) {
+      double sn, cs;
+      sincos(p, &sn, &cs);
+      t.c[h] = cs;
+      t.s[h] = sn;
+    }
+  }
+}
+
+// One list entry into the accumulator: the FP32 form learns the stencil cell
+// offset through row() and the column bounds (sel = (m >= b2) - (m < b1)).
+template <class Acc, class S>
+__device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, const KParams &P) {
+  const int j = ent >> 4, dcx = (ent & 3) - 1, dcy = ((ent >> 2) & 3) - 1;
+  const int b1 = dcx < 0 ? j + 1 : (dcx == 0 ? j : INT_MIN);
+  const int b2 = dcx < 0 ? INT_MAX : (dcx == 0 ? j + 1 : j);
+  acc.row(dcy, b1, b2, P);
+  acc.pair(src, j, P);
+}
+
+template <int PREC>
+__device__ __forceinline__ typename SrcOf<PREC>::type verlet_src(const Tiles &t, const EpiCtx &e, int pass) {
+  if constexpr (PREC == P32K) {
+    return Src32{t.f4, nullptr, pass == 2 ? e.aphi_slot_f : nullptr};
+  } else {
+    return Src64{t.x, t.y, t.p, t.c, t.s, pass == 2 ? e.aphi_slot : nullptr};
+  }
+}
+
+// Home of the thread: workgroups are dealt to the 8 XCDs round robin; each XCD
+// takes a contiguous run of homes so that its L2 holds neighbouring tiles.
+__device__ __forceinline__ int verlet_home() {
+  return xcd_segment(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+}
+
+template <class Acc, int PREC, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass1_verlet(KParams P, const int *__restrict__ vcnt,
+                                                         const int *__restrict__ wrow, const int *__restrict__ vlist,
+                                                         Tiles t, const int *__restrict__ hid, const double *gp,
+                                                         EpiCtx e, unsigned long long *counts) {
+  const int i = verlet_home();
+  const bool active = i < P.n;
+  Pass1Out o{};
+  int cnt = 0;
+  if (active) {
+    const auto src = verlet_src<PREC>(t, e, 1);
+    Acc acc;
+    acc.init(src, i, i, t, P);
+    cnt = vcnt[i];
+    const int *l = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+#pragma unroll 2
+    for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
+    acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
+    epilogue1<EPI>(P, e, i, hid[i], i, o);
+  }
+  if (counts) count_add(counts, cnt, active ? o.n : 0, active ? o.n2 : 0);
+}
+
+template <class Acc, int PREC, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *__restrict__ vcnt,
+                                                         const int *__restrict__ wrow, const int *__restrict__ vlist,
+                                                         Tiles t, const int *__restrict__ hid, const double *gp,
+                                                         EpiCtx e) {
+  const int i = verlet_home();
+  if (i >= P.n) return;
+  const auto src = verlet_src<PREC>(t, e, 2);
+  Pass2In in{e.aphi_slot[i], e.n_slot[i], e.n2_slot[i], 0.0, 0.0};
+  if (EPI == EPI_ITO2) {
+    in.in0 = e.a0[i];
+    in.in1 = e.a1[i];
+  }
+  Acc acc;
+  acc.init(src, i, i, t, P, in);
+  const int cnt = vcnt[i];
+  const int *l = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+#pragma unroll 2
+  for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
+  Pass2Out o;
+  acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
+  epilogue2<EPI>(P, e, i, hid[i], o);
+}
+
+// Largest squared displacement since the build (min image), two-level and
+// without atomics: per-workgroup maxima, then one workgroup.
+__global__ __launch_bounds__(kBlock) void k_drift_partial(KParams P, const double *sx, const double *sy,
+                                                          const double *xb, const double *yb, double *red) {
+  __shared__ double sm[kBlock];
+  double m = 0.0;
+  for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < P.n; h += gridDim.x * blockDim.x) {
+    double dx = min_image(sx[h] - xb[h], P.Lx, P.rLx), dy = min_image(sy[h] - yb[h], P.Ly, P.rLy);
+    m = fmax(m, dx * dx + dy * dy);
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + d]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red[blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_drift_final(int nb, const double *red, double *out) {
+  __shared__ double sm[kBlock];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) m = fmax(m, red[b]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + d]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+template <int PREC, int SYS, int EPI>
+void vpass1_typed(Sim &s, const EpiCtx &e) {
+  unsigned long long *cnt = s.counting ? s.counts : nullptr;
+  const dim3 grid(blocks_for(s.n)), blk(kBlock);
+  if (SYS == SPC2_CHIMERA)
+    launch_pair(s, k_pass1_verlet<ChimAcc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e, cnt);
+  else
+    launch_pair(s, k_pass1_verlet<VortAcc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e, cnt);
+}
+
+template <int PREC, int SYS>
+void vpass1_epi(Sim &s, int epi, const EpiCtx &e) {
+  switch (epi) {
+    case EPI_K: vpass1_typed<PREC, SYS, EPI_K>(s, e); break;
+    case EPI_ITO1: vpass1_typed<PREC, SYS, EPI_ITO1>(s, e); break;
+    case EPI_SE: vpass1_typed<PREC, SYS, EPI_SE>(s, e); break;
+    case EPI_RK4_FINAL: vpass1_typed<PREC, SYS, EPI_RK4_FINAL>(s, e); break;
+  }
+}
+
+template <int PREC>
+void vpass1_sys(Sim &s, int epi, const EpiCtx &e) {
+  if (s.P.system == SPC2_CHIMERA)
+    vpass1_epi<PREC, SPC2_CHIMERA>(s, epi, e);
+  else
+    vpass1_epi<PREC, SPC2_VORTEX>(s, epi, e);
+}
+
+template <int PREC, int SYS, int EPI>
+void vpass2_typed(Sim &s, const EpiCtx &e) {
+  const dim3 grid(blocks_for(s.n)), blk(kBlock);
+  if (SYS == SPC2_CHIMERA)
+    launch_pair(s, k_pass2_verlet<Chim2Acc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e);
+  else
+    launch_pair(s, k_pass2_verlet<Vort2Acc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e);
+}
+
+template <int PREC>
+void vpass2_sys(Sim &s, int epi, const EpiCtx &e) {
+  const bool ito = epi == EPI_ITO2;
+  if (s.P.system == SPC2_CHIMERA) {
+    if (ito) vpass2_typed<PREC, SPC2_CHIMERA, EPI_ITO2>(s, e);
+    else vpass2_typed<PREC, SPC2_CHIMERA, EPI_SRK>(s, e);
+  } else {
+    if (ito) vpass2_typed<PREC, SPC2_VORTEX, EPI_ITO2>(s, e);
+    else vpass2_typed<PREC, SPC2_VORTEX, EPI_SRK>(s, e);
+  }
+}
+
+}  // namespace
+
+void launch_verlet_count(Sim &s) {
+  const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
+  hipLaunchKernelGGL(k_verlet_count, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
+                     s.t, r2, s.vcnt, s.wmax);
+}
+
+void launch_verlet_fill(Sim &s) {
+  const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
+  hipLaunchKernelGGL(k_verlet_fill, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
+                     s.t, r2, (const int *)s.wrow, s.vlist);
+}
+
+void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt) {
+  const dim3 g(blocks_for(s.n)), b(kBlock);
+  switch (s.prec) {
+    case P64:
+      hipLaunchKernelGGL(k_verlet_stage<P64>, g, b, 0, s.stream, s.kp, s.sx, s.sy, s.sp, kx, ky, kp, kcdt, s.t, s.gp);
+      break;
+    case P64F:
+      hipLaunchKernelGGL(k_verlet_stage<P64F>, g, b, 0, s.stream, s.kp, s.sx, s.sy, s.sp, kx, ky, kp, kcdt, s.t,
+                         s.gp);
+      break;
+    default:
+      hipLaunchKernelGGL(k_verlet_stage<P32K>, g, b, 0, s.stream, s.kp, s.sx, s.sy, s.sp, kx, ky, kp, kcdt, s.t,
+                         s.gp);
+  }
+}
+
+void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
+                         double *const k2[3], double *const k3[3]) {
+  EpiCtx e = make_ctx(s, ox, oy, op, sa, k1, k2, k3);
+  switch (s.prec) {
+    case P64: vpass1_sys<P64>(s, epi, e); break;
+    case P64F: vpass1_sys<P64F>(s, epi, e); break;
+    default: vpass1_sys<P32K>(s, epi, e);
+  }
+}
+
+void launch_verlet_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa) {
+  EpiCtx e = make_ctx(s, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
+  e.a0 = a0;
+  e.a1 = a1;
+  switch (s.prec) {
+    case P64: vpass2_sys<P64>(s, epi, e); break;
+    case P64F: vpass2_sys<P64F>(s, epi, e); break;
+    default: vpass2_sys<P32K>(s, epi, e);
+  }
+}
+
+void launch_drift(Sim &s, double *out) {
+  const int nb = std::min(1024, std::max(1, blocks_for(s.n)));
+  hipLaunchKernelGGL(k_drift_partial, dim3(nb), dim3(kBlock), 0, s.stream, s.kp, s.sx, s.sy, s.xb, s.yb, s.red);
+  hipLaunchKernelGGL(k_drift_final, dim3(1), dim3(kBlock), 0, s.stream, nb, s.red, out);
+}
+
+}  // namespace spc2

@@ -1,0 +1,180 @@
+"""Verlet neighbour lists (SPC2_NEIGH_VERLET) against the oracle and against the
+linked-cell path.
+
+The list must give the linked cells' neighbour sets at every evaluation, so:
+  * pair counts (within rho, cohesion) equal the oracle's exactly, per call and
+    summed over whole runs that cross several list rebuilds;
+  * RHS values meet the same bars as the cell path (1e-12 FP64, 1e-5 FP32);
+  * trajectories equal the cell path's up to the summation order (FP64).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import spc2_langevin_integration_amd as spc2
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp64f": 1e-12, "fp32k": 1e-5}
+VORT = dict(velocity=0.025, mu_plus=100.0, mu_minus=100.0, xi_a=0.01, xi_r=0.005, kappa=20.0)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    spc2.load()
+    with spc2.Sim(spc2.params(n=16)) as s:  # fails loudly without a GPU
+        s.init_random_state()
+
+
+def periodic_diff(a, b, L):
+    d = a - b
+    d[0::3] -= np.round(d[0::3] / L) * L
+    d[1::3] -= np.round(d[1::3] / L) * L
+    return d
+
+
+def assert_rhs(gpu, ref, v0, phi_scale, tol, what):
+    dv = np.abs(gpu - ref)
+    assert max(dv[0::3].max(), dv[1::3].max()) <= tol * v0, f"{what}: velocity"
+    assert dv[2::3].max() <= tol * phi_scale, f"{what}: phase derivative {dv[2::3].max():.3e}"
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
+@pytest.mark.parametrize("kc", [0.5, 1.0])  # stage reach below / above skin / 2 (list from the stage)
+def test_verlet_chimera_rhs_k(gpu, prec, kc):
+    n, rho = 40000, 0.1
+    L = math.sqrt(n / 4000.0)
+    st = O.make_state(17, 1, n, L, rho)
+    kp = O.uniform(170, 3 * n, -1.0, 1.0)
+    kn = O.uniform(171, 3 * n, 5.0, 6.0)
+    dt = 0.01  # reach |kc dt k| ~ 0.007 (kc = 0.5) or 0.014 (kc = 1) against skin / 2 = 0.01
+    ref, rc = O.rhs_k(O.chimera(n, L, rho, 0.3, 1.2, 1.4, 0.05, threads=8), st, kp, kc, dt, kn)
+    p = spc2.params("chimera", prec, "verlet", n=n, L=L, rho=rho, v0=0.3, sigma=1.2, alpha=1.4, D=0.05)
+    with spc2.Sim(p) as s:
+        s.set_state(st)
+        s.set_counting(True)
+        out = s.eval_rhs(kp, kc, dt, kn)
+        cnt = s.counts()
+        builds, entries = s.verlet_stats()
+    assert_rhs(out, ref, 0.3, 1.2, TOL[prec], f"chimera verlet/{prec}")
+    assert np.array_equal(cnt[1:], rc[1:]), (cnt, rc)
+    assert builds == (2 if kc == 1.0 else 1) and entries == cnt[0]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
+def test_verlet_vortex_rhs_both_forms(gpu, prec):
+    n, rho, D = 40000, 0.05, 0.02
+    L = math.sqrt(n / 4000.0)
+    st = O.make_state(23, 1, n, L, rho)
+    sysp = O.vortex(n, L, rho=rho, D=D, threads=8, **VORT)
+    kp = O.uniform(230, 3 * n, -1.0, 1.0)
+    kn = O.uniform(231, 3 * n, 5.0, 6.0)
+    refk, rck = O.rhs_k(sysp, st, kp, 0.5, 0.01, kn)
+    rd, ra, rci = O.rhs_ito(sysp, st)
+    p = spc2.params("vortex", prec, "verlet", n=n, L=L, rho=rho, D=D, **VORT)
+    with spc2.Sim(p) as s:
+        s.set_state(st)
+        s.set_counting(True)
+        outk = s.eval_rhs(kp, 0.5, 0.01, kn)
+        ck = s.counts()
+        s.set_counting(True)
+        gd, ga = s.eval_rhs_ito()
+        ci = s.counts()
+    tol = TOL[prec]
+    assert_rhs(outk, refk, 0.025, 120.0, tol, f"vortex verlet k/{prec}")
+    assert_rhs(gd, rd, 0.025, 120.0, tol, f"vortex verlet ito/{prec}")
+    for c in (0, 1):
+        scale = np.abs(ra[c::2]).max()
+        assert np.abs(ga[c::2] - ra[c::2]).max() <= tol * scale, f"vortex verlet ito/{prec} additional[{c}]"
+    assert np.array_equal(ck[1:], rck[1:]) and np.array_equal(ci[1:], rci[1:])
+
+
+def run(params, st, method, dt, steps, omega=None, chunks=(1,)):
+    with spc2.Sim(params) as s:
+        s.set_state(st)
+        if omega is not None:
+            s.set_omega(omega)
+        s.set_counting(True)
+        k = 0
+        for c in chunks:
+            s.step(method, dt, k, c)
+            k += c
+        s.step(method, dt, k, steps - k)
+        out = s.get_state(), s.counts()
+        stats = s.verlet_stats() if params.neighbour == spc2.NEIGH_VERLET else None
+    return out[0], out[1], stats
+
+
+@pytest.mark.parametrize("method", ["se", "rk4", "srk"])
+def test_verlet_trajectory_matches_cells_across_rebuilds(gpu, method):
+    """Fast particles (v0 dt = skin / 8) force a list rebuild every few steps;
+    the run must see exactly the cell path's neighbour sets at every RHS."""
+    n, rho, dt, steps = 30000, 0.1, 0.01, 40
+    L = math.sqrt(n / 4000.0)
+    v0 = 0.2 * rho / 8 / dt
+    st = O.make_state(8, 0, n, L, rho)
+    om = O.uniform(80, n, -0.5, 0.5)
+    kw = dict(n=n, L=L, rho=rho, v0=v0, sigma=1.0, alpha=1.45, D=0.02, seed=11)
+    a, ca, _ = run(spc2.params("chimera", "fp64", "cells", **kw), st, method, dt, steps, om, chunks=(3, 10))
+    b, cb, (builds, _) = run(spc2.params("chimera", "fp64", "verlet", **kw), st, method, dt, steps, om,
+                             chunks=(3, 10))
+    assert np.array_equal(ca[1:], cb[1:]), (ca, cb)
+    assert builds >= steps // 6
+    assert np.abs(periodic_diff(a, b, L)).max() < 1e-9
+
+
+def test_verlet_vortex_rk4_fp32_matches_fp64_cells(gpu):
+    """The bench workload at a reduced N: vortex RK4 with Verlet lists, FP64
+    against FP64 linked cells (the same neighbour sets at every RHS), and
+    FP32-compensated against FP64 (phases and order parameter after 50 steps;
+    the two trajectories drift apart by rounding, so their neighbour sets may
+    differ at the cutoffs)."""
+    n, rho, dt, steps = 65536, 0.05, 0.01, 50
+    L = math.sqrt(n / 4000.0)
+    st, _ = O.init_state(42, n, L, L, 0.0)
+    kw = dict(n=n, L=L, rho=rho, D=0.0, **VORT)
+    a, ca, _ = run(spc2.params("vortex", "fp64", "cells", **kw), st, "rk4", dt, steps)
+    v, cv, _ = run(spc2.params("vortex", "fp64", "verlet", **kw), st, "rk4", dt, steps)
+    assert np.array_equal(ca[1:], cv[1:]), (ca, cv)
+    assert np.abs(periodic_diff(a, v, L)).max() < 1e-9
+    b, cb, (builds, entries) = run(spc2.params("vortex", "fp32k", "verlet", **kw), st, "rk4", dt, steps)
+    c, cc, _ = run(spc2.params("vortex", "fp32k", "cells", **kw), st, "rk4", dt, steps)
+    assert builds >= 2 and entries > 0
+    Ra = np.abs(np.exp(1j * a[2::3]).mean())
+    rms = {}
+    for name, g in (("verlet", b), ("cells", c)):
+        dphi = np.abs(a[2::3] - g[2::3])
+        rms[name] = np.sqrt((dphi ** 2).mean())
+        dR = abs(np.abs(np.exp(1j * g[2::3]).mean()) - Ra)
+        print(f"fp32k {name} vs fp64 cells after {steps} steps: max |dphi| {dphi.max():.3e}, "
+              f"rms {rms[name]:.3e}, |dR| {dR:.3e}")
+        assert dR < 1e-4
+    # the list changes the FP32 summation order only: the same error level as the cells
+    assert rms["verlet"] < 1e-3 and rms["verlet"] < 3 * rms["cells"] + 1e-6
+
+
+def test_verlet_rebuild_schedule(gpu):
+    """RK4 rebuilds exactly when the next step could carry a particle beyond
+    skin / 2 of its build position: after m steps with (m + 1) v0 dt > skin / 2."""
+    n, rho, dt = 20000, 0.1, 0.01
+    L = math.sqrt(n / 4000.0)
+    skin = 0.2 * rho
+    v0 = skin / 2 / dt / 4.5  # 4 steps fit, the 5th would not
+    p = spc2.params("chimera", "fp64", "verlet", n=n, L=L, rho=rho, v0=v0, sigma=1.0, alpha=1.45, D=0.0)
+    st = O.make_state(2, 0, n, L, rho)
+    with spc2.Sim(p) as s:
+        s.set_state(st)  # build 1
+        s.step("rk4", dt, 0, 4)
+        assert s.verlet_stats()[0] == 1  # step m (0-based) fits while (m + 1) v0 dt <= skin / 2
+        s.step("rk4", dt, 4, 1)
+        assert s.verlet_stats()[0] == 2  # 5 v0 dt > skin / 2: rebuilt before step 4
+        s.step("rk4", dt, 5, 7)
+        assert s.verlet_stats()[0] == 3  # and again before step 8
+
+
+def test_verlet_rejects_too_few_cells(gpu):
+    p = spc2.params("chimera", "fp64", "verlet", n=100, L=1.0, rho=0.3)
+    with pytest.raises(spc2.Spc2Error, match="Verlet"):
+        spc2.Sim(p)
