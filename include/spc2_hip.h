@@ -185,6 +185,63 @@ spc2_status spc2_verlet_stats(spc2_sim_t h, int64_t *builds, int64_t *entries);
 
 spc2_status spc2_synchronize(spc2_sim_t h);
 
+/* ---- Batched replicas ---------------------------------------------------
+ * Many independent systems of uneven N stepped together, one launch per phase
+ * per step for the whole batch.  Replaces the reference's parameter-scan mode,
+ * one parameter set per MPI rank (SimulationEngine.cpp:82-88 under
+ * MPI_PARAMETER_SCAN, main.cpp:15-18).  Each replica keeps its own parameters,
+ * seed and step counter; its trajectory is bit-identical to a single handle
+ * created with the same parameters: noise is keyed by the replica's seed, its
+ * own step and the particle id within it.  All replicas share one precision and
+ * use the reference's neighbour dispatch (linked cells iff rho <= L/4).
+ * Method: Stochastic Euler (StochasticEulerStepperPtr.hpp:32-76). */
+typedef struct spc2_batch *spc2_batch_t;
+
+/* Device bytes one replica takes in a batch (admission control). */
+uint64_t spc2_batch_footprint(const spc2_params *p);
+
+/* A batch with the Philox initial state of every replica (natural frequencies
+ * zero; see spc2_batch_init_random_state). */
+spc2_status spc2_batch_create(const spc2_params *replicas, int32_t count, spc2_batch_t *out);
+
+/* A batch whose replica r continues replica prev_index[r] of `prev` (state,
+ * natural frequencies and step counter copied on the device) or, where
+ * prev_index[r] < 0, starts from its Philox initial state with natural
+ * frequencies N(0, omega_std).  Used to re-pack a running batch. */
+spc2_status spc2_batch_create_from(const spc2_params *replicas, int32_t count, spc2_batch_t prev,
+                                   const int32_t *prev_index, double omega_std, spc2_batch_t *out);
+spc2_status spc2_batch_destroy(spc2_batch_t b);
+spc2_status spc2_batch_init_random_state(spc2_batch_t b, double omega_std);
+spc2_status spc2_batch_set_state(spc2_batch_t b, int32_t replica, const double *state);
+spc2_status spc2_batch_get_state(spc2_batch_t b, int32_t replica, double *state);
+spc2_status spc2_batch_set_omega(spc2_batch_t b, int32_t replica, const double *omega);
+
+/* Advances every replica by min(nsteps, limit[r]) steps (limit == NULL: nsteps)
+ * with time step dt; replica r's step k uses noise of its own step
+ * steps_done[r] + k. */
+spc2_status spc2_batch_step(spc2_batch_t b, int32_t method, double dt, int64_t nsteps, const int64_t *limit);
+spc2_status spc2_batch_steps_done(spc2_batch_t b, int64_t *steps);
+spc2_status spc2_batch_order_parameters(spc2_batch_t b, double *R, double *psi);
+
+/* Float frames [x, y, phi] by particle id of every replica, concatenated in
+ * replica order (replica r starts at 3 * first_particle[r]). */
+spc2_status spc2_batch_snapshot_async(spc2_batch_t b, float *dst);
+spc2_status spc2_batch_snapshot_wait(spc2_batch_t b);
+spc2_status spc2_batch_layout(spc2_batch_t b, int32_t *first_particle, int32_t *particles);
+
+/* Row-segment tile size of each linked replica's pair kernel, in particles
+ * per workgroup segment (the autotuner's knob). */
+spc2_status spc2_batch_set_tile_targets(spc2_batch_t b, const double *particles_per_segment);
+spc2_status spc2_batch_get_tile_targets(spc2_batch_t b, double *particles_per_segment);
+
+/* Picks the tile target of each bucket of replicas with the same
+ * floor(log2 N) by timing trial_steps steps of a copy of the batch for each
+ * candidate (coordinate descent over the buckets) and applies the best to b;
+ * b's replicas do not move.  *ms_per_step (nullable) gets the tuned step time.
+ * No counterpart in the reference (its parameter scan runs one CPU rank per
+ * replica). */
+spc2_status spc2_batch_autotune(spc2_batch_t b, double dt, int32_t trial_steps, double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,24 @@ def load():
         "spc2_get_profile": (st, [h, dp, C.POINTER(C.c_int64)]),
         "spc2_synchronize": (st, [h]),
         "spc2_verlet_stats": (st, [h, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "spc2_batch_footprint": (C.c_uint64, [C.POINTER(Params)]),
+        "spc2_batch_create": (st, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
+        "spc2_batch_create_from": (st, [C.POINTER(Params), C.c_int32, h, C.POINTER(C.c_int32), C.c_double,
+                                        C.POINTER(C.c_void_p)]),
+        "spc2_batch_destroy": (st, [h]),
+        "spc2_batch_init_random_state": (st, [h, C.c_double]),
+        "spc2_batch_set_state": (st, [h, C.c_int32, dp]),
+        "spc2_batch_get_state": (st, [h, C.c_int32, dp]),
+        "spc2_batch_set_omega": (st, [h, C.c_int32, dp]),
+        "spc2_batch_step": (st, [h, C.c_int32, C.c_double, C.c_int64, C.POINTER(C.c_int64)]),
+        "spc2_batch_steps_done": (st, [h, C.POINTER(C.c_int64)]),
+        "spc2_batch_order_parameters": (st, [h, dp, dp]),
+        "spc2_batch_snapshot_async": (st, [h, C.c_void_p]),
+        "spc2_batch_snapshot_wait": (st, [h]),
+        "spc2_batch_layout": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "spc2_batch_set_tile_targets": (st, [h, dp]),
+        "spc2_batch_get_tile_targets": (st, [h, dp]),
+        "spc2_batch_autotune": (st, [h, C.c_double, C.c_int32, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -253,6 +271,120 @@ class Sim:
         b, e = C.c_int64(), C.c_int64()
         _check(load().spc2_verlet_stats(self.h, C.byref(b), C.byref(e)))
         return b.value, e.value
+
+
+class Batch:
+    """Replicas stepped together (an spc2_batch_t handle)."""
+
+    def __init__(self, replicas, prev=None, prev_index=None, omega_std=0.0):
+        self.params = list(replicas)
+        self.count = len(self.params)
+        arr = (Params * self.count)(*self.params)
+        h = C.c_void_p()
+        if prev is None:
+            _check(load().spc2_batch_create(arr, self.count, C.byref(h)))
+        else:
+            idx = np.ascontiguousarray(prev_index, np.int32)
+            assert idx.size == self.count
+            _check(load().spc2_batch_create_from(arr, self.count, prev.h, idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                 omega_std, C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def footprint(p):
+        return int(load().spc2_batch_footprint(C.byref(p)))
+
+    def close(self):
+        if self.h:
+            load().spc2_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_random_state(self, omega_std=0.0):
+        _check(load().spc2_batch_init_random_state(self.h, omega_std))
+
+    def set_state(self, r, state):
+        st = np.ascontiguousarray(state, np.float64)
+        assert st.size == 3 * self.params[r].n
+        _check(load().spc2_batch_set_state(self.h, r, _dp(st)))
+
+    def get_state(self, r):
+        st = np.empty(3 * self.params[r].n, np.float64)
+        _check(load().spc2_batch_get_state(self.h, r, _dp(st)))
+        return st
+
+    def set_omega(self, r, omega):
+        om = np.ascontiguousarray(omega, np.float64)
+        _check(load().spc2_batch_set_omega(self.h, r, _dp(om)))
+
+    def step(self, dt, nsteps, limit=None, method="se"):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        lim = None
+        if limit is not None:
+            lim = np.ascontiguousarray(limit, np.int64)
+            assert lim.size == self.count
+        _check(load().spc2_batch_step(self.h, m, dt, nsteps,
+                                      lim.ctypes.data_as(C.POINTER(C.c_int64)) if lim is not None else None))
+
+    def steps_done(self):
+        out = np.zeros(self.count, np.int64)
+        _check(load().spc2_batch_steps_done(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def order_parameters(self):
+        R = np.empty(self.count)
+        psi = np.empty(self.count)
+        _check(load().spc2_batch_order_parameters(self.h, _dp(R), _dp(psi)))
+        return R, psi
+
+    def layout(self):
+        first = np.empty(self.count, np.int32)
+        n = np.empty(self.count, np.int32)
+        _check(load().spc2_batch_layout(self.h, _ip(first), _ip(n)))
+        return first, n
+
+    def snapshot(self, buf=None):
+        """Float frames of every replica (list of arrays, AoS [x, y, phi] by id)."""
+        first, n = self.layout()
+        total = int(n.sum())
+        own = buf is None
+        if own:
+            buf = PinnedFrame(3 * total)
+        try:
+            _check(load().spc2_batch_snapshot_async(self.h, buf.ptr))
+            _check(load().spc2_batch_snapshot_wait(self.h))
+            a = buf.array().copy()
+        finally:
+            if own:
+                buf.free()
+        return [a[3 * f:3 * (f + k)] for f, k in zip(first, n)]
+
+    def set_tile_targets(self, targets):
+        t = np.ascontiguousarray(targets, np.float64)
+        assert t.size == self.count
+        _check(load().spc2_batch_set_tile_targets(self.h, _dp(t)))
+
+    def autotune(self, dt, trial_steps=4):
+        """Tunes the tile target per N bucket; returns the tuned ms per step."""
+        ms = C.c_double()
+        _check(load().spc2_batch_autotune(self.h, dt, trial_steps, C.byref(ms)))
+        return ms.value
+
+    def tile_targets(self):
+        t = np.empty(self.count)
+        _check(load().spc2_batch_get_tile_targets(self.h, _dp(t)))
+        return t
 
 
 class PinnedFrame:

@@ -71,19 +71,37 @@ void free_all(Sim &s) {
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
 
-void fill_kparams(Sim &s) {
-  const spc2_params &P = s.P;
-  KParams &k = s.kp;
-  k.n = s.n;
-  k.nx = s.nx;
-  k.ny = s.ny;
-  k.linked = s.linked;
+}  // namespace
+
+namespace spc2 {
+
+spc2_status set_error(spc2_status st, const std::string &msg) { return fail(st, msg); }
+
+int precision_code(int32_t p) { return p == SPC2_FP64 ? P64 : (p == SPC2_FP32_KAHAN ? P32K : P64F); }
+
+spc2_status cell_grid(const spc2_params &p, bool &linked, int &nx, int &ny) {
+  linked = p.rho <= 0.25 * p.Lx;  // ChimeraSystem.cpp:90, VortexArraysSystemPtr.cpp:113
+  nx = ny = 1;
+  if (!linked) return SPC2_OK;
+  double fx = p.Lx / p.rho, fy = p.Ly / p.rho;
+  if (fx > 2.0e9 || fy > 2.0e9 || fx * fy > 1.0e9) return fail(SPC2_ERR_INVALID, "too many cells for the linked-cell path");
+  nx = (int)(p.Lx / p.rho);  // num_subcells_, ChimeraSystem.cpp:32-33
+  ny = (int)(p.Ly / p.rho);
+  if (nx < 3 || ny < 3) return fail(SPC2_ERR_INVALID, "linked cells need at least 3 cells per side (rho <= L/3)");
+  return SPC2_OK;
+}
+
+void fill_kparams_from(KParams &k, const spc2_params &P, int n, int nx, int ny, bool linked) {
+  k.n = n;
+  k.nx = nx;
+  k.ny = ny;
+  k.linked = linked;
   k.Lx = P.Lx;
   k.Ly = P.Ly;
   k.rLx = 1.0 / P.Lx;
   k.rLy = 1.0 / P.Ly;
-  k.hx = s.nx > 0 ? P.Lx / s.nx : P.Lx;
-  k.hy = s.ny > 0 ? P.Ly / s.ny : P.Ly;
+  k.hx = nx > 0 ? P.Lx / nx : P.Lx;
+  k.hy = ny > 0 ? P.Ly / ny : P.Ly;
   k.hxf = (float)k.hx;
   k.hyf = (float)k.hy;
   k.rho = P.rho;
@@ -119,6 +137,12 @@ void fill_kparams(Sim &s) {
   const double close = 1e-2 * std::min(k.hx, k.hy);
   k.close2f = (float)(close * close);
 }
+
+}  // namespace spc2
+
+namespace {
+
+void fill_kparams(Sim &s) { fill_kparams_from(s.kp, s.P, s.n, s.nx, s.ny, s.linked); }
 
 spc2_status alloc_all(Sim &s) {
   const size_t n = (size_t)s.n, nc = (size_t)std::max(s.nc, 1);

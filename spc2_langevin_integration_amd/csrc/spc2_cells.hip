@@ -21,33 +21,6 @@ constexpr int kScanTile = kBlock * kScanItems;  // counts per workgroup
 
 inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
-// Histogram increment with one atomic per distinct cell per wavefront: lanes
-// holding the same cell take consecutive arrival ranks from a single
-// atomicAdd issued by the lowest such lane.  Every lane of the wavefront takes
-// part (valid == false for lanes past the end).
-__device__ __forceinline__ int wave_aggregated_rank(int *count, int c, bool valid) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  bool done = !valid;
-  int rank = 0;
-  while (true) {
-    unsigned long long pending = __ballot(!done);
-    if (pending == 0ull) break;
-    const int leader = __builtin_ctzll(pending);
-    const int lc = __shfl(c, leader, 64);
-    const bool mine = !done && c == lc;
-    const unsigned long long same = __ballot(mine);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&count[lc], __popcll(same));
-    base = __shfl(base, leader, 64);
-    if (mine) {
-      rank = base + __popcll(same & below);
-      done = true;
-    }
-  }
-  return rank;
-}
-
 // K1: stage s + (k_coef * dt) * k (ChimeraSystem.cpp:274-276, evaluated as
 // (k_coef * dt) * k_prev like the reference); on the linked-cell path the
 // position is wrapped if outside the box and binned (ChimeraSystem.cpp:291-309).
@@ -216,19 +189,24 @@ void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *
 int scan_blocks(int nc) { return (nc + kScanTile - 1) / kScanTile; }
 
 // Exclusive scan of count[0..m) into out[0..m], out[m] = total; clears count.
-void launch_scan_array(Sim &s, int *count, int m, int *out) {
+// bsum holds scan_blocks(m) + 1 ints.
+void launch_scan_raw(hipStream_t st, int *bsum, int *count, int m, int *out) {
   const int nb = scan_blocks(m);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s.stream, count, m, s.bsum);
-  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kBlock), 0, s.stream, s.bsum, nb, out, m);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s.stream, count, m, s.bsum, out);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, st, count, m, bsum);
+  hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kBlock), 0, st, bsum, nb, out, m);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, st, count, m, bsum, out);
 }
+
+void launch_scan_array(Sim &s, int *count, int m, int *out) { launch_scan_raw(s.stream, s.bsum, count, m, out); }
 
 void launch_scan(Sim &s) { launch_scan_array(s, s.count, s.nc, s.start); }
 
-void launch_place(Sim &s) {
-  hipLaunchKernelGGL(k_place, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.cell_of, s.local, s.start,
-                     s.hid, s.tmp, s.tmpid);
+void launch_place_raw(hipStream_t st, int n, const int *cell_of, const int *local, const int *start, const int *hid,
+                      int *tmp, int *tmpid) {
+  hipLaunchKernelGGL(k_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, cell_of, local, start, hid, tmp, tmpid);
 }
+
+void launch_place(Sim &s) { launch_place_raw(s.stream, s.n, s.cell_of, s.local, s.start, s.hid, s.tmp, s.tmpid); }
 
 void launch_rank_tiles(Sim &s) {
   dim3 g(blocks_for(s.n)), b(kBlock);

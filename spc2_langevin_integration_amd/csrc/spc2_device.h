@@ -199,4 +199,31 @@ __device__ __forceinline__ void uniform_pair(uint64_t seed, uint64_t step, uint3
   u1 = (double)(b >> 11) * 0x1.0p-53;
 }
 
+// Histogram increment with one atomic per distinct cell per wavefront: lanes
+// holding the same cell take consecutive arrival ranks from a single
+// atomicAdd issued by the lowest such lane.  Every lane of the wavefront takes
+// part (valid == false for lanes past the end).
+__device__ __forceinline__ int wave_aggregated_rank(int *count, int c, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  bool done = !valid;
+  int rank = 0;
+  while (true) {
+    unsigned long long pending = __ballot(!done);
+    if (pending == 0ull) break;
+    const int leader = __builtin_ctzll(pending);
+    const int lc = __shfl(c, leader, 64);
+    const bool mine = !done && c == lc;
+    const unsigned long long same = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[lc], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (mine) {
+      rank = base + __popcll(same & below);
+      done = true;
+    }
+  }
+  return rank;
+}
+
 }  // namespace spc2

@@ -158,14 +158,27 @@ struct Sim {
 };
 
 // ---- launchers -----------------------------------------------------------------
+// spc2_api.hip
+spc2_status set_error(spc2_status st, const std::string &msg);
+int precision_code(int32_t p);
+// linked cells or all pairs, and the cell grid, as the reference decides them
+// (ChimeraSystem.cpp:32-33, :90; VortexArraysSystemPtr.cpp:35-36, :113)
+spc2_status cell_grid(const spc2_params &p, bool &linked, int &nx, int &ny);
+void fill_kparams_from(KParams &k, const spc2_params &P, int n, int nx, int ny, bool linked);
 // spc2_cells.hip
 void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
 int scan_blocks(int nc);
 void launch_scan(Sim &s);
 void launch_scan_array(Sim &s, int *count, int m, int *out);
+void launch_scan_raw(hipStream_t st, int *bsum, int *count, int m, int *out);
+void launch_place_raw(hipStream_t st, int n, const int *cell_of, const int *local, const int *start, const int *hid,
+                      int *tmp, int *tmpid);
 void launch_place(Sim &s);
 void launch_rank_tiles(Sim &s);
 // spc2_pairs.hip
+constexpr double kSegTarget = 218.0;  // default particles per row segment
+int lds_elem_bytes(int prec);
+void configure_segments_kp(KParams &k, int prec, double target);
 void configure_segments(Sim &s);
 void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
                   double *const k2[3], double *const k3[3]);

@@ -1000,5 +1000,103 @@ void launch_pair(Sim &s, void (*kern)(KArgs...), dim3 grid, dim3 blk, size_t sh,
   }
 }
 
+// All pairs (rho > L/4): thread per particle, j = 0..N-1 ascending, stage
+// positions not wrapped (ChimeraSystem.cpp:131-167, :184-254); arrays by
+// particle id.
+template <int PREC, int SYS>
+__device__ __forceinline__ void allpairs_pass1(const KParams &P, int i, const double *gx, const double *gy,
+                                               const double *gp, Pass1Out &o) {
+  const double xi = gx[i], yi = gy[i], pi = gp[i];
+  if (SYS == SPC2_CHIMERA) {
+    if (PREC == P64) {
+      double A = 0.0;
+      int n = 0;
+      for (int j = 0; j < P.n; ++j) {
+        if (j == i) continue;
+        double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+        double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+        double r2 = dx * dx + dy * dy;
+        if (r2 <= P.rho2) {
+          A += sin(gp[j] - pi - P.alpha);
+          ++n;
+        }
+      }
+      o.n = n;
+      o.written = n != 0;
+      o.aphi = n != 0 ? P.sigma * A / (double)n : 0.0;
+    } else {
+      // factored form; the FP32 variant accumulates in Kahan FP32 over FP64 geometry
+      double S = 0.0, C = 0.0;
+      Kahan Sf, Cf;
+      Sf.init();
+      Cf.init();
+      int n = 0;
+      for (int j = 0; j < P.n; ++j) {
+        if (j == i) continue;
+        double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+        double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+        double r2 = dx * dx + dy * dy;
+        if (r2 <= P.rho2) {
+          double sj, cj;
+          sincos(gp[j], &sj, &cj);
+          if (PREC == P32K) {
+            Sf.add((float)sj);
+            Cf.add((float)cj);
+          } else {
+            S += sj;
+            C += cj;
+          }
+          ++n;
+        }
+      }
+      if (PREC == P32K) {
+        S = Sf.value();
+        C = Cf.value();
+      }
+      double sa, ca;
+      sincos(pi + P.alpha, &sa, &ca);
+      o.n = n;
+      o.written = n != 0;
+      o.aphi = n != 0 ? P.sigma * (ca * S - sa * C) / (double)n : 0.0;
+    }
+    o.n2 = 0;
+    o.vortex_add = 0;
+  } else {
+    double A = 0.0, B = 0.0, Cc = 0.0;
+    int n0 = 0, n1 = 0, n2 = 0;
+    for (int j = 0; j < P.n; ++j) {
+      if (j == i) continue;
+      double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+      double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+      double dr = sqrt(dx * dx + dy * dy);
+      if (dr <= P.xi_a) {
+        A += vortex_w(P, dr) * sin(gp[j] - pi);
+        ++n0;
+      } else if (dr <= P.rho) {
+        B += vortex_w(P, dr) * sin(gp[j] - pi);
+        ++n1;
+      }
+      if (dr <= P.xi_r) {
+        Cc += -P.kappa * sin(atan2(dy, dx) - pi);
+        ++n2;
+      }
+    }
+    o.n = n0 + n1;
+    o.n2 = n2;
+    if (n0 + n1 > 0) {
+      o.aphi = (A + B) / ((double)n0 + (double)n1);
+      if (n2 > 0) o.aphi += Cc / (double)n2;
+      o.written = 1;
+      o.vortex_add = 0;
+    } else {
+      o.aphi = 0.0;
+      o.written = 0;
+      o.vortex_add = n2 > 0;
+      o.coh = n2 > 0 ? Cc / (double)n2 : 0.0;
+    }
+  }
+  o.phase = pi;
+}
+
 }  // namespace
 }  // namespace spc2

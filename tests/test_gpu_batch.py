@@ -1,0 +1,114 @@
+"""Batched replicas (spc2_batch_*) against single handles.
+
+A replica's arithmetic, summation order and noise keys do not depend on the
+batch it runs in, so every replica must match a single handle created with the
+same parameters bit for bit: after plain steps, with per-replica step limits,
+and after the batch is re-packed with other replicas (continuous batching)."""
+import math
+
+import numpy as np
+import pytest
+
+import spc2_langevin_integration_amd as spc2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    spc2.load()
+    with spc2.Sim(spc2.params(n=16)) as s:  # fails loudly without a GPU
+        s.init_random_state()
+
+
+def replica(system, n, prec, seed, sigma=1.0, alpha=1.45, D=0.01):
+    L = math.sqrt(n / 4000.0)
+    if system == "chimera":
+        return spc2.params("chimera", prec, n=n, L=L, rho=0.1, v0=0.01, sigma=sigma, alpha=alpha, D=D, seed=seed)
+    return spc2.params("vortex", prec, n=n, L=L, rho=0.1, velocity=0.05, mu_plus=100.0, mu_minus=80.0, xi_a=0.04,
+                       xi_r=0.02, kappa=20.0, D=D, seed=seed)
+
+
+MIX = [("chimera", 500, 1, 0.2, 1.2, 0.001), ("chimera", 633, 2, 2.0, 1.57, 0.1), ("vortex", 620, 3, 1, 1, 0.02),
+       ("chimera", 1000, 4, 1.0, 1.45, 0.01), ("vortex", 5000, 5, 1, 1, 0.05), ("chimera", 7000, 6, 0.6, 1.3, 0.03),
+       ("chimera", 20000, 7, 1.4, 1.5, 0.002), ("vortex", 16384, 8, 1, 1, 0.0)]
+
+
+def single(p, steps, dt, omega_std):
+    with spc2.Sim(p) as s:
+        s.init_random_state(omega_std)
+        s.step("se", dt, 0, steps)
+        return s.get_state(), s.order_parameter()
+
+
+@pytest.mark.parametrize("prec", ["fp32k", "fp64"])
+def test_batch_matches_single_handles(gpu, prec):
+    reps = [replica(sy, n, prec, seed, sg, al, D) for sy, n, seed, sg, al, D in MIX]
+    dt, steps = 0.01, 25
+    with spc2.Batch(reps) as b:
+        b.init_random_state(0.1)
+        b.step(dt, 10)
+        b.step(dt, steps - 10)
+        R, _ = b.order_parameters()
+        assert (b.steps_done() == steps).all()
+        frames = b.snapshot()
+        for r, p in enumerate(reps):
+            got = b.get_state(r)
+            ref, (Rr, _) = single(p, steps, dt, 0.1)
+            assert np.array_equal(got, ref), f"replica {r} (N={p.n}) differs: {np.abs(got - ref).max():.3e}"
+            assert R[r] == pytest.approx(Rr, rel=1e-12, abs=1e-14)
+            assert np.array_equal(frames[r], got.astype(np.float32))
+
+
+def test_batch_limits_and_repacking(gpu):
+    """Per-replica step limits, then a re-pack that drops, reorders and adds
+    replicas; carried replicas keep state, frequencies and step counters."""
+    reps = [replica(sy, n, "fp32k", seed, sg, al, D) for sy, n, seed, sg, al, D in MIX]
+    dt = 0.01
+    limit = [12, 0, 5, 12, 7, 12, 3, 9]
+    with spc2.Batch(reps) as a:
+        a.init_random_state(0.2)
+        a.step(dt, 12, limit=limit)
+        assert list(a.steps_done()) == limit
+        for r in (0, 1, 4):
+            ref, _ = single(reps[r], limit[r], dt, 0.2)
+            assert np.array_equal(a.get_state(r), ref)
+        new = [replica("chimera", 3000, "fp32k", 99, 1.1, 1.4, 0.02), replica("vortex", 550, "fp32k", 98)]
+        keep = [6, 0, 4, 7]
+        reps2 = [reps[k] for k in keep[:2]] + [new[0]] + [reps[k] for k in keep[2:]] + [new[1]]
+        prev_index = [6, 0, -1, 4, 7, -1]
+        with spc2.Batch(reps2, prev=a, prev_index=prev_index, omega_std=0.2) as b:
+            a.close()
+            b.step(dt, 6)
+            done = b.steps_done()
+            for r, (p, q) in enumerate(zip(reps2, prev_index)):
+                total = (limit[q] if q >= 0 else 0) + 6
+                assert done[r] == total
+                ref, _ = single(p, total, dt, 0.2)
+                assert np.array_equal(b.get_state(r), ref), f"replica {r} (from {q})"
+
+
+def test_batch_tile_targets_do_not_change_results(gpu):
+    reps = [replica(sy, n, "fp32k", seed, sg, al, D) for sy, n, seed, sg, al, D in MIX]
+    out = []
+    for target in (None, 96.0, 640.0):
+        with spc2.Batch(reps) as b:
+            if target:
+                b.set_tile_targets([target] * len(reps))
+            b.init_random_state(0.0)
+            b.step(0.01, 8)
+            out.append([b.get_state(r) for r in range(len(reps))])
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
+    for a, c in zip(out[0], out[2]):
+        assert np.array_equal(a, c)
+
+
+def test_batch_rejects_bad_input(gpu):
+    reps = [replica("chimera", 1000, "fp32k", 1), replica("chimera", 1000, "fp64", 2)]
+    with pytest.raises(spc2.Spc2Error, match="precision"):
+        spc2.Batch(reps)
+    with spc2.Batch(reps[:1]) as b:
+        with pytest.raises(spc2.Spc2Error, match="Stochastic Euler"):
+            b.step(0.01, 1, method="rk4")
+    assert spc2.Batch.footprint(reps[0]) > 1000 * 100
