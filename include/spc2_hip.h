@@ -185,6 +185,9 @@ spc2_status spc2_verlet_stats(spc2_sim_t h, int64_t *builds, int64_t *entries);
 
 spc2_status spc2_synchronize(spc2_sim_t h);
 
+/* Free and total bytes of the current device (admission budgets). */
+spc2_status spc2_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* ---- Batched replicas ---------------------------------------------------
  * Many independent systems of uneven N stepped together, one launch per phase
  * per step for the whole batch.  Replaces the reference's parameter-scan mode,

@@ -405,6 +405,107 @@ class PinnedFrame:
             self.ptr = None
 
 
+HOST_LIB_PATH = os.path.join(PKG_DIR, "libspc2_host.so")
+SERVER_HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "spc2_server.h")
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_uint64), ("max_particles", C.c_int64), ("max_replicas", C.c_int32),
+        ("super_steps", C.c_int32), ("snapshot_every", C.c_int32), ("keep_final", C.c_int32),
+        ("dt", C.c_double), ("omega_std", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64), ("queued", C.c_int64),
+        ("in_flight", C.c_int64), ("repacks", C.c_int64), ("frames", C.c_int64), ("frame_bytes", C.c_uint64),
+        ("particle_steps", C.c_double), ("wall_seconds", C.c_double), ("busy_seconds", C.c_double),
+        ("latency_p50", C.c_double), ("latency_p99", C.c_double), ("latency_max", C.c_double),
+        ("queue_wait_p50", C.c_double), ("queue_wait_p99", C.c_double),
+        ("max_in_flight", C.c_int64), ("max_particles_in_flight", C.c_int64),
+    ]
+
+
+_host = None
+
+
+def load_host():
+    """Loads libspc2_host.so (the C++ host layer with the serving loop)."""
+    global _host
+    if _host is not None:
+        return _host
+    load()
+    if not os.path.exists(HOST_LIB_PATH):
+        raise Spc2Error(f"{HOST_LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(HOST_LIB_PATH)
+    h = C.c_void_p
+    st = C.c_int
+    sig = {
+        "spc2_server_default_config": (None, [C.POINTER(ServerConfig)]),
+        "spc2_server_create": (st, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+        "spc2_server_destroy": (st, [h]),
+        "spc2_server_submit": (st, [h, C.POINTER(Params), C.c_int64, C.POINTER(C.c_int64)]),
+        "spc2_server_wait": (st, [h, C.c_int64, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "spc2_server_drain": (st, [h, C.c_double]),
+        "spc2_server_stats_get": (st, [h, C.POINTER(ServerStats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _host = L
+    return L
+
+
+class Server:
+    """The continuous-batching request loop (spc2_server_t)."""
+
+    def __init__(self, **cfg):
+        c = ServerConfig()
+        load_host().spc2_server_default_config(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.cfg = c
+        self.n = {}
+        h = C.c_void_p()
+        _check(load_host().spc2_server_create(C.byref(c), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load_host().spc2_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, p, steps):
+        rid = C.c_int64()
+        _check(load_host().spc2_server_submit(self.h, C.byref(p), steps, C.byref(rid)))
+        self.n[rid.value] = int(p.n)
+        return rid.value
+
+    def wait(self, rid, timeout=120.0):
+        st = np.empty(3 * self.n[rid], np.float64) if self.cfg.keep_final else None
+        lat = C.c_double()
+        _check(load_host().spc2_server_wait(self.h, rid, timeout, _dp(st) if st is not None else None,
+                                            C.byref(lat)))
+        return st, lat.value
+
+    def drain(self, timeout=-1.0):
+        _check(load_host().spc2_server_drain(self.h, timeout))
+
+    def stats(self):
+        s = ServerStats()
+        _check(load_host().spc2_server_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+
 def declared_symbols(header=HEADER_PATH):
     """Names of the functions include/spc2_hip.h declares."""
     import re

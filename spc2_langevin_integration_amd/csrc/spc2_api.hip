@@ -688,6 +688,15 @@ spc2_status spc2_verlet_stats(spc2_sim_t h, int64_t *builds, int64_t *entries) {
   return SPC2_OK;
 }
 
+spc2_status spc2_device_memory(uint64_t *free_bytes, uint64_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(SPC2_ERR_INVALID, "null argument");
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  *free_bytes = f;
+  *total_bytes = t;
+  return SPC2_OK;
+}
+
 spc2_status spc2_num_cells(spc2_sim_t h, int32_t *nx, int32_t *ny) {
   if (check_handle(h) || !nx || !ny) return fail(SPC2_ERR_INVALID, "null argument");
   *nx = h->s.nx;

@@ -58,3 +58,11 @@ def test_runner_help_and_config_errors(built, tmp_path):
     assert r.returncode == 1 and "method must be se, srk or rk4" in r.stderr
     r = subprocess.run([RUNNER, "loop=both"], capture_output=True, text=True)
     assert r.returncode == 1 and "loop must be device or host" in r.stderr
+
+
+def test_host_library_exports_the_server_interface(built):
+    import spc2_langevin_integration_amd as spc2
+    out = subprocess.run(["nm", "-D", "--defined-only", HOST_LIB], check=True, capture_output=True, text=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    declared = spc2.declared_symbols(spc2._abi.SERVER_HEADER_PATH)
+    assert declared and not [d for d in declared if d not in names], declared
