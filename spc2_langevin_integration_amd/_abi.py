@@ -421,7 +421,8 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64), ("queued", C.c_int64),
         ("in_flight", C.c_int64), ("repacks", C.c_int64), ("frames", C.c_int64), ("frame_bytes", C.c_uint64),
-        ("particle_steps", C.c_double), ("wall_seconds", C.c_double), ("busy_seconds", C.c_double),
+        ("particle_steps", C.c_double), ("particle_steps_executed", C.c_double), ("wall_seconds", C.c_double),
+        ("busy_seconds", C.c_double),
         ("latency_p50", C.c_double), ("latency_p99", C.c_double), ("latency_max", C.c_double),
         ("queue_wait_p50", C.c_double), ("queue_wait_p99", C.c_double),
         ("max_in_flight", C.c_int64), ("max_particles_in_flight", C.c_int64),

@@ -140,13 +140,15 @@ int main(int argc, char **argv) {
       "\"arrival_seconds\": %.6g, \"wall_seconds\": %.6g, \"submitted\": %lld, \"admitted\": %lld, "
       "\"completed\": %lld, \"queued_at_end\": %lld, \"in_flight_at_end\": %lld, \"max_in_flight\": %lld, "
       "\"max_particles_in_flight\": %lld, \"repacks\": %lld, \"frames\": %lld, \"frame_bytes\": %llu, "
-      "\"particle_steps_completed\": %.6g, \"particle_steps_per_s\": %.6g, \"busy_fraction\": %.4f, "
+      "\"particle_steps_completed\": %.6g, \"particle_steps_per_s\": %.6g, \"particle_steps_executed_per_s\": %.6g, "
+      "\"busy_fraction\": %.4f, "
       "\"latency_p50_s\": %.6g, \"latency_p99_s\": %.6g, \"latency_max_s\": %.6g, \"queue_wait_p50_s\": %.6g, "
       "\"queue_wait_p99_s\": %.6g, \"dtype\": \"fp32k\", \"method\": \"se\", \"drained\": %s}\n",
       a.rate, a.duration, s.wall_seconds, (long long)s.submitted, (long long)s.admitted, (long long)s.completed,
       (long long)s.queued, (long long)s.in_flight, (long long)s.max_in_flight, (long long)s.max_particles_in_flight,
       (long long)s.repacks, (long long)s.frames, (unsigned long long)s.frame_bytes, s.particle_steps,
       s.wall_seconds > 0 ? s.particle_steps / s.wall_seconds : 0.0,
+      s.wall_seconds > 0 ? s.particle_steps_executed / s.wall_seconds : 0.0,
       s.wall_seconds > 0 ? s.busy_seconds / s.wall_seconds : 0.0, s.latency_p50, s.latency_p99, s.latency_max,
       s.queue_wait_p50, s.queue_wait_p99, a.drain ? "true" : "false");
   std::fflush(stdout);
