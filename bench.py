@@ -119,6 +119,26 @@ def error_horizon(spc2, w, nb, horizon):
     return out
 
 
+COPY_BYTES = 1 << 29  # calibration copy: reads and writes this many bytes
+
+
+def pmc_run(spc2, w, nb):
+    """Workload steps and a calibration copy for the FETCH_SIZE / WRITE_SIZE
+    counters: the copy reads and writes COPY_BYTES, so the counters' ratio to
+    it converts their units to bytes."""
+    import torch
+    s = make_sim(spc2, w, "fp32k", nb)
+    s.step("rk4", w["dt"], 0, 3)
+    s.synchronize()
+    s.close()
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_run": True, "copy_bytes": COPY_BYTES, "n": w["n"], "neighbour": nb}), flush=True)
+
+
 def cpu_baseline(w, steps=2):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -150,6 +170,9 @@ def main():
     ap.add_argument("--precisions", default="fp32k,fp64,fp64f",
                     help="forms to time; the first is the headline value")
     ap.add_argument("--neighbour", default="verlet", choices=["verlet", "cells"])
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="for rocprofv3 --pmc: a few steps of the FP32 workload and a plain device copy of "
+                         "COPY_BYTES bytes (the counter calibration), no timing")
     ap.add_argument("--error-horizon", type=int, default=100,
                     help="steps after which FP32 and FP64 runs are compared before chaotic decorrelation")
     args = ap.parse_args()
@@ -160,6 +183,9 @@ def main():
 
     import spc2_langevin_integration_amd as spc2
     w = workload(args.n)
+    if args.pmc_run:
+        pmc_run(spc2, w, args.neighbour)
+        return
     res = {}
     precs = args.precisions.split(",")
     nb = args.neighbour
