@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32k": 157.3, "fp64": 78.6, "fp64f": 78.6}  # TFLOP/s, MI355X vector FP32 / FP64
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_pass1_verlet<VortAcc<P32K>> on this workload, from the
+# FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) counter passes calibrated against a
+# 512 MiB device copy; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
+TRAFFIC_VERLET_FP32K = 346.6e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -248,7 +252,8 @@ def main():
             "peak": HBM_PEAK_GBS if hbm else PEAK["fp32k"],
             "unit": "GB/s" if hbm else "TFLOP/s",
             "frac": prof["gbs"] / HBM_PEAK_GBS if hbm else prof["tflops"] / PEAK["fp32k"],
-            "traffic": None,
+            "traffic": TRAFFIC_VERLET_FP32K if hbm and args.n == 1048576 else None,
+            "traffic_GBs": (TRAFFIC_VERLET_FP32K / (prof["avg_ms"] / 1e3) / 1e9) if hbm and args.n == 1048576 else None,
             "avg_launch_ms": prof["avg_ms"], "bytes_per_launch": prof["bytes_per_launch"],
             "flops_per_launch": prof["flops_per_launch"], "valu_tflops": prof["tflops"],
             "valu_frac": prof["tflops"] / PEAK["fp32k"],

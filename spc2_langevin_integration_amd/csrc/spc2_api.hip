@@ -228,9 +228,15 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt) {
   HIP_TRY(hipMemcpyAsync(s.yb, s.t.y, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
   launch_verlet_count(s);
   launch_scan_array(s, s.wmax, s.nw, s.wrow);
+  if (spc2_status st = check_launch("verlet count")) return st;
   int rows = 0;
   HIP_TRY(hipMemcpyAsync(&rows, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
+  // rows = sum over wavefronts of their longest list; more than 4096 entries
+  // per particle means a skin or density the list is not meant for
+  if (rows < 0 || (long long)rows > 4096LL * s.nw)
+    return fail(SPC2_ERR_INVALID, "Verlet list of " + std::to_string(rows) + " rows for " + std::to_string(s.nw) +
+                                      " wavefronts: skin or density too large for a neighbour list");
   const size_t need = (size_t)rows * 64;
   if (need > s.vlist_cap || !s.vlist) {
     if (s.vlist) (void)hipFree(s.vlist);
