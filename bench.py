@@ -39,28 +39,37 @@ FLOP_COHESION = 12 + 2
 FLOP_EPILOGUE = 4 + 2
 
 
-def workload(n=1048576):
+def workload(n=1048576, name="vortex-rk4"):
     L = math.sqrt(n / 4000.0)
-    return dict(n=n, L=L, rho=0.05, velocity=0.025, mu_plus=100.0, mu_minus=100.0, xi_a=0.01, xi_r=0.005,
-                kappa=20.0, D=0.0, dt=0.01, seed=42)
+    if name == "chimera-srk":
+        # chimera, SRK, dt = 0.005, rho = 0.1, v0 = 0.01, sigma = 1, alpha = 1.45,
+        # D = 0.01, omega ~ N(0, 0.1) (N = 262,144: ~125 neighbours per particle)
+        return dict(system="chimera", method="srk", n=n, L=L, rho=0.1, v0=0.01, sigma=1.0, alpha=1.45, D=0.01,
+                    dt=0.005, omega_std=0.1, seed=42)
+    return dict(system="vortex", method="rk4", n=n, L=L, rho=0.05, velocity=0.025, mu_plus=100.0, mu_minus=100.0,
+                xi_a=0.01, xi_r=0.005, kappa=20.0, D=0.0, dt=0.01, omega_std=0.0, seed=42)
 
 
 def make_sim(spc2, w, precision, neighbour="verlet"):
-    p = spc2.params("vortex", precision, neighbour, n=w["n"], L=w["L"], rho=w["rho"], velocity=w["velocity"],
-                    mu_plus=w["mu_plus"], mu_minus=w["mu_minus"], xi_a=w["xi_a"], xi_r=w["xi_r"],
-                    kappa=w["kappa"], D=w["D"], seed=w["seed"])
+    if w["system"] == "chimera":
+        p = spc2.params("chimera", precision, neighbour, n=w["n"], L=w["L"], rho=w["rho"], v0=w["v0"],
+                        sigma=w["sigma"], alpha=w["alpha"], D=w["D"], seed=w["seed"])
+    else:
+        p = spc2.params("vortex", precision, neighbour, n=w["n"], L=w["L"], rho=w["rho"], velocity=w["velocity"],
+                        mu_plus=w["mu_plus"], mu_minus=w["mu_minus"], xi_a=w["xi_a"], xi_r=w["xi_r"],
+                        kappa=w["kappa"], D=w["D"], seed=w["seed"])
     s = spc2.Sim(p)
-    s.init_random_state(0.0)
+    s.init_random_state(w["omega_std"])
     return s
 
 
 def time_precision(spc2, w, precision, steps, warmup, neighbour):
     s = make_sim(spc2, w, precision, neighbour)
-    s.step("rk4", w["dt"], 0, warmup)
+    s.step(w["method"], w["dt"], 0, warmup)
     s.synchronize()
     b0 = s.verlet_stats()[0] if neighbour == "verlet" else 0
     t0 = time.perf_counter()
-    s.step("rk4", w["dt"], warmup, steps)
+    s.step(w["method"], w["dt"], warmup, steps)
     s.synchronize()
     t1 = time.perf_counter()
     builds = (s.verlet_stats()[0] - b0) if neighbour == "verlet" else None
@@ -75,17 +84,23 @@ def time_precision(spc2, w, precision, steps, warmup, neighbour):
 def profile_pair_kernel(spc2, w, precision, steps, neighbour):
     """HIP-event timing of the pair kernels on the handle's stream, with pair counts."""
     s = make_sim(spc2, w, precision, neighbour)
-    s.step("rk4", w["dt"], 0, 2)
+    s.step(w["method"], w["dt"], 0, 2)
     s.set_profiling(True)            # timing pass: events bound to the pair-kernel dispatches
-    s.step("rk4", w["dt"], 2, steps)
+    s.step(w["method"], w["dt"], 2, steps)
     ms, launches = s.profile()
     s.set_profiling(False)
     s.set_counting(True)             # counting pass over the same number of launches
-    s.step("rk4", w["dt"], 2 + steps, steps)
+    s.step(w["method"], w["dt"], 2 + steps, steps)
     cnt = s.counts()
     s.close()
     per = cnt / launches
-    flops = (FLOP_CANDIDATE * per[0] + FLOP_VORTEX_PAIR * per[1] + FLOP_COHESION * per[2] + FLOP_EPILOGUE * w["n"])
+    if w["system"] == "chimera":
+        # pass 1 and pass 2 alternate: candidates once per RHS, 7 + 1T per pass-1
+        # and 8 + 1T per pass-2 interaction, averaged over the two launches
+        flops = (FLOP_CANDIDATE * per[0] * 2 + (8 + 9) * per[1] * 2 + 2 * FLOP_EPILOGUE * w["n"]) / 2
+    else:
+        flops = (FLOP_CANDIDATE * per[0] + FLOP_VORTEX_PAIR * per[1] + FLOP_COHESION * per[2] +
+                 FLOP_EPILOGUE * w["n"])
     avg_s = ms / launches / 1e3
     # algorithmic HBM bytes per launch, per particle: own tile (FP32 16 B; FP64
     # 24 B) + FP64 stage phase 8 B + home->id and counts 8 B + k written 24 B;
@@ -110,7 +125,7 @@ def error_horizon(spc2, w, nb, horizon):
     ph = {}
     for prec, neigh in (("fp64", nb), ("fp32k", nb), ("fp64", "cells" if nb == "verlet" else "verlet")):
         s = make_sim(spc2, w, prec, neigh)
-        s.step("rk4", w["dt"], 0, horizon)
+        s.step(w["method"], w["dt"], 0, horizon)
         ph[(prec, neigh)] = s.get_state()[2::3]
         s.close()
     ref = ph[("fp64", nb)]
@@ -132,7 +147,7 @@ def pmc_run(spc2, w, nb):
     it converts their units to bytes."""
     import torch
     s = make_sim(spc2, w, "fp32k", nb)
-    s.step("rk4", w["dt"], 0, 3)
+    s.step(w["method"], w["dt"], 0, 3)
     s.synchronize()
     s.close()
     src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
@@ -143,22 +158,41 @@ def pmc_run(spc2, w, nb):
     print(json.dumps({"pmc_run": True, "copy_bytes": COPY_BYTES, "n": w["n"], "neighbour": nb}), flush=True)
 
 
-def cpu_baseline(w, steps=2):
+def cpu_baseline(w, target_s=12.0, max_steps=40):
+    """The oracle on the host cores: one step to time it, then as many more as
+    fit about target_s seconds (the rate is over all steps taken)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import numpy as np
     threads = max(1, min(16, os.cpu_count() or 1))
     n = w["n"]
-    st, _ = O.init_state(w["seed"], n, w["L"], w["L"], 0.0)
-    sysp = O.vortex(n, w["L"], rho=w["rho"], velocity=w["velocity"], mu_plus=w["mu_plus"], mu_minus=w["mu_minus"],
-                    xi_a=w["xi_a"], xi_r=w["xi_r"], kappa=w["kappa"], D=w["D"], threads=threads)
-    kbuf = np.zeros(12 * n)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        st = O.step_rk4(sysp, st, w["dt"], kbuf)
-    sec = time.perf_counter() - t0
+    st, om = O.init_state(w["seed"], n, w["L"], w["L"], w["omega_std"])
+    if w["system"] == "chimera":
+        sysp = O.chimera(n, w["L"], w["rho"], w["v0"], w["sigma"], w["alpha"], w["D"], threads=threads)
+
+        def one(k, st):
+            xi = O.philox_normals(w["seed"], k, 0, n, 2)
+            t = time.perf_counter()
+            st = O.step_srk(sysp, st, w["dt"], xi, omega=om)
+            return st, time.perf_counter() - t
+    else:
+        sysp = O.vortex(n, w["L"], rho=w["rho"], velocity=w["velocity"], mu_plus=w["mu_plus"],
+                        mu_minus=w["mu_minus"], xi_a=w["xi_a"], xi_r=w["xi_r"], kappa=w["kappa"], D=w["D"],
+                        threads=threads)
+        kbuf = np.zeros(12 * n)
+
+        def one(k, st):
+            t = time.perf_counter()
+            st = O.step_rk4(sysp, st, w["dt"], kbuf)
+            return st, time.perf_counter() - t
+    st, sec = one(0, st)
+    steps = max(1, min(max_steps, int(target_s / max(sec, 1e-9))))
+    for k in range(1, steps):
+        st, dt_s = one(k, st)
+        sec += dt_s
     return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} RK4 steps of the same N={n} vortex workload, FP64 reference method "
+            "sample": f"{steps} {w['method'].upper()} steps of the same N={n} {w['system']} workload, FP64 "
+                      f"reference method "
                       f"(oracle/spc2_oracle.c, -O2), {threads} threads, contiguous particle slices",
             "seconds": sec}
 
@@ -173,7 +207,10 @@ def main():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precisions", default="fp32k,fp64,fp64f",
                     help="forms to time; the first is the headline value")
-    ap.add_argument("--neighbour", default="verlet", choices=["verlet", "cells"])
+    ap.add_argument("--neighbour", default=None, choices=["verlet", "cells"],
+                    help="default: verlet for vortex-rk4, cells (the reference's) for chimera-srk")
+    ap.add_argument("--workload", default="vortex-rk4", choices=["vortex-rk4", "chimera-srk"],
+                    help="vortex-rk4: the headline (N=1M); chimera-srk: chimera SRK at the given --n")
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: a few steps of the FP32 workload and a plain device copy of "
                          "COPY_BYTES bytes (the counter calibration), no timing")
@@ -186,7 +223,9 @@ def main():
         sys.exit(2)
 
     import spc2_langevin_integration_amd as spc2
-    w = workload(args.n)
+    w = workload(args.n, args.workload)
+    if args.neighbour is None:
+        args.neighbour = "verlet" if w["system"] == "vortex" else "cells"
     if args.pmc_run:
         pmc_run(spc2, w, args.neighbour)
         return
@@ -210,6 +249,8 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(w)
 
     v = res["fp32k"]
+    hbm = nb == "verlet"
+    headline = hbm and args.n == 1048576 and w["system"] == "vortex"
     err = {}
     if "fp64" in res:
         import numpy as np
@@ -220,11 +261,12 @@ def main():
                 "order_param_abs_err": abs(v["R_final"] - res["fp64"]["R_final"])},
             **({f"after_{args.error_horizon}_steps": hz["fp32"],
                 f"fp64_reordered_after_{args.error_horizon}_steps": hz["fp64_reordered"]} if hz else {})}}
-    hbm = nb == "verlet"
-    kname = ("k_pass1_verlet<VortAcc<P32K>>" if hbm else "k_pass1_lds<VortAcc<P32K>>") + \
-        " (vortex pair sums, FP32 Kahan)"
+    acc = "VortAcc" if w["system"] == "vortex" else "ChimAcc, Chim2Acc"
+    kname = (f"k_pass1_verlet<{acc}<P32K>>" if hbm else f"k_pass1_lds<{acc}<P32K>>") + \
+        f" ({w['system']} pair sums, FP32 Kahan)"
     line = {
-        "metric": "particle-steps/s at N=1M (FP32-comp vs FP64, same GPU)",
+        "metric": ("particle-steps/s at N=1M (FP32-comp vs FP64, same GPU)" if headline else
+                   f"particle-steps/s, {args.workload} at N={w['n']} (FP32-comp vs FP64, same GPU)"),
         "value": v["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -236,11 +278,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 pair sums (Kahan) + f64 state",
         "data": "synthetic (Philox uniform positions and phases, seed 42)",
-        "config": {"workload": "vortex-arrays RK4, N=1048576, density 4000 (L=16.19086), rho=0.05, xi_a=0.01, "
-                               "xi_r=0.005, v=0.025, mu+-=100, kappa=20, D=0, dt=0.01, "
+        "config": {"workload": (f"vortex-arrays RK4, N={w['n']}, density 4000 (L={w['L']:.5f}), rho=0.05, "
+                                "xi_a=0.01, xi_r=0.005, v=0.025, mu+-=100, kappa=20, D=0, dt=0.01, "
+                                if w["system"] == "vortex" else
+                                f"chimera SRK, N={w['n']}, density 4000 (L={w['L']:.5f}), rho=0.1, v0=0.01, "
+                                "sigma=1, alpha=1.45, D=0.01, omega~N(0,0.1), dt=0.005, ")
                                + ("Verlet list, skin 0.2 rho, rebuilt at skin/2 displacement" if hbm
                                   else "linked cells per RHS"),
-                   "n": w["n"], "method": "rk4", "neighbour": nb,
+                   "n": w["n"], "method": w["method"], "neighbour": nb,
                    "list_builds_in_timed_steps": v["list_builds"]},
         "fp64_value": res["fp64"]["particle_steps_per_s"] if "fp64" in res else None,
         "fp64_factored_value": res["fp64f"]["particle_steps_per_s"] if "fp64f" in res else None,
@@ -252,8 +297,8 @@ def main():
             "peak": HBM_PEAK_GBS if hbm else PEAK["fp32k"],
             "unit": "GB/s" if hbm else "TFLOP/s",
             "frac": prof["gbs"] / HBM_PEAK_GBS if hbm else prof["tflops"] / PEAK["fp32k"],
-            "traffic": TRAFFIC_VERLET_FP32K if hbm and args.n == 1048576 else None,
-            "traffic_GBs": (TRAFFIC_VERLET_FP32K / (prof["avg_ms"] / 1e3) / 1e9) if hbm and args.n == 1048576 else None,
+            "traffic": TRAFFIC_VERLET_FP32K if headline else None,
+            "traffic_GBs": (TRAFFIC_VERLET_FP32K / (prof["avg_ms"] / 1e3) / 1e9) if headline else None,
             "avg_launch_ms": prof["avg_ms"], "bytes_per_launch": prof["bytes_per_launch"],
             "flops_per_launch": prof["flops_per_launch"], "valu_tflops": prof["tflops"],
             "valu_frac": prof["tflops"] / PEAK["fp32k"],
