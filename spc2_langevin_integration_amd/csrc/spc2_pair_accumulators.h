@@ -207,8 +207,9 @@ struct ChimAcc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
-    float4 v = s.f4[m];
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_v(s, m, s.f4[m], 0.0f, P); }
+  // pair() with the partner's tile already loaded (aj: pass-2 input, unused here)
+  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float, const KParams &P) {
     float dx = v.x + co.ox(m), dy = v.y + co.oy;
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f;
@@ -371,8 +372,9 @@ struct VortAcc<P32K> {
   }
   // Branch-free except for the rare cases (cutoff band, cohesion range): every
   // lane evaluates W and adds W sin, W cos masked to zero outside rho.
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
-    float4 v = s.f4[m];
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_v(s, m, s.f4[m], 0.0f, P); }
+  // pair() with the partner's tile already loaded (aj: pass-2 input, unused here)
+  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float, const KParams &P) {
     float dx = v.x + co.ox(m), dy = v.y + co.oy;
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f;
@@ -557,7 +559,18 @@ struct Chim2Acc<P32K> {
     bool in = r2 <= P.rho2f;
     if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
       in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
-    float aj = in ? s.aj[m] : 0.0f;
+    sum(v, in, in ? s.aj[m] : 0.0f);
+  }
+  // pair() with the partner's tile and pass-1 derivative already loaded
+  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float ajv, const KParams &P) {
+    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+    float r2 = __builtin_fmaf(dx, dx, dy * dy);
+    bool in = r2 <= P.rho2f;
+    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
+      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    sum(v, in, in ? ajv : 0.0f);
+  }
+  __device__ __forceinline__ void sum(const float4 &v, bool in, float aj) {
     float cz = in ? v.z : 0.0f, sw = in ? v.w : 0.0f;
     C.add(cz);
     S.add(sw);
@@ -741,8 +754,13 @@ struct Vort2Acc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
-    float4 v = s.f4[m];
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_t(s, m, s.f4[m], 0.0f, false, P); }
+  // pair() with the partner's tile and pass-1 derivative already loaded
+  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float ajv, const KParams &P) {
+    pair_t(s, m, v, ajv, true, P);
+  }
+  __device__ __forceinline__ void pair_t(const Src32 &s, int m, const float4 &v, float ajv, bool have_aj,
+                                         const KParams &P) {
     float dx = v.x + co.ox(m), dy = v.y + co.oy;
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f, inner = r2 < P.xa2f;
@@ -764,7 +782,7 @@ struct Vort2Acc<P32K> {
       float w = inner ? __builtin_fmaf(-P.mu_p_over_xa2f, r2, P.mu_pf) : P.c4f * (r - P.xaf) * (P.rhof - r);
       // W'(r) / r
       float g = inner ? -2.0f * P.mu_p_over_xa2f : P.c4f * (P.rhof + P.xaf - 2.0f * r) * ir;
-      float aj = s.aj[m];
+      float aj = have_aj ? ajv : s.aj[m];
       A0.add(w * cph);
       A1.add((dvx * dx + dvy * dy) * g * sph + (aj - aif) * w * cph - 2.0f * P.Df * sph * w);
     }

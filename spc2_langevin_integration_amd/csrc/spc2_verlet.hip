@@ -146,6 +146,46 @@ __device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, con
   acc.pair(src, j, P);
 }
 
+// FP32 list walk with kWalk entries in flight: the next group's list entries
+// and the current group's partner tiles (and pass-2 derivatives) are requested
+// before the group is summed, in list order, so one memory latency covers the
+// group.  Indices past the end repeat the last entry (in bounds, not summed).
+constexpr int kWalk = 4;
+
+template <int PASS, class Acc>
+__device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int *l, int cnt, const KParams &P) {
+  if (cnt <= 0) return;
+  const int last = cnt - 1;
+  int e[kWalk];
+#pragma unroll
+  for (int u = 0; u < kWalk; ++u) e[u] = l[(size_t)min(u, last) * 64];
+  for (int q = 0; q < cnt; q += kWalk) {
+    float4 v[kWalk];
+    float a[kWalk];
+#pragma unroll
+    for (int u = 0; u < kWalk; ++u) {
+      v[u] = src.f4[e[u] >> 4];
+      a[u] = PASS == 2 ? src.aj[e[u] >> 4] : 0.0f;
+    }
+    int nx[kWalk];
+#pragma unroll
+    for (int u = 0; u < kWalk; ++u) nx[u] = l[(size_t)min(q + kWalk + u, last) * 64];
+#pragma unroll
+    for (int u = 0; u < kWalk; ++u) {
+      if (u == 0 || q + u < cnt) {
+        const int ent = e[u];
+        const int j = ent >> 4, dcx = (ent & 3) - 1, dcy = ((ent >> 2) & 3) - 1;
+        const int b1 = dcx < 0 ? j + 1 : (dcx == 0 ? j : INT_MIN);
+        const int b2 = dcx < 0 ? INT_MAX : (dcx == 0 ? j + 1 : j);
+        acc.row(dcy, b1, b2, P);
+        acc.pair_v(src, j, v[u], a[u], P);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kWalk; ++u) e[u] = nx[u];
+  }
+}
+
 template <int PREC>
 __device__ __forceinline__ typename SrcOf<PREC>::type verlet_src(const Tiles &t, const EpiCtx &e, int pass) {
   if constexpr (PREC == P32K) {
@@ -176,8 +216,12 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet(KParams P, const int *_
     acc.init(src, i, i, t, P);
     cnt = vcnt[i];
     const int *l = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+    if constexpr (PREC == P32K) {
+      walk32<1>(acc, src, l, cnt, P);
+    } else {
 #pragma unroll 2
-    for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
+      for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
+    }
     acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
     epilogue1<EPI>(P, e, i, hid[i], i, o);
   }
@@ -201,8 +245,12 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   acc.init(src, i, i, t, P, in);
   const int cnt = vcnt[i];
   const int *l = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+  if constexpr (PREC == P32K) {
+    walk32<2>(acc, src, l, cnt, P);
+  } else {
 #pragma unroll 2
-  for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
+    for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
+  }
   Pass2Out o;
   acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
   epilogue2<EPI>(P, e, i, hid[i], o);
