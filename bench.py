@@ -51,13 +51,14 @@ def workload(n=1048576, name="vortex-rk4"):
 
 
 def make_sim(spc2, w, precision, neighbour="verlet"):
+    skin = w.get("skin", 0.2) * w["rho"]
     if w["system"] == "chimera":
         p = spc2.params("chimera", precision, neighbour, n=w["n"], L=w["L"], rho=w["rho"], v0=w["v0"],
-                        sigma=w["sigma"], alpha=w["alpha"], D=w["D"], seed=w["seed"])
+                        sigma=w["sigma"], alpha=w["alpha"], D=w["D"], seed=w["seed"], verlet_skin=skin)
     else:
         p = spc2.params("vortex", precision, neighbour, n=w["n"], L=w["L"], rho=w["rho"], velocity=w["velocity"],
                         mu_plus=w["mu_plus"], mu_minus=w["mu_minus"], xi_a=w["xi_a"], xi_r=w["xi_r"],
-                        kappa=w["kappa"], D=w["D"], seed=w["seed"])
+                        kappa=w["kappa"], D=w["D"], seed=w["seed"], verlet_skin=skin)
     s = spc2.Sim(p)
     s.init_random_state(w["omega_std"])
     return s
@@ -214,6 +215,8 @@ def main():
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: a few steps of the FP32 workload and a plain device copy of "
                          "COPY_BYTES bytes (the counter calibration), no timing")
+    ap.add_argument("--skin", type=float, default=0.2,
+                    help="Verlet skin as a fraction of rho")
     ap.add_argument("--error-horizon", type=int, default=100,
                     help="steps after which FP32 and FP64 runs are compared before chaotic decorrelation")
     args = ap.parse_args()
@@ -224,6 +227,7 @@ def main():
 
     import spc2_langevin_integration_amd as spc2
     w = workload(args.n, args.workload)
+    w["skin"] = args.skin
     if args.neighbour is None:
         args.neighbour = "verlet" if w["system"] == "vortex" else "cells"
     if args.pmc_run:
@@ -283,7 +287,7 @@ def main():
                                 if w["system"] == "vortex" else
                                 f"chimera SRK, N={w['n']}, density 4000 (L={w['L']:.5f}), rho=0.1, v0=0.01, "
                                 "sigma=1, alpha=1.45, D=0.01, omega~N(0,0.1), dt=0.005, ")
-                               + ("Verlet list, skin 0.2 rho, rebuilt at skin/2 displacement" if hbm
+                               + (f"Verlet list, skin {args.skin:g} rho, rebuilt at skin/2 displacement" if hbm
                                   else "linked cells per RHS"),
                    "n": w["n"], "method": w["method"], "neighbour": nb,
                    "list_builds_in_timed_steps": v["list_builds"]},

@@ -65,6 +65,7 @@ void free_all(Sim &s) {
   for (auto &p : s.spare) f(p);
   for (auto &p : s.io) f(p);
   f(s.t.x); f(s.t.y); f(s.t.p); f(s.t.c); f(s.t.s); f(s.t.f4); f(s.t.cell);
+  f(s.t2.x); f(s.t2.y); f(s.t2.p); f(s.t2.c); f(s.t2.s); f(s.t2.f4); f(s.gp2);
   f(s.aphi_slot); f(s.n_slot); f(s.n2_slot); f(s.aphi_slot_f);
   f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
   f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb);
@@ -159,14 +160,21 @@ spc2_status alloc_all(Sim &s) {
   if (s.linked)
     for (auto &p : s.spare) A(p, n);
   A(s.t.cell, n);
-  if (s.prec == P32K) {
-    A(s.t.f4, n);
-    A(s.t.x, n); A(s.t.y, n);  // FP64 positions for the cutoff-band re-check
-  } else {
-    A(s.t.x, n); A(s.t.y, n); A(s.t.p, n);
-    if (s.prec == P64F) {
-      A(s.t.c, n); A(s.t.s, n);
+  for (Tiles *tt : {&s.t, &s.t2}) {
+    if (tt == &s.t2 && !s.verlet) break;
+    if (s.prec == P32K) {
+      A(tt->f4, n);
+      A(tt->x, n); A(tt->y, n);  // FP64 positions for the cutoff-band re-check
+    } else {
+      A(tt->x, n); A(tt->y, n); A(tt->p, n);
+      if (s.prec == P64F) {
+        A(tt->c, n); A(tt->s, n);
+      }
     }
+  }
+  if (s.verlet) {
+    A(s.gp2, n);
+    s.t2.cell = s.t.cell;
   }
   A(s.aphi_slot, n); A(s.n_slot, n); A(s.n2_slot, n); A(s.aphi_slot_f, n);
   A(s.counts, 3 * kCountSlots); A(s.red, 2 * 1024 + 8);
@@ -237,7 +245,7 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt) {
   if (rows < 0 || (long long)rows > 4096LL * s.nw)
     return fail(SPC2_ERR_INVALID, "Verlet list of " + std::to_string(rows) + " rows for " + std::to_string(s.nw) +
                                       " wavefronts: skin or density too large for a neighbour list");
-  const size_t need = (size_t)rows * 64;
+  const size_t need = ((size_t)rows + kVerletSpareRows) * 64;
   if (need > s.vlist_cap || !s.vlist) {
     if (s.vlist) (void)hipFree(s.vlist);
     s.vlist = nullptr;
@@ -247,9 +255,11 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt) {
     s.vlist_cap = cap;
   }
   launch_verlet_fill(s);
+  HIP_TRY(hipMemsetAsync(s.vlist + (size_t)rows * 64, 0, sizeof(int) * 64 * kVerletSpareRows, s.stream));
   if (spc2_status st = check_launch("verlet build")) return st;
   s.vbuilt = true;
   s.built = false;
+  s.tile_fresh = false;
   s.steps_since_rehome = 0;
   s.drift = kprev ? std::numeric_limits<double>::infinity() : 0.0;
   s.vbuilds += 1;
@@ -284,6 +294,7 @@ spc2_status verlet_ensure(Sim &s, double reach) {
 // Stage + cell list + tiles for one RHS evaluation (Verlet: stage and tiles
 // by home only; the list is kept valid by verlet_ensure).
 void build(Sim &s, double *const *kprev, double kcdt) {
+  s.tile_fresh = false;
   if (s.verlet) {
     if (kprev)
       launch_verlet_stage(s, kprev[0], kprev[1], kprev[2], kcdt);
@@ -324,6 +335,13 @@ void maybe_rehome(Sim &s) {
   launch_rehome(s);
   s.built = false;
   s.steps_since_rehome = 0;
+}
+
+// Verlet: the next pass-1 launch also writes the next stage's tiles (mode 1:
+// s + kcdt k of that launch; 2: the state it writes).
+void next_stage(Sim &s, int mode, double kcdt) {
+  s.vnext = mode;
+  s.vnext_kcdt = kcdt;
 }
 
 // RHS passes on the handle's neighbour structure.
@@ -593,8 +611,17 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
     StepArgs sa = step_args(s, dt, (uint64_t)(step0 + k));
     sa.wrap = (flags & SPC2_STEP_NO_WRAP) ? 0 : 1;
     if (noise && method != SPC2_RK4) sa.noise = s.noise + (size_t)k * npp * (size_t)s.n;
+    // Verlet SE and RK4: each pass-1 launch also writes the tiles of the next
+    // stage (the next step's first stage after the update), so a separate
+    // stage pass is needed only after a list build or another call
+    const bool fused = s.verlet && method != SPC2_SRK;
+    if (fused && !s.tile_fresh) build(s, nullptr, 0.0);
     if (method == SPC2_SE) {
-      build(s, nullptr, 0.0);
+      if (fused) {
+        next_stage(s, 2, 0.0);
+      } else {
+        build(s, nullptr, 0.0);
+      }
       rhs_pass1(s, EPI_SE, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
     } else if (method == SPC2_SRK) {
       build(s, nullptr, 0.0);
@@ -605,15 +632,27 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
       // RungeKutta4StepperPtr.cpp:25-43: k1 = f(s, k1, 0), k2 = f(s, k1, 1/2),
       // k3 = f(s, k2, 1/2), k4 = f(s, k3, 1), stage = s + (c * dt) * k
       double *const *k1 = s.k[0], *const *k2 = s.k[1], *const *k3 = s.k[2], *const *k4 = s.k[3];
-      build(s, k1, 0.0 * dt);
-      rhs_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);
-      build(s, k1, 0.5 * dt);
-      rhs_pass1(s, EPI_K, k2[0], k2[1], k2[2], sa, nullptr, nullptr, nullptr);
-      build(s, k2, 0.5 * dt);
-      rhs_pass1(s, EPI_K, k3[0], k3[1], k3[2], sa, nullptr, nullptr, nullptr);
-      build(s, k3, 1.0 * dt);
-      rhs_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
+      if (fused) {  // the first stage s + 0 k1 is the state itself
+        next_stage(s, 1, 0.5 * dt);
+        rhs_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);
+        next_stage(s, 1, 0.5 * dt);
+        rhs_pass1(s, EPI_K, k2[0], k2[1], k2[2], sa, nullptr, nullptr, nullptr);
+        next_stage(s, 1, 1.0 * dt);
+        rhs_pass1(s, EPI_K, k3[0], k3[1], k3[2], sa, nullptr, nullptr, nullptr);
+        next_stage(s, 2, 0.0);
+        rhs_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
+      } else {
+        build(s, k1, 0.0 * dt);
+        rhs_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);
+        build(s, k1, 0.5 * dt);
+        rhs_pass1(s, EPI_K, k2[0], k2[1], k2[2], sa, nullptr, nullptr, nullptr);
+        build(s, k2, 0.5 * dt);
+        rhs_pass1(s, EPI_K, k3[0], k3[1], k3[2], sa, nullptr, nullptr, nullptr);
+        build(s, k3, 1.0 * dt);
+        rhs_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
+      }
     }
+    if (fused) s.tile_fresh = true;
     if (s.verlet) {
       // SE and RK4 move a particle by at most v0 dt per step; SRK adds noise to
       // the positions, so its displacement is measured before the next use

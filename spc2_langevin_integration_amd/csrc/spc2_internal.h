@@ -101,6 +101,13 @@ struct Sim {
   int *spare_i = nullptr;
   double *io[3] = {};
   Tiles t{};
+  // Verlet: the next stage's tiles and phases, written by a pass-1 epilogue
+  // (t2.cell aliases t.cell); swapped with t / gp after that launch
+  Tiles t2{};
+  double *gp2 = nullptr;
+  int vnext = 0;            // next launch's stage write: 0 none, 1 s + vnext_kcdt k, 2 the new state
+  double vnext_kcdt = 0.0;
+  bool tile_fresh = false;  // t / gp hold the tiles of the current state
   // pass-1 results kept per slot for pass 2
   double *aphi_slot = nullptr, *n_slot = nullptr, *n2_slot = nullptr;
   float *aphi_slot_f = nullptr;
@@ -184,6 +191,8 @@ void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const Ste
                   double *const k2[3], double *const k3[3]);
 void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
 // spc2_verlet.hip
+// valid rows kept after a Verlet list for the passes' read-ahead
+constexpr int kVerletSpareRows = 8;
 void launch_verlet_count(Sim &s);
 void launch_verlet_fill(Sim &s);
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);

@@ -96,9 +96,21 @@ struct Pass1Out {
 struct ColOffsets {
   float oM, hxf, oy;
   int b1, b2;
+  float kx, ky;  // -h - xl, -h - yl: the offsets of stencil column / row d are (d + 1) h + k
   __device__ __forceinline__ void set(float xl, float h) {
     oM = -xl;
     hxf = h;
+  }
+  __device__ __forceinline__ void set_codes(float xl, float yl, float hx, float hy) {
+    kx = -hx - xl;
+    ky = -hy - yl;
+  }
+  // x / y offsets from a list entry's stencil code (dcx + 1) | (dcy + 1) << 2
+  __device__ __forceinline__ float cdx(int code, float hx) const {
+    return __builtin_fmaf((float)(code & 3), hx, kx);
+  }
+  __device__ __forceinline__ float cdy(int code, float hy) const {
+    return __builtin_fmaf((float)((code >> 2) & 3), hy, ky);
   }
   __device__ __forceinline__ float ox(int m) const {
     float sel = (float)((m >= b2) - (m < b1));
@@ -195,6 +207,7 @@ struct ChimAcc<P32K> {
     xl = v.x;
     yl = v.y;
     co.set(xl, P.hxf);
+    co.set_codes(xl, yl, P.hxf, P.hyf);
     S.init();
     C.init();
     n = 0;
@@ -207,10 +220,16 @@ struct ChimAcc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_v(s, m, s.f4[m], 0.0f, P); }
-  // pair() with the partner's tile already loaded (aj: pass-2 input, unused here)
-  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float, const KParams &P) {
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    const float4 v = s.f4[m];
+    pair_d(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
+  }
+  // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float, const KParams &P) {
+    pair_d(s, m, v, v.x + co.cdx(code, P.hxf), v.y + co.cdy(code, P.hyf), P);
+  }
+  __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
+                                         const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f;
     if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
@@ -356,6 +375,7 @@ struct VortAcc<P32K> {
     xl = v.x;
     yl = v.y;
     co.set(xl, P.hxf);
+    co.set_codes(xl, yl, P.hxf, P.hyf);
     SW.init();
     CW.init();
     Tx.init();
@@ -372,13 +392,20 @@ struct VortAcc<P32K> {
   }
   // Branch-free except for the rare cases (cutoff band, cohesion range): every
   // lane evaluates W and adds W sin, W cos masked to zero outside rho.
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_v(s, m, s.f4[m], 0.0f, P); }
-  // pair() with the partner's tile already loaded (aj: pass-2 input, unused here)
-  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float, const KParams &P) {
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    const float4 v = s.f4[m];
+    pair_d(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
+  }
+  // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float, const KParams &P) {
+    pair_d(s, m, v, v.x + co.cdx(code, P.hxf), v.y + co.cdy(code, P.hyf), P);
+  }
+  __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
+                                         const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f || __builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f) {
+    // one branch for either band (no short circuit: both tests are cheaper than a branch)
+    if ((__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f) | (__builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f)) {
       // the reference tests sqrt(r^2) <= cutoff in FP64 (VortexArraysSystemPtr.cpp:473-488)
       double d = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy));
       inr = d <= P.rho;
@@ -535,6 +562,7 @@ struct Chim2Acc<P32K> {
     xl = v.x;
     yl = v.y;
     co.set(xl, P.hxf);
+    co.set_codes(xl, yl, P.hxf, P.hyf);
     ai = in.ai;
     n = in.n;
     in0 = in.in0;
@@ -561,9 +589,10 @@ struct Chim2Acc<P32K> {
       in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
     sum(v, in, in ? s.aj[m] : 0.0f);
   }
-  // pair() with the partner's tile and pass-1 derivative already loaded
-  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float ajv, const KParams &P) {
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+  // a list entry with the partner's tile and pass-1 derivative already loaded
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float ajv,
+                                         const KParams &P) {
+    float dx = v.x + co.cdx(code, P.hxf), dy = v.y + co.cdy(code, P.hyf);
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f;
     if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
@@ -736,6 +765,7 @@ struct Vort2Acc<P32K> {
     cif = v.z;
     sif = v.w;
     co.set(xl, P.hxf);
+    co.set_codes(xl, yl, P.hxf, P.hyf);
     ai = in.ai;
     aif = (float)in.ai;
     n1 = in.n;
@@ -754,18 +784,21 @@ struct Vort2Acc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_t(s, m, s.f4[m], 0.0f, false, P); }
-  // pair() with the partner's tile and pass-1 derivative already loaded
-  __device__ __forceinline__ void pair_v(const Src32 &s, int m, const float4 &v, float ajv, const KParams &P) {
-    pair_t(s, m, v, ajv, true, P);
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    const float4 v = s.f4[m];
+    pair_t(s, m, v, v.x + co.ox(m), v.y + co.oy, 0.0f, false, P);
   }
-  __device__ __forceinline__ void pair_t(const Src32 &s, int m, const float4 &v, float ajv, bool have_aj,
+  // a list entry with the partner's tile and pass-1 derivative already loaded
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float ajv,
                                          const KParams &P) {
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+    pair_t(s, m, v, v.x + co.cdx(code, P.hxf), v.y + co.cdy(code, P.hyf), ajv, true, P);
+  }
+  __device__ __forceinline__ void pair_t(const Src32 &s, int m, const float4 &v, float dx, float dy, float ajv,
+                                         bool have_aj, const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f, inner = r2 < P.xa2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f || __builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f ||
-        __builtin_fabsf(r2 - P.xa2f) <= P.band_xa2f) {
+    if ((__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f) | (__builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f) |
+        (__builtin_fabsf(r2 - P.xa2f) <= P.band_xa2f)) {
       double d = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy));
       inr = d <= P.rho;
       inc = d <= P.xi_r;
@@ -836,6 +869,18 @@ struct EpiCtx {
   float *aphi_slot_f;
   double *a0, *a1;                // additional by particle (in/out)
   StepArgs sa;
+  // Verlet passes: the next stage's tiles, written by the pass-1 epilogue
+  // (next = 1: s + kcdt_n k of this launch; 2: the state this launch wrote)
+  Tiles tn;
+  double *gpn;
+  double kcdt_n;
+  int next;
+};
+
+// What a pass-1 epilogue computed for its particle: the velocity, the phi
+// derivative and, for the state updates, the new state.
+struct EpiState {
+  double ax, ay, ap, x, y, p;
 };
 
 __device__ __forceinline__ void wrap_state(const KParams &P, double &x, double &y) {
@@ -853,12 +898,15 @@ __device__ __forceinline__ double phi_out(const Pass1Out &o, double old) {
 }
 
 template <int EPI>
-__device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int h, int id, int slot, const Pass1Out &o) {
+__device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e, int h, int id, int slot,
+                                              const Pass1Out &o) {
   double sn, cs;
   sincos(o.phase, &sn, &cs);
   double ax = P.v0 * cs, ay = P.v0 * sn;
+  EpiState r{ax, ay, 0.0, 0.0, 0.0, 0.0};
   if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
     double ap = phi_out(o, e.sa.zero_in ? 0.0 : e.op[h]);
+    r.ap = ap;
     e.ox[h] = ax;
     e.oy[h] = ay;
     e.op[h] = ap;
@@ -880,6 +928,9 @@ __device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int
       e.sx[h] = x;
       e.sy[h] = y;
       e.sp[h] = p;
+      r.x = x;
+      r.y = y;
+      r.p = p;
     }
   } else if (EPI == EPI_SE) {
     // StochasticEulerStepperPtr.hpp:44-53 with a zero-filled derivative and the
@@ -903,7 +954,12 @@ __device__ __forceinline__ void epilogue1(const KParams &P, const EpiCtx &e, int
     e.sx[h] = x;
     e.sy[h] = y;
     e.sp[h] = p;
+    r.ap = ap;
+    r.x = x;
+    r.y = y;
+    r.p = p;
   }
+  return r;
 }
 
 template <int EPI>

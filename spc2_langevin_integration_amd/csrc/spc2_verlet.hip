@@ -80,17 +80,21 @@ __global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *s
   if ((threadIdx.x & 63) == 0 && w * 64 < P.n) wmax[w] = m;
 }
 
-// Build pass 2: the entries, in the same order as the count.
+// Build pass 2: the entries, in the same order as the count, then the column
+// padded to the wavefront's longest list with a valid entry (itself, never
+// summed: the passes stop at the count).
 __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *start, Tiles t, double rlist2,
                                                         const int *wrow, int *vlist) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  int *dst = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+  const int w = i >> 6, rows = wrow[w + 1] - wrow[w];
+  int *dst = vlist + (size_t)wrow[w] * 64 + (i & 63);
   int e = 0;
   for_each_build_neighbour(P, start, t, i, rlist2, [&](int j, int code) {
     dst[(size_t)e * 64] = (j << 4) | code;
     ++e;
   });
+  for (; e < rows; ++e) dst[(size_t)e * 64] = (i << 4) | 5;
 }
 
 // Per-RHS stage: s + (k_coef dt) k by home, the FP64 tile (conditionally
@@ -98,17 +102,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *st
 // coordinates are relative to the particle's build cell, taken at the nearest
 // periodic image (a particle is within skin / 2 of its build position).
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_verlet_stage(KParams P, const double *sx, const double *sy,
-                                                         const double *sp, const double *kx, const double *ky,
-                                                         const double *kp, double kcdt, Tiles t, double *gp) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= P.n) return;
-  double x = sx[h], y = sy[h], p = sp[h];
-  if (kx) {
-    x = x + kcdt * kx[h];
-    y = y + kcdt * ky[h];
-    p = p + kcdt * kp[h];
-  }
+__device__ __forceinline__ void write_stage(const KParams &P, const Tiles &t, double *gp, int h, double x, double y,
+                                            double p) {
   Box b{P.Lx, P.Ly, P.rLx, P.rLy};
   cond_wrap(b, x, y);
   t.x[h] = x;
@@ -135,6 +130,21 @@ __global__ __launch_bounds__(kBlock) void k_verlet_stage(KParams P, const double
   }
 }
 
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_verlet_stage(KParams P, const double *sx, const double *sy,
+                                                         const double *sp, const double *kx, const double *ky,
+                                                         const double *kp, double kcdt, Tiles t, double *gp) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= P.n) return;
+  double x = sx[h], y = sy[h], p = sp[h];
+  if (kx) {
+    x = x + kcdt * kx[h];
+    y = y + kcdt * ky[h];
+    p = p + kcdt * kp[h];
+  }
+  write_stage<PREC>(P, t, gp, h, x, y, p);
+}
+
 // One list entry into the accumulator: the FP32 form learns the stencil cell
 // offset through row() and the column bounds (sel = (m >= b2) - (m < b1)).
 template <class Acc, class S>
@@ -149,37 +159,38 @@ __device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, con
 // FP32 list walk with kWalk entries in flight: the next group's list entries
 // and the current group's partner tiles (and pass-2 derivatives) are requested
 // before the group is summed, in list order, so one memory latency covers the
-// group.  Indices past the end repeat the last entry (in bounds, not summed).
+// group.  Reads run up to 2 kWalk - 1 rows past a particle's last entry: the
+// build pads every column to its wavefront's longest list with a valid entry
+// and leaves kVerletSpareRows valid rows after the list.  An entry's upper bits
+// (j << 4) are the byte offset of tile j, so a gather is one mask from the
+// entry, addressed from the uniform tile base.
 constexpr int kWalk = 4;
+static_assert(2 * kWalk <= kVerletSpareRows, "list prefetch runs past the spare rows");
 
 template <int PASS, class Acc>
 __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int *l, int cnt, const KParams &P) {
   if (cnt <= 0) return;
-  const int last = cnt - 1;
+  const char *f4b = (const char *)src.f4;
+  const char *ajb = (const char *)src.aj;
   int e[kWalk];
 #pragma unroll
-  for (int u = 0; u < kWalk; ++u) e[u] = l[(size_t)min(u, last) * 64];
+  for (int u = 0; u < kWalk; ++u) e[u] = l[u * 64];
   for (int q = 0; q < cnt; q += kWalk) {
     float4 v[kWalk];
     float a[kWalk];
 #pragma unroll
     for (int u = 0; u < kWalk; ++u) {
-      v[u] = src.f4[e[u] >> 4];
-      a[u] = PASS == 2 ? src.aj[e[u] >> 4] : 0.0f;
+      const uint32_t off = (uint32_t)e[u] & ~15u;
+      v[u] = *(const float4 *)(f4b + off);
+      a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
     }
+    l += kWalk * 64;
     int nx[kWalk];
 #pragma unroll
-    for (int u = 0; u < kWalk; ++u) nx[u] = l[(size_t)min(q + kWalk + u, last) * 64];
+    for (int u = 0; u < kWalk; ++u) nx[u] = l[u * 64];
 #pragma unroll
     for (int u = 0; u < kWalk; ++u) {
-      if (u == 0 || q + u < cnt) {
-        const int ent = e[u];
-        const int j = ent >> 4, dcx = (ent & 3) - 1, dcy = ((ent >> 2) & 3) - 1;
-        const int b1 = dcx < 0 ? j + 1 : (dcx == 0 ? j : INT_MIN);
-        const int b2 = dcx < 0 ? INT_MAX : (dcx == 0 ? j + 1 : j);
-        acc.row(dcy, b1, b2, P);
-        acc.pair_v(src, j, v[u], a[u], P);
-      }
+      if (u == 0 || q + u < cnt) acc.pair_c(src, e[u] >> 4, e[u] & 15, v[u], a[u], P);
     }
 #pragma unroll
     for (int u = 0; u < kWalk; ++u) e[u] = nx[u];
@@ -223,7 +234,14 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet(KParams P, const int *_
       for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
     }
     acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
-    epilogue1<EPI>(P, e, i, hid[i], i, o);
+    const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
+    if (EPI == EPI_K && e.next == 1) {
+      // the next stage s + kcdt_n k, as k_verlet_stage forms it from k in memory
+      write_stage<PREC>(P, e.tn, e.gpn, i, e.sx[i] + e.kcdt_n * r.ax, e.sy[i] + e.kcdt_n * r.ay,
+                        e.sp[i] + e.kcdt_n * r.ap);
+    } else if ((EPI == EPI_RK4_FINAL || EPI == EPI_SE) && e.next == 2) {
+      write_stage<PREC>(P, e.tn, e.gpn, i, r.x, r.y, r.p);  // the next step's first stage
+    }
   }
   if (counts) count_add(counts, cnt, active ? o.n : 0, active ? o.n2 : 0);
 }
@@ -294,10 +312,12 @@ void vpass1_typed(Sim &s, const EpiCtx &e) {
   const dim3 grid(blocks_for(s.n)), blk(kBlock);
   if (SYS == SPC2_CHIMERA)
     launch_pair(s, k_pass1_verlet<ChimAcc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
-                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e, cnt);
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid,
+                (const double *)s.gp, e, cnt);
   else
     launch_pair(s, k_pass1_verlet<VortAcc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
-                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e, cnt);
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid,
+                (const double *)s.gp, e, cnt);
 }
 
 template <int PREC, int SYS>
@@ -323,10 +343,12 @@ void vpass2_typed(Sim &s, const EpiCtx &e) {
   const dim3 grid(blocks_for(s.n)), blk(kBlock);
   if (SYS == SPC2_CHIMERA)
     launch_pair(s, k_pass2_verlet<Chim2Acc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
-                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e);
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid,
+                (const double *)s.gp, e);
   else
     launch_pair(s, k_pass2_verlet<Vort2Acc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
-                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid, (const double *)s.gp, e);
+                (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid,
+                (const double *)s.gp, e);
 }
 
 template <int PREC>
@@ -374,10 +396,27 @@ void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const doubl
 void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
                          double *const k2[3], double *const k3[3]) {
   EpiCtx e = make_ctx(s, ox, oy, op, sa, k1, k2, k3);
+  const int next = s.vnext;
+  s.vnext = 0;
+  if (next) {
+    e.tn = s.t2;
+    e.gpn = s.gp2;
+    e.kcdt_n = s.vnext_kcdt;
+    e.next = next;
+  }
   switch (s.prec) {
     case P64: vpass1_sys<P64>(s, epi, e); break;
     case P64F: vpass1_sys<P64F>(s, epi, e); break;
     default: vpass1_sys<P32K>(s, epi, e);
+  }
+  if (next) {  // the written tiles become the current ones (cells are shared)
+    std::swap(s.t.x, s.t2.x);
+    std::swap(s.t.y, s.t2.y);
+    std::swap(s.t.p, s.t2.p);
+    std::swap(s.t.c, s.t2.c);
+    std::swap(s.t.s, s.t2.s);
+    std::swap(s.t.f4, s.t2.f4);
+    std::swap(s.gp, s.gp2);
   }
 }
 
