@@ -223,7 +223,7 @@ spc2_status download_aos(Sim &s, double *aos, int stride, double *const *src, in
 // Verlet list from the positions s + kcdt * kprev (the state when kprev is
 // null): bin them on the build grid, move the particle arrays into the build's
 // cell order, then count, scan and fill the wavefront-interleaved list.
-spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt) {
+spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_live) {
   if (kprev)
     launch_stage_bin(s, kprev[0], kprev[1], kprev[2], kcdt);
   else
@@ -231,31 +231,32 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt) {
   launch_scan(s);
   launch_place(s);
   launch_rank_tiles(s);
-  launch_rehome(s);  // slot k becomes home k: tiles, cells and start[] are now by home
+  launch_rehome(s, k_xy_live);  // slot k becomes home k: tiles, cells and start[] are now by home
   HIP_TRY(hipMemcpyAsync(s.xb, s.t.x, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
   HIP_TRY(hipMemcpyAsync(s.yb, s.t.y, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
   launch_verlet_count(s);
   launch_scan_array(s, s.wmax, s.nw, s.wrow);
   if (spc2_status st = check_launch("verlet count")) return st;
-  int rows = 0;
-  HIP_TRY(hipMemcpyAsync(&rows, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  int groups = 0;
+  HIP_TRY(hipMemcpyAsync(&groups, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
-  // rows = sum over wavefronts of their longest list; more than 4096 entries
-  // per particle means a skin or density the list is not meant for
-  if (rows < 0 || (long long)rows > 4096LL * s.nw)
-    return fail(SPC2_ERR_INVALID, "Verlet list of " + std::to_string(rows) + " rows for " + std::to_string(s.nw) +
-                                      " wavefronts: skin or density too large for a neighbour list");
-  const size_t need = ((size_t)rows + kVerletSpareRows) * 64;
+  // groups = sum over wavefronts of their longest list in groups of four; more
+  // than 4096 entries per particle means a skin or density the list is not
+  // meant for
+  if (groups < 0 || (long long)groups * 4 > 4096LL * s.nw)
+    return fail(SPC2_ERR_INVALID, "Verlet list of " + std::to_string(groups) + " groups for " +
+                                      std::to_string(s.nw) + " wavefronts: skin or density too large for a neighbour list");
+  const size_t need = ((size_t)groups + kVerletSpareGroups) * 256;
   if (need > s.vlist_cap || !s.vlist) {
     if (s.vlist) (void)hipFree(s.vlist);
     s.vlist = nullptr;
     s.vlist_cap = 0;
-    const size_t cap = need + need / 8 + 64 * 64;
+    const size_t cap = need + need / 8 + 64 * 256;
     HIP_TRY(dalloc(&s.vlist, cap));
     s.vlist_cap = cap;
   }
   launch_verlet_fill(s);
-  HIP_TRY(hipMemsetAsync(s.vlist + (size_t)rows * 64, 0, sizeof(int) * 64 * kVerletSpareRows, s.stream));
+  HIP_TRY(hipMemsetAsync(s.vlist + (size_t)groups * 256, 0, sizeof(int) * 256 * kVerletSpareGroups, s.stream));
   if (spc2_status st = check_launch("verlet build")) return st;
   s.vbuilt = true;
   s.built = false;
@@ -281,14 +282,14 @@ spc2_status verlet_measure(Sim &s) {
 // state: stage positions are within drift + reach of the build positions, and
 // every pair within rho at the stage was within rho + skin at the build while
 // that stays <= skin / 2.
-spc2_status verlet_ensure(Sim &s, double reach) {
+spc2_status verlet_ensure(Sim &s, double reach, bool k_xy_live) {
   const double half = 0.5 * s.skin * (1.0 - 1e-9);
   if (s.vbuilt && s.drift + reach <= half) return SPC2_OK;
   if (s.vbuilt && std::isinf(s.drift)) {
     if (spc2_status st = verlet_measure(s)) return st;
     if (s.drift + reach <= half) return SPC2_OK;
   }
-  return verlet_build(s, nullptr, 0.0);
+  return verlet_build(s, nullptr, 0.0, k_xy_live);
 }
 
 // Stage + cell list + tiles for one RHS evaluation (Verlet: stage and tiles
@@ -317,7 +318,7 @@ void build(Sim &s, double *const *kprev, double kcdt) {
 // Moves the particle arrays to the cell order of the current state (used
 // when a new state arrives).
 spc2_status rehome_now(Sim &s) {
-  if (s.verlet) return verlet_build(s, nullptr, 0.0);
+  if (s.verlet) return verlet_build(s, nullptr, 0.0, true);
   if (!s.linked) return SPC2_OK;
   build(s, nullptr, 0.0);
   launch_rehome(s);
@@ -550,8 +551,8 @@ spc2_status spc2_eval_rhs(spc2_sim_t h, const double *k_prev, double k_coef, dou
     double kmax = 0.0;
     for (int64_t i = 0; i < s.n; ++i) kmax = std::max(kmax, std::hypot(k_prev[3 * i], k_prev[3 * i + 1]));
     const double reach = std::fabs(k_coef * dt) * kmax * (1.0 + 1e-12);
-    spc2_status st = reach <= 0.5 * s.skin * (1.0 - 1e-9) ? verlet_ensure(s, reach)
-                                                           : verlet_build(s, s.k[1], k_coef * dt);
+    spc2_status st = reach <= 0.5 * s.skin * (1.0 - 1e-9) ? verlet_ensure(s, reach, true)
+                                                           : verlet_build(s, s.k[1], k_coef * dt, true);
     if (st) return st;
   }
   build(s, s.k[1], k_coef * dt);
@@ -567,7 +568,7 @@ spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *addition
   if (spc2_status st = upload_aos(s, derivative, 3, s.der, 3)) return st;
   if (spc2_status st = upload_aos(s, additional, 2, s.add, 2)) return st;
   if (s.verlet)
-    if (spc2_status st = verlet_ensure(s, 0.0)) return st;
+    if (spc2_status st = verlet_ensure(s, 0.0, true)) return st;
   build(s, nullptr, 0.0);
   StepArgs sa = step_args(s, 0.0, 0);
   rhs_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
@@ -605,7 +606,9 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
       // RK4 stages lie within v0 dt of the state (|k_x, k_y| = v0), SE and SRK
       // evaluate at the state.  A rebuild re-homes the particle arrays, so the
       // step's arguments are taken after it.
-      if (spc2_status st = verlet_ensure(s, method == SPC2_RK4 ? std::fabs(s.P.v0) * dt * (1.0 + 1e-12) : 0.0))
+      // (the x / y components of k are dead here: a step writes them first)
+      if (spc2_status st = verlet_ensure(s, method == SPC2_RK4 ? std::fabs(s.P.v0) * dt * (1.0 + 1e-12) : 0.0,
+                                         false))
         return st;
     }
     StepArgs sa = step_args(s, dt, (uint64_t)(step0 + k));

@@ -123,6 +123,8 @@ struct Sim {
   double skin = 0.0, rlist = 0.0;
   double drift = 0.0;  // bound on the largest |state - build position| (inf: unknown)
   int nw = 0;          // wavefronts of 64 homes
+  // wmax[w] / wrow[w]: wavefront w's group count and first group (see the list
+  // layout in spc2_verlet.hip); vlist_cap in ints
   int *vcnt = nullptr, *wmax = nullptr, *wrow = nullptr, *vlist = nullptr;
   size_t vlist_cap = 0;
   double *xb = nullptr, *yb = nullptr;  // build positions by home
@@ -191,8 +193,9 @@ void launch_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const Ste
                   double *const k2[3], double *const k3[3]);
 void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
 // spc2_verlet.hip
-// valid rows kept after a Verlet list for the passes' read-ahead
-constexpr int kVerletSpareRows = 8;
+// valid groups (64 lanes x 4 entries) kept after a Verlet list for the
+// passes' read-ahead
+constexpr int kVerletSpareGroups = 2;
 void launch_verlet_count(Sim &s);
 void launch_verlet_fill(Sim &s);
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
@@ -206,7 +209,9 @@ void launch_from_home(Sim &s, const double *src_home, double *dst_id);
 void launch_from_home_int(Sim &s, const int *src_home, int *dst_id);
 void launch_identity_homes(Sim &s);
 void launch_snapshot(Sim &s, float *out);
-void launch_rehome(Sim &s);
+// Re-homes the particle arrays to the last build's cell order; k_xy_live =
+// false leaves the x / y components of k_1..k_4 behind (dead between steps).
+void launch_rehome(Sim &s, bool k_xy_live = true);
 void launch_order_param(Sim &s, double *out2);
 void launch_init_state(Sim &s, double omega_std);
 void launch_noise(Sim &s, int per_particle, uint64_t step, double *out);

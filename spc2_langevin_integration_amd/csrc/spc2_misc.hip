@@ -130,13 +130,15 @@ struct PermuteSet {
   double *dst[kPermuteArrays];
 };
 
-__global__ __launch_bounds__(kBlock) void k_permute(int n, const int *slot_h, PermuteSet ps, const int *hid_in,
-                                                    int *hid_out, int *hpos) {
+// The first `count` arrays of the set move (the rest are dead at the call).
+__global__ __launch_bounds__(kBlock) void k_permute(int n, const int *slot_h, PermuteSet ps, int count,
+                                                    const int *hid_in, int *hid_out, int *hpos) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int h = slot_h[k];
 #pragma unroll
-  for (int a = 0; a < kPermuteArrays; ++a) ps.dst[a][k] = ps.src[a][h];
+  for (int a = 0; a < kPermuteArrays; ++a)
+    if (a < count) ps.dst[a][k] = ps.src[a][h];
   const int id = hid_in[h];
   hid_out[k] = id;
   hpos[id] = k;
@@ -198,18 +200,22 @@ void launch_identity_homes(Sim &s) {
 
 // Moves every persistent per-particle array to the cell order of the last
 // build (slot k becomes home k) and swaps in the permuted buffers.
-void launch_rehome(Sim &s) {
-  double **arrs[kPermuteArrays] = {&s.sx, &s.sy, &s.sp, &s.omega, &s.k[0][0], &s.k[0][1], &s.k[0][2],
-                                   &s.k[1][0], &s.k[1][1], &s.k[1][2], &s.k[2][0], &s.k[2][1], &s.k[2][2],
-                                   &s.k[3][0], &s.k[3][1], &s.k[3][2]};
+void launch_rehome(Sim &s, bool k_xy_live) {
+  // state, omega and the phi components of k_1..k_4 (in/out across steps: a
+  // particle without partners keeps its previous phi derivative), then the
+  // x / y components, which a step always writes before reading
+  double **arrs[kPermuteArrays] = {&s.sx, &s.sy, &s.sp, &s.omega, &s.k[0][2], &s.k[1][2], &s.k[2][2], &s.k[3][2],
+                                   &s.k[0][0], &s.k[0][1], &s.k[1][0], &s.k[1][1], &s.k[2][0], &s.k[2][1],
+                                   &s.k[3][0], &s.k[3][1]};
+  const int count = k_xy_live ? kPermuteArrays : 8;
   PermuteSet ps;
   for (int a = 0; a < kPermuteArrays; ++a) {
     ps.src[a] = *arrs[a];
     ps.dst[a] = s.spare[a];
   }
-  hipLaunchKernelGGL(k_permute, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.slot_h, ps, s.hid,
+  hipLaunchKernelGGL(k_permute, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.slot_h, ps, count, s.hid,
                      s.spare_i, s.hpos);
-  for (int a = 0; a < kPermuteArrays; ++a) std::swap(*arrs[a], s.spare[a]);
+  for (int a = 0; a < count; ++a) std::swap(*arrs[a], s.spare[a]);
   std::swap(s.hid, s.spare_i);
 }
 

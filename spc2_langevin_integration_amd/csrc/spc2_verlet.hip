@@ -39,62 +39,112 @@ constexpr int kBlock = 256;
 
 inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
+// List layout: entries of the 64 homes of wavefront w are stored in groups of
+// four per lane, group g of lane p at int4 (wrow[w] + g) * 64 + p, so that a
+// wavefront reads 1 KiB per group load and a lane takes four entries per load.
+// Entry = (j << 4) | code, code = (dcx + 1) | (dcy + 1) << 2 for the stencil
+// cell (dcx, dcy) of partner j.  Every lane's column runs to the wavefront's
+// group count, padded with a valid entry (the home itself, never summed: the
+// passes stop at the count), and kVerletSpareGroups valid groups follow the
+// list, so the passes may read one group ahead without bounds.
+
 // Visits the candidates of home i in a 3 x 3 stencil of build cells (cells of
 // side >= rlist), ascending home within each cell, in the reference's stencil
-// order; calls f(j, code) for every j != i within rlist (FP64, min image of
-// the wrapped build positions).
-template <class F>
+// order; calls f(j, code) for every j != i within rlist.  FP64 forms: min image
+// of the wrapped build positions.  FP32 form: the cell-local FP32 tiles with
+// the pass kernels' offset arithmetic, against rlist^2 (1 + 2^-13), a
+// superset of the FP64 test (the passes test every entry against the cutoffs
+// themselves, and a pair outside rlist at the build stays outside rho while
+// the list is used).
+template <int PREC, class F>
 __device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const int *start, const Tiles &t, int i,
                                                          double rlist2, F &&f) {
   const int c = t.cell[i];
   const int cx = c % P.nx, cy = c / P.nx;
-  const double xi = t.x[i], yi = t.y[i];
-  for (int dyo = -1; dyo <= 1; ++dyo) {
-    const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
-    for (int dxo = -1; dxo <= 1; ++dxo) {
-      const int jc = row + adjust_cell(cx + dxo, P.nx);
-      const int code = (dxo + 1) | ((dyo + 1) << 2);
-      for (int j = start[jc], e = start[jc + 1]; j < e; ++j) {
-        if (j == i) continue;
-        double dx = min_image(t.x[j] - xi, P.Lx, P.rLx);
-        double dy = min_image(t.y[j] - yi, P.Ly, P.rLy);
-        if (dx * dx + dy * dy <= rlist2) f(j, code);
+  if constexpr (PREC == P32K) {
+    const float4 vi = t.f4[i];
+    const float kx = -P.hxf - vi.x, ky = -P.hyf - vi.y;
+    const float rl2 = (float)(rlist2 * (1.0 + 0x1p-13));
+    for (int dyo = -1; dyo <= 1; ++dyo) {
+      const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
+      const float oy = __builtin_fmaf((float)(dyo + 1), P.hyf, ky);
+      for (int dxo = -1; dxo <= 1; ++dxo) {
+        const int jc = row + adjust_cell(cx + dxo, P.nx);
+        const int code = (dxo + 1) | ((dyo + 1) << 2);
+        const float ox = __builtin_fmaf((float)(dxo + 1), P.hxf, kx);
+        const int e = start[jc + 1];
+#pragma unroll 4
+        for (int j = start[jc]; j < e; ++j) {
+          const float4 v = t.f4[j];
+          const float dx = v.x + ox, dy = v.y + oy;
+          if ((__builtin_fmaf(dx, dx, dy * dy) <= rl2) & (j != i)) f(j, code);
+        }
+      }
+    }
+  } else {
+    const double xi = t.x[i], yi = t.y[i];
+    for (int dyo = -1; dyo <= 1; ++dyo) {
+      const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
+      for (int dxo = -1; dxo <= 1; ++dxo) {
+        const int jc = row + adjust_cell(cx + dxo, P.nx);
+        const int code = (dxo + 1) | ((dyo + 1) << 2);
+        const int e = start[jc + 1];
+#pragma unroll 2
+        for (int j = start[jc]; j < e; ++j) {
+          double dx = min_image(t.x[j] - xi, P.Lx, P.rLx);
+          double dy = min_image(t.y[j] - yi, P.Ly, P.rLy);
+          if ((dx * dx + dy * dy <= rlist2) & (j != i)) f(j, code);
+        }
       }
     }
   }
 }
 
-// Build pass 1: entries per home and the largest count of each wavefront.
+// Build pass 1: entries per home and each wavefront's group count (its
+// longest list in groups of four).
+template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *start, Tiles t, double rlist2,
-                                                         int *vcnt, int *wmax) {
+                                                         int *vcnt, int *wgroups) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
   if (i < P.n) {
-    for_each_build_neighbour(P, start, t, i, rlist2, [&](int, int) { ++cnt; });
+    for_each_build_neighbour<PREC>(P, start, t, i, rlist2, [&](int, int) { ++cnt; });
     vcnt[i] = cnt;
   }
   int m = cnt;
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
   const int w = i >> 6;
-  if ((threadIdx.x & 63) == 0 && w * 64 < P.n) wmax[w] = m;
+  if ((threadIdx.x & 63) == 0 && w * 64 < P.n) wgroups[w] = (m + 3) >> 2;
 }
 
-// Build pass 2: the entries, in the same order as the count, then the column
-// padded to the wavefront's longest list with a valid entry (itself, never
-// summed: the passes stop at the count).
+// Build pass 2: the entries, in the order of the count, four per store, then
+// the column padded to the wavefront's group count.
+template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *start, Tiles t, double rlist2,
                                                         const int *wrow, int *vlist) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  const int w = i >> 6, rows = wrow[w + 1] - wrow[w];
-  int *dst = vlist + (size_t)wrow[w] * 64 + (i & 63);
+  const int w = i >> 6, groups = wrow[w + 1] - wrow[w];
+  int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (i & 63);
+  const int self = (i << 4) | 5;
+  int4 buf = make_int4(self, self, self, self);
   int e = 0;
-  for_each_build_neighbour(P, start, t, i, rlist2, [&](int j, int code) {
-    dst[(size_t)e * 64] = (j << 4) | code;
+  for_each_build_neighbour<PREC>(P, start, t, i, rlist2, [&](int j, int code) {
+    const int ent = (j << 4) | code;
+    switch (e & 3) {
+      case 0: buf.x = ent; break;
+      case 1: buf.y = ent; break;
+      case 2: buf.z = ent; break;
+      default:
+        buf.w = ent;
+        dst[(size_t)(e >> 2) * 64] = buf;
+        buf = make_int4(self, self, self, self);
+    }
     ++e;
   });
-  for (; e < rows; ++e) dst[(size_t)e * 64] = (i << 4) | 5;
+  for (int g = (e + 3) >> 2; g < groups; ++g) dst[(size_t)g * 64] = make_int4(self, self, self, self);
+  if (e & 3) dst[(size_t)(e >> 2) * 64] = buf;
 }
 
 // Per-RHS stage: s + (k_coef dt) k by home, the FP64 tile (conditionally
@@ -156,44 +206,45 @@ __device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, con
   acc.pair(src, j, P);
 }
 
-// FP32 list walk with kWalk entries in flight: the next group's list entries
+// FP32 list walk, a group of four entries in flight: the next group's entries
 // and the current group's partner tiles (and pass-2 derivatives) are requested
-// before the group is summed, in list order, so one memory latency covers the
-// group.  Reads run up to 2 kWalk - 1 rows past a particle's last entry: the
-// build pads every column to its wavefront's longest list with a valid entry
-// and leaves kVerletSpareRows valid rows after the list.  An entry's upper bits
-// (j << 4) are the byte offset of tile j, so a gather is one mask from the
-// entry, addressed from the uniform tile base.
-constexpr int kWalk = 4;
-static_assert(2 * kWalk <= kVerletSpareRows, "list prefetch runs past the spare rows");
-
+// before the group is summed, in list order, so one memory latency covers four
+// entries.  An entry's upper bits (j << 4) are the byte offset of tile j, so a
+// gather is one mask from the entry, addressed from the uniform tile base.
 template <int PASS, class Acc>
-__device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int *l, int cnt, const KParams &P) {
+__device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P) {
   if (cnt <= 0) return;
   const char *f4b = (const char *)src.f4;
   const char *ajb = (const char *)src.aj;
-  int e[kWalk];
+  int4 g = *l;
+  for (int q = 0; q < cnt; q += 4) {
+    const int e[4] = {g.x, g.y, g.z, g.w};
+    float4 v[4];
+    float a[4];
 #pragma unroll
-  for (int u = 0; u < kWalk; ++u) e[u] = l[u * 64];
-  for (int q = 0; q < cnt; q += kWalk) {
-    float4 v[kWalk];
-    float a[kWalk];
-#pragma unroll
-    for (int u = 0; u < kWalk; ++u) {
+    for (int u = 0; u < 4; ++u) {
       const uint32_t off = (uint32_t)e[u] & ~15u;
       v[u] = *(const float4 *)(f4b + off);
       a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
     }
-    l += kWalk * 64;
-    int nx[kWalk];
+    l += 64;
+    g = *l;  // the next group (a spare group follows the list)
 #pragma unroll
-    for (int u = 0; u < kWalk; ++u) nx[u] = l[u * 64];
-#pragma unroll
-    for (int u = 0; u < kWalk; ++u) {
+    for (int u = 0; u < 4; ++u) {
       if (u == 0 || q + u < cnt) acc.pair_c(src, e[u] >> 4, e[u] & 15, v[u], a[u], P);
     }
-#pragma unroll
-    for (int u = 0; u < kWalk; ++u) e[u] = nx[u];
+  }
+}
+
+// FP64 forms: the entries of each group in turn.
+template <class Acc, class S>
+__device__ __forceinline__ void walk64(Acc &acc, const S &src, const int4 *l, int cnt, const KParams &P) {
+  for (int q = 0; q < cnt; q += 4) {
+    const int4 g = l[(size_t)(q >> 2) * 64];
+    visit_entry(acc, src, g.x, P);
+    if (q + 1 < cnt) visit_entry(acc, src, g.y, P);
+    if (q + 2 < cnt) visit_entry(acc, src, g.z, P);
+    if (q + 3 < cnt) visit_entry(acc, src, g.w, P);
   }
 }
 
@@ -226,13 +277,11 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet(KParams P, const int *_
     Acc acc;
     acc.init(src, i, i, t, P);
     cnt = vcnt[i];
-    const int *l = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
-    if constexpr (PREC == P32K) {
+    const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+    if constexpr (PREC == P32K)
       walk32<1>(acc, src, l, cnt, P);
-    } else {
-#pragma unroll 2
-      for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
-    }
+    else
+      walk64(acc, src, l, cnt, P);
     acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
     const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
     if (EPI == EPI_K && e.next == 1) {
@@ -262,13 +311,11 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   Acc acc;
   acc.init(src, i, i, t, P, in);
   const int cnt = vcnt[i];
-  const int *l = vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
-  if constexpr (PREC == P32K) {
+  const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
+  if constexpr (PREC == P32K)
     walk32<2>(acc, src, l, cnt, P);
-  } else {
-#pragma unroll 2
-    for (int q = 0; q < cnt; ++q) visit_entry(acc, src, l[(size_t)q * 64], P);
-  }
+  else
+    walk64(acc, src, l, cnt, P);
   Pass2Out o;
   acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
   epilogue2<EPI>(P, e, i, hid[i], o);
@@ -367,14 +414,37 @@ void vpass2_sys(Sim &s, int epi, const EpiCtx &e) {
 
 void launch_verlet_count(Sim &s) {
   const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
-  hipLaunchKernelGGL(k_verlet_count, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
-                     s.t, r2, s.vcnt, s.wmax);
+  const dim3 g(blocks_for(s.n)), b(kBlock);
+  switch (s.prec) {
+    case P64:
+      hipLaunchKernelGGL(k_verlet_count<P64>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vcnt, s.wmax);
+      break;
+    case P64F:
+      hipLaunchKernelGGL(k_verlet_count<P64F>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vcnt,
+                         s.wmax);
+      break;
+    default:
+      hipLaunchKernelGGL(k_verlet_count<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vcnt,
+                         s.wmax);
+  }
 }
 
 void launch_verlet_fill(Sim &s) {
   const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
-  hipLaunchKernelGGL(k_verlet_fill, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
-                     s.t, r2, (const int *)s.wrow, s.vlist);
+  const dim3 g(blocks_for(s.n)), b(kBlock);
+  switch (s.prec) {
+    case P64:
+      hipLaunchKernelGGL(k_verlet_fill<P64>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2,
+                         (const int *)s.wrow, s.vlist);
+      break;
+    case P64F:
+      hipLaunchKernelGGL(k_verlet_fill<P64F>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2,
+                         (const int *)s.wrow, s.vlist);
+      break;
+    default:
+      hipLaunchKernelGGL(k_verlet_fill<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2,
+                         (const int *)s.wrow, s.vlist);
+  }
 }
 
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt) {
