@@ -61,6 +61,13 @@ struct Kahan {
     c = (t - s) - y;
     s = t;
   }
+  // add(a * b) with the product's rounding folded into the compensated term
+  __device__ __forceinline__ void add_prod(float a, float b) {
+    float y = __builtin_fmaf(a, b, -c);
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
   __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 

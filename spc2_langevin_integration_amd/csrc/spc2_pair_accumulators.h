@@ -416,14 +416,14 @@ struct VortAcc<P32K> {
     float w_an = P.c4f * (r - P.xaf) * (P.rhof - r);
     float w = r2 <= P.xa2f ? w_in : w_an;
     w = inr ? w : 0.0f;
-    SW.add(w * v.w);
-    CW.add(w * v.z);
+    SW.add_prod(w, v.w);
+    CW.add_prod(w, v.z);
     n01 += inr;
     if (inc) {
       if (r2 >= P.close2f) {
         float ir = __builtin_amdgcn_rsqf(r2);
-        Tx.add(dx * ir);
-        Ty.add(dy * ir);
+        Tx.add_prod(dx, ir);
+        Ty.add_prod(dy, ir);
       } else {
         double ddx, ddy;
         exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
@@ -816,7 +816,7 @@ struct Vort2Acc<P32K> {
       // W'(r) / r
       float g = inner ? -2.0f * P.mu_p_over_xa2f : P.c4f * (P.rhof + P.xaf - 2.0f * r) * ir;
       float aj = have_aj ? ajv : s.aj[m];
-      A0.add(w * cph);
+      A0.add_prod(w, cph);
       A1.add((dvx * dx + dvy * dy) * g * sph + (aj - aif) * w * cph - 2.0f * P.Df * sph * w);
     }
     if (inc) {
