@@ -27,10 +27,11 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32k": 157.3, "fp64": 78.6, "fp64f": 78.6}  # TFLOP/s, MI355X vector FP32 / FP64
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_pass1_verlet<VortAcc<P32K>> on this workload, from the
-# FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) counter passes calibrated against a
-# 512 MiB device copy; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 346.6e6
+# HBM bytes per launch of k_pass1_verlet<VortAcc<P32K>> on this workload (skin
+# 0.1 rho), from the FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) counter passes
+# calibrated against a 512 MiB device copy, stages 1-3 and 4 weighted 3:1;
+# profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
+TRAFFIC_VERLET_FP32K = 366.7e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -107,10 +108,14 @@ def profile_pair_kernel(spc2, w, precision, steps, neighbour):
     # 24 B) + FP64 stage phase 8 B + home->id and counts 8 B + k written 24 B;
     # the last RK4 stage also reads k1..k3 (72 B) and reads and writes the state
     # (48 B), a quarter of the launches; Verlet: 4 B per list entry (neighbour
-    # tiles are re-read from cache, not HBM)
+    # tiles are re-read from cache, not HBM), the phi derivative read back (in/out,
+    # 8 B) and the next stage's tiles written by the epilogue (FP32: tile 16 B +
+    # FP64 position 16 B + phase 8 B; FP64: 24 B + phase 8 B) from the state
+    # (24 B, stages 1-3)
     own = 16 if precision == "fp32k" else 24
     list_b = 4.0 * per[0] if neighbour == "verlet" else 0.0
-    bytes_per = w["n"] * (own + 8 + 8 + 24 + (72 + 48) / 4.0) + list_b
+    fused = (8 + (40 if precision == "fp32k" else 32) + 24 * 3 / 4.0) if neighbour == "verlet" else 0.0
+    bytes_per = w["n"] * (own + 8 + 8 + 24 + (72 + 48) / 4.0 + fused) + list_b
     return {"avg_ms": ms / launches, "launches": launches, "flops_per_launch": flops,
             "tflops": flops / avg_s / 1e12, "bytes_per_launch": bytes_per, "gbs": bytes_per / avg_s / 1e9,
             "candidates_per_particle": per[0] / w["n"], "neighbours_per_particle": per[1] / w["n"],
