@@ -114,7 +114,8 @@ def profile_pair_kernel(spc2, w, precision, steps, neighbour):
     # (24 B, stages 1-3)
     own = 16 if precision == "fp32k" else 24
     list_b = 4.0 * per[0] if neighbour == "verlet" else 0.0
-    fused = (8 + (40 if precision == "fp32k" else 32) + 24 * 3 / 4.0) if neighbour == "verlet" else 0.0
+    fused = ((8 + (40 if precision == "fp32k" else 32) + 24 * 3 / 4.0)
+             if neighbour == "verlet" and w["method"] == "rk4" else 0.0)
     bytes_per = w["n"] * (own + 8 + 8 + 24 + (72 + 48) / 4.0 + fused) + list_b
     return {"avg_ms": ms / launches, "launches": launches, "flops_per_launch": flops,
             "tflops": flops / avg_s / 1e12, "bytes_per_launch": bytes_per, "gbs": bytes_per / avg_s / 1e9,
@@ -213,8 +214,8 @@ def main():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precisions", default="fp32k,fp64,fp64f",
                     help="forms to time; the first is the headline value")
-    ap.add_argument("--neighbour", default=None, choices=["verlet", "cells"],
-                    help="default: verlet for vortex-rk4, cells (the reference's) for chimera-srk")
+    ap.add_argument("--neighbour", default="verlet", choices=["verlet", "cells"],
+                    help="verlet (default) or cells: the reference's linked cells rebuilt at every RHS")
     ap.add_argument("--workload", default="vortex-rk4", choices=["vortex-rk4", "chimera-srk"],
                     help="vortex-rk4: the headline (N=1M); chimera-srk: chimera SRK at the given --n")
     ap.add_argument("--pmc-run", action="store_true",
@@ -233,8 +234,6 @@ def main():
     import spc2_langevin_integration_amd as spc2
     w = workload(args.n, args.workload)
     w["skin"] = args.skin
-    if args.neighbour is None:
-        args.neighbour = "verlet" if w["system"] == "vortex" else "cells"
     if args.pmc_run:
         pmc_run(spc2, w, args.neighbour)
         return
