@@ -183,6 +183,14 @@ spc2_status spc2_get_profile(spc2_sim_t h, double *pair_kernel_ms, int64_t *laun
  * (SPC2_ERR_INVALID unless the handle uses SPC2_NEIGH_VERLET). */
 spc2_status spc2_verlet_stats(spc2_sim_t h, int64_t *builds, int64_t *entries);
 
+/* Layout of the current Verlet list: lds_staged = 1 when the FP32 passes stage
+ * each workgroup's halo of tiles in LDS (capacity lds_capacity particles) and
+ * the entries index that halo, 0 for the gather passes.  No counterpart in the
+ * reference (it has no neighbour list); for tests and tuning.  Setting the
+ * environment variable SPC2_VERLET_LDS=0 before spc2_create keeps the gather
+ * passes. */
+spc2_status spc2_verlet_layout(spc2_sim_t h, int32_t *lds_staged, int32_t *lds_capacity);
+
 spc2_status spc2_synchronize(spc2_sim_t h);
 
 /* Free and total bytes of the current device (admission budgets). */

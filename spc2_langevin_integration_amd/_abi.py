@@ -81,6 +81,7 @@ def load():
         "spc2_get_profile": (st, [h, dp, C.POINTER(C.c_int64)]),
         "spc2_synchronize": (st, [h]),
         "spc2_verlet_stats": (st, [h, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "spc2_verlet_layout": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "spc2_batch_footprint": (C.c_uint64, [C.POINTER(Params)]),
         "spc2_batch_create": (st, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
         "spc2_batch_create_from": (st, [C.POINTER(Params), C.c_int32, h, C.POINTER(C.c_int32), C.c_double,
@@ -271,6 +272,12 @@ class Sim:
         b, e = C.c_int64(), C.c_int64()
         _check(load().spc2_verlet_stats(self.h, C.byref(b), C.byref(e)))
         return b.value, e.value
+
+    def verlet_layout(self):
+        """(1 if the current list feeds the LDS-staged passes else 0, their LDS capacity)."""
+        lds, cap = C.c_int32(), C.c_int32()
+        _check(load().spc2_verlet_layout(self.h, C.byref(lds), C.byref(cap)))
+        return lds.value, cap.value
 
 
 class Batch:

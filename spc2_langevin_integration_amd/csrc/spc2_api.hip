@@ -69,6 +69,7 @@ void free_all(Sim &s) {
   f(s.aphi_slot); f(s.n_slot); f(s.n2_slot); f(s.aphi_slot_f);
   f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
   f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb);
+  f(s.bh_nrun); f(s.bh_run); f(s.bh_off); f(s.bh_max);
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
 
@@ -181,6 +182,11 @@ spc2_status alloc_all(Sim &s) {
   A(s.bsum, (size_t)scan_blocks(std::max({s.nc, s.nw, 1})) + 1);
   if (s.verlet) {
     A(s.vcnt, n); A(s.wmax, (size_t)s.nw + 1); A(s.wrow, (size_t)s.nw + 1); A(s.xb, n); A(s.yb, n);
+    if (s.prec == P32K) {
+      s.nblk = (s.n + 255) / 256;
+      A(s.bh_nrun, (size_t)s.nblk); A(s.bh_run, (size_t)s.nblk * 2 * kVerletRunMax);
+      A(s.bh_off, (size_t)s.nblk * (kVerletRunMax + 1)); A(s.bh_max, 1);
+    }
   }
 #undef A
   HIP_TRY(hipMemset(s.count, 0, sizeof(int) * nc));
@@ -237,9 +243,19 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   launch_verlet_count(s);
   launch_scan_array(s, s.wmax, s.nw, s.wrow);
   if (spc2_status st = check_launch("verlet count")) return st;
-  int groups = 0;
+  // the LDS layout (FP32 form): every pass workgroup's halo, and the largest
+  const bool try_lds = s.vlds_allowed && s.bh_max;
+  if (try_lds) {
+    HIP_TRY(hipMemsetAsync(s.bh_max, 0, sizeof(int), s.stream));
+    launch_block_halo(s);
+  }
+  int groups = 0, halo = 0;
   HIP_TRY(hipMemcpyAsync(&groups, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  if (try_lds) HIP_TRY(hipMemcpyAsync(&halo, s.bh_max, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
+  const int cap_limit = kLdsBytes / 24;  // pass 2 stages 24 B per halo particle
+  s.vlds = try_lds && halo >= 0 && halo <= cap_limit;
+  s.vlds_cap = s.vlds ? std::min(cap_limit, std::max(64, (halo + 63) / 64 * 64)) : 0;
   // groups = sum over wavefronts of their longest list in groups of four; more
   // than 4096 entries per particle means a skin or density the list is not
   // meant for
@@ -255,7 +271,10 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
     HIP_TRY(dalloc(&s.vlist, cap));
     s.vlist_cap = cap;
   }
-  launch_verlet_fill(s);
+  if (s.vlds)
+    launch_verlet_fill_lds(s);
+  else
+    launch_verlet_fill(s);
   HIP_TRY(hipMemsetAsync(s.vlist + (size_t)groups * 256, 0, sizeof(int) * 256 * kVerletSpareGroups, s.stream));
   if (spc2_status st = check_launch("verlet build")) return st;
   s.vbuilt = true;
@@ -468,6 +487,7 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
   fill_kparams(s);
   configure_segments(s);
   if (const char *env = std::getenv("SPC2_REHOME_EVERY")) s.rehome_every = std::max(1, std::atoi(env));
+  if (const char *env = std::getenv("SPC2_VERLET_LDS")) s.vlds_allowed = std::atoi(env) != 0;
   hipError_t e = hipGetDevice(&s.device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -733,6 +753,15 @@ spc2_status spc2_verlet_stats(spc2_sim_t h, int64_t *builds, int64_t *entries) {
   }
   *builds = s.vbuilds;
   *entries = s.vbuilt ? s.ventries : 0;
+  return SPC2_OK;
+}
+
+spc2_status spc2_verlet_layout(spc2_sim_t h, int32_t *lds_staged, int32_t *lds_capacity) {
+  if (check_handle(h) || !lds_staged || !lds_capacity) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  if (!s.verlet) return fail(SPC2_ERR_INVALID, "the handle does not use Verlet lists");
+  *lds_staged = s.vbuilt && s.vlds ? 1 : 0;
+  *lds_capacity = s.vbuilt && s.vlds ? s.vlds_cap : 0;
   return SPC2_OK;
 }
 

@@ -126,6 +126,13 @@ struct Sim {
   // wmax[w] / wrow[w]: wavefront w's group count and first group (see the list
   // layout in spc2_verlet.hip); vlist_cap in ints
   int *vcnt = nullptr, *wmax = nullptr, *wrow = nullptr, *vlist = nullptr;
+  // LDS-staged FP32 passes: per pass workgroup (kBlock homes) its halo as runs of
+  // consecutive build cells (bh_run: first and last cell, bh_off: LDS offsets,
+  // bh_nrun: run count or -1), bh_max: the largest halo; vlds when the current
+  // list holds halo indices
+  bool vlds = false, vlds_allowed = true;
+  int nblk = 0, vlds_cap = 0;
+  int *bh_nrun = nullptr, *bh_run = nullptr, *bh_off = nullptr, *bh_max = nullptr;
   size_t vlist_cap = 0;
   double *xb = nullptr, *yb = nullptr;  // build positions by home
   long long vbuilds = 0, ventries = 0;
@@ -186,6 +193,7 @@ void launch_place(Sim &s);
 void launch_rank_tiles(Sim &s);
 // spc2_pairs.hip
 constexpr double kSegTarget = 218.0;  // default particles per row segment
+constexpr int kLdsBytes = 60 * 1024;  // dynamic LDS of one pair workgroup at most
 int lds_elem_bytes(int prec);
 void configure_segments_kp(KParams &k, int prec, double target);
 void configure_segments(Sim &s);
@@ -198,6 +206,9 @@ void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
 constexpr int kVerletSpareGroups = 2;
 void launch_verlet_count(Sim &s);
 void launch_verlet_fill(Sim &s);
+constexpr int kVerletRunMax = 16;  // halo runs per pass workgroup (kRunMax of spc2_verlet.hip)
+void launch_block_halo(Sim &s);
+void launch_verlet_fill_lds(Sim &s);
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
 void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa,
                          double *const k1[3], double *const k2[3], double *const k3[3]);

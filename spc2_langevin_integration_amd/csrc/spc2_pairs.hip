@@ -224,7 +224,7 @@ void configure_segments_kp(KParams &k, int prec, double target) {
   k.segs = (k.nx + w - 1) / w;
   const double halo = 3.0 * (w + 2) * per_cell;
   int cap = (int)std::ceil((1.6 * halo + 160.0) / 64.0) * 64;
-  cap = std::min(cap, (60 * 1024) / lds_elem_bytes(prec));
+  cap = std::min(cap, kLdsBytes / lds_elem_bytes(prec));
   if (const char *env = std::getenv("SPC2_LDS_CAP")) cap = std::max(0, std::min(cap, std::atoi(env)));
   k.lds_cap = cap;
 }

@@ -147,6 +147,152 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *st
   if (e & 3) dst[(size_t)(e >> 2) * 64] = buf;
 }
 
+// ---- LDS-staged passes (FP32 form) -----------------------------------------------
+// A pass workgroup covers kBlock consecutive homes: a run of consecutive build
+// cells in one grid row, or the end of one row and the start of the next.  Its
+// halo is the union of those cells' 3 x 3 stencils.  Since the homes of
+// consecutive cells are contiguous, the halo is a few runs of consecutive cells
+// (one per halo row and column interval), each one contiguous run of tiles.  At a
+// build every workgroup's runs and their LDS offsets are recorded, and list
+// entries hold the partner's index in its workgroup's staged halo instead of its
+// home.  The passes stage the halo tiles in LDS with coalesced reads and gather
+// from LDS.  A build where a workgroup spans more than two cell rows, or whose
+// largest halo does not fit the LDS capacity, keeps the gather layout.
+constexpr int kRunMax = kVerletRunMax;
+
+// Runs of halo cells of workgroup b in increasing cell order: cells
+// [run0[r], run1[r]], LDS offset roff[r] (roff[nrun] = halo size).  nrun = -1:
+// the workgroup spans more than two rows (or needs more than kRunMax runs).
+__global__ __launch_bounds__(64) void k_block_halo(KParams P, const int *start, const int *cell, int nblk,
+                                                   int *bh_nrun, int *bh_run, int *bh_off, int *bh_max) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const int nx = P.nx, ny = P.ny;
+  const int c0 = cell[b * kBlock], c1 = cell[min(P.n, (b + 1) * kBlock) - 1];
+  const int y0 = c0 / nx, y1 = c1 / nx;
+  int *run = bh_run + (size_t)b * 2 * kRunMax;  // run[2r], run[2r + 1]: first and last cell
+  int nrun = 0;
+  bool ok = y1 - y0 <= 1;
+  // column intervals of halo row r: the workgroup's intervals in rows r - 1 .. r + 1
+  // widened by one column (and wrapped), emitted as runs in increasing column order
+  for (int r = y0 - 1; ok && r <= y1 + 1; ++r) {
+    int a[2], e[2], m = 0;
+    for (int yb = y0; yb <= y1; ++yb) {
+      if (yb < r - 1 || yb > r + 1) continue;
+      a[m] = (yb == y0 ? c0 % nx : 0) - 1;
+      e[m] = (yb == y1 ? c1 % nx : nx - 1) + 1;
+      ++m;
+    }
+    if (m == 2 && a[1] <= e[0] + 1 && a[0] <= e[1] + 1) {
+      a[0] = min(a[0], a[1]);
+      e[0] = max(e[0], e[1]);
+      m = 1;
+    }
+    const int row = adjust_cell(r, ny) * nx;
+    for (int q = 0; q < m; ++q) {
+      int lo = a[q], hi = e[q];
+      if (hi - lo + 1 >= nx) {
+        lo = 0;
+        hi = nx - 1;
+      }
+      // [lo, hi] within [-1, nx]: at most three pieces after wrapping
+      const int plo[3] = {lo < 0 ? nx - 1 : -1, max(lo, 0), hi >= nx ? 0 : -1};
+      const int phi[3] = {lo < 0 ? nx - 1 : -2, min(hi, nx - 1), hi >= nx ? 0 : -2};
+      for (int k = 0; k < 3; ++k) {
+        if (phi[k] < plo[k]) continue;
+        if (nrun == kRunMax) {
+          ok = false;
+          break;
+        }
+        run[2 * nrun] = row + plo[k];
+        run[2 * nrun + 1] = row + phi[k];
+        ++nrun;
+      }
+    }
+  }
+  if (!ok) {
+    bh_nrun[b] = -1;
+    atomicMax(bh_max, INT_MAX);
+    return;
+  }
+  // sort the runs by first cell, then merge overlapping or adjacent ones (wrapped
+  // rows and columns may repeat cells)
+  for (int k = 1; k < nrun; ++k) {
+    const int f = run[2 * k], l = run[2 * k + 1];
+    int j = k - 1;
+    while (j >= 0 && run[2 * j] > f) {
+      run[2 * j + 2] = run[2 * j];
+      run[2 * j + 3] = run[2 * j + 1];
+      --j;
+    }
+    run[2 * j + 2] = f;
+    run[2 * j + 3] = l;
+  }
+  int u = 0;
+  for (int k = 0; k < nrun; ++k) {
+    if (u > 0 && run[2 * k] <= run[2 * u - 1] + 1) {
+      run[2 * u - 1] = max(run[2 * u - 1], run[2 * k + 1]);
+    } else {
+      run[2 * u] = run[2 * k];
+      run[2 * u + 1] = run[2 * k + 1];
+      ++u;
+    }
+  }
+  int *off = bh_off + (size_t)b * (kRunMax + 1);
+  int tot = 0;
+  for (int k = 0; k < u; ++k) {
+    off[k] = tot;
+    tot += start[run[2 * k + 1] + 1] - start[run[2 * k]];
+  }
+  off[u] = tot;
+  bh_nrun[b] = u;
+  atomicMax(bh_max, tot);
+}
+
+// LDS index base of build cell c in workgroup b's halo: index = base + home.
+__device__ __forceinline__ int halo_base(const int *run, const int *off, int nrun, const int *start, int c) {
+  int k = 0;
+  while (k + 1 < nrun && run[2 * k + 2] <= c) ++k;
+  return off[k] - start[run[2 * k]];
+}
+
+// Build pass 2 for the LDS layout (FP32 form): as k_verlet_fill, with the
+// partner's index in the workgroup's staged halo in place of its home.
+__global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int *start, Tiles t, double rlist2,
+                                                            const int *wrow, const int *bh_nrun, const int *bh_run,
+                                                            const int *bh_off, int *vlist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int b = i / kBlock, nrun = bh_nrun[b];
+  const int *run = bh_run + (size_t)b * 2 * kRunMax, *off = bh_off + (size_t)b * (kRunMax + 1);
+  const int w = i >> 6, groups = wrow[w + 1] - wrow[w];
+  int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (i & 63);
+  const int c = t.cell[i], cx = c % P.nx, cy = c / P.nx;
+  const int self = ((halo_base(run, off, nrun, start, c) + i) << 4) | 5;
+  int4 buf = make_int4(self, self, self, self);
+  int e = 0, last = -1, base = 0;
+  for_each_build_neighbour<P32K>(P, start, t, i, rlist2, [&](int j, int code) {
+    if (code != last) {
+      const int jc = adjust_cell(cy + (code >> 2) - 1, P.ny) * P.nx + adjust_cell(cx + (code & 3) - 1, P.nx);
+      base = halo_base(run, off, nrun, start, jc);
+      last = code;
+    }
+    const int ent = ((base + j) << 4) | code;
+    switch (e & 3) {
+      case 0: buf.x = ent; break;
+      case 1: buf.y = ent; break;
+      case 2: buf.z = ent; break;
+      default:
+        buf.w = ent;
+        dst[(size_t)(e >> 2) * 64] = buf;
+        buf = make_int4(self, self, self, self);
+    }
+    ++e;
+  });
+  for (int g = (e + 3) >> 2; g < groups; ++g) dst[(size_t)g * 64] = make_int4(self, self, self, self);
+  if (e & 3) dst[(size_t)(e >> 2) * 64] = buf;
+}
+
 // Per-RHS stage: s + (k_coef dt) k by home, the FP64 tile (conditionally
 // wrapped, ChimeraSystem.cpp:291-309) and the precision's own tile.  FP32
 // coordinates are relative to the particle's build cell, taken at the nearest
@@ -321,6 +467,93 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   epilogue2<EPI>(P, e, i, hid[i], o);
 }
 
+// Stages workgroup seg's halo: FP32 tiles, their homes (for the FP64 re-checks)
+// and, for pass 2, the pass-1 phi derivatives.  Returns the LDS sources.
+template <int PASS>
+__device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, const int *bh_nrun,
+                                            const int *bh_run, const int *bh_off, int seg, const Tiles &t,
+                                            const EpiCtx &e, void *dyn, int cap) {
+  __shared__ int s_gst[kRunMax], s_off[kRunMax + 1], s_n;
+  if (threadIdx.x == 0) s_n = bh_nrun[seg];
+  if (threadIdx.x <= kRunMax) {
+    const int k = threadIdx.x, nrun = bh_nrun[seg];
+    if (k <= nrun) s_off[k] = bh_off[(size_t)seg * (kRunMax + 1) + k];
+    if (k < nrun) s_gst[k] = start[bh_run[(size_t)seg * 2 * kRunMax + 2 * k]];
+  }
+  __syncthreads();
+  const int nrun = s_n, total = s_off[nrun];
+  float4 *f4l = (float4 *)dyn;
+  int *slot = (int *)(f4l + cap);
+  float *ajl = (float *)(slot + cap);
+  for (int el = threadIdx.x; el < total; el += kBlock) {
+    int k = 0;
+    while (k + 1 < nrun && s_off[k + 1] <= el) ++k;
+    const int g = s_gst[k] + (el - s_off[k]);
+    f4l[el] = t.f4[g];
+    slot[el] = g;
+    if (PASS == 2) ajl[el] = e.aphi_slot_f[g];
+  }
+  __syncthreads();
+  return Src32{f4l, slot, PASS == 2 ? ajl : nullptr};
+}
+
+template <class Acc, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const int *__restrict__ vcnt,
+                                                             const int *__restrict__ wrow,
+                                                             const int *__restrict__ vlist, const int *start,
+                                                             const int *bh_nrun, const int *bh_run,
+                                                             const int *bh_off, Tiles t, const int *__restrict__ hid,
+                                                             const double *gp, EpiCtx e,
+                                                             unsigned long long *counts) {
+  extern __shared__ float4 dyn_lds[];
+  const int seg = xcd_segment(blockIdx.x, gridDim.x);
+  const Src32 src = stage_halo<1>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap);
+  const int i = seg * kBlock + threadIdx.x;
+  const bool active = i < P.n;
+  Pass1Out o{};
+  int cnt = 0;
+  if (active) {
+    Acc acc;
+    acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P);
+    cnt = vcnt[i];
+    walk32<1>(acc, src, (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63), cnt, P);
+    acc.finish(P, gp[i], o);
+    const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
+    if (EPI == EPI_K && e.next == 1) {
+      write_stage<P32K>(P, e.tn, e.gpn, i, e.sx[i] + e.kcdt_n * r.ax, e.sy[i] + e.kcdt_n * r.ay,
+                        e.sp[i] + e.kcdt_n * r.ap);
+    } else if ((EPI == EPI_RK4_FINAL || EPI == EPI_SE) && e.next == 2) {
+      write_stage<P32K>(P, e.tn, e.gpn, i, r.x, r.y, r.p);
+    }
+  }
+  if (counts) count_add(counts, cnt, active ? o.n : 0, active ? o.n2 : 0);
+}
+
+template <class Acc, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const int *__restrict__ vcnt,
+                                                             const int *__restrict__ wrow,
+                                                             const int *__restrict__ vlist, const int *start,
+                                                             const int *bh_nrun, const int *bh_run,
+                                                             const int *bh_off, Tiles t, const int *__restrict__ hid,
+                                                             const double *gp, EpiCtx e) {
+  extern __shared__ float4 dyn_lds[];
+  const int seg = xcd_segment(blockIdx.x, gridDim.x);
+  const Src32 src = stage_halo<2>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap);
+  const int i = seg * kBlock + threadIdx.x;
+  if (i >= P.n) return;
+  Pass2In in{e.aphi_slot[i], e.n_slot[i], e.n2_slot[i], 0.0, 0.0};
+  if (EPI == EPI_ITO2) {
+    in.in0 = e.a0[i];
+    in.in1 = e.a1[i];
+  }
+  Acc acc;
+  acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P, in);
+  walk32<2>(acc, src, (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63), vcnt[i], P);
+  Pass2Out o;
+  acc.finish(P, gp[i], o);
+  epilogue2<EPI>(P, e, i, hid[i], o);
+}
+
 // Largest squared displacement since the build (min image), two-level and
 // without atomics: per-workgroup maxima, then one workgroup.
 __global__ __launch_bounds__(kBlock) void k_drift_partial(KParams P, const double *sx, const double *sy,
@@ -357,6 +590,24 @@ template <int PREC, int SYS, int EPI>
 void vpass1_typed(Sim &s, const EpiCtx &e) {
   unsigned long long *cnt = s.counting ? s.counts : nullptr;
   const dim3 grid(blocks_for(s.n)), blk(kBlock);
+  if constexpr (PREC == P32K) {
+    if (s.vlds) {
+      KParams kp = s.kp;
+      kp.lds_cap = s.vlds_cap;
+      const size_t sh = (size_t)s.vlds_cap * 20;
+      if (SYS == SPC2_CHIMERA)
+        launch_pair(s, k_pass1_verlet_lds<ChimAcc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
+                    e, cnt);
+      else
+        launch_pair(s, k_pass1_verlet_lds<VortAcc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
+                    e, cnt);
+      return;
+    }
+  }
   if (SYS == SPC2_CHIMERA)
     launch_pair(s, k_pass1_verlet<ChimAcc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
                 (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid,
@@ -388,6 +639,24 @@ void vpass1_sys(Sim &s, int epi, const EpiCtx &e) {
 template <int PREC, int SYS, int EPI>
 void vpass2_typed(Sim &s, const EpiCtx &e) {
   const dim3 grid(blocks_for(s.n)), blk(kBlock);
+  if constexpr (PREC == P32K) {
+    if (s.vlds) {
+      KParams kp = s.kp;
+      kp.lds_cap = s.vlds_cap;
+      const size_t sh = (size_t)s.vlds_cap * 24;
+      if (SYS == SPC2_CHIMERA)
+        launch_pair(s, k_pass2_verlet_lds<Chim2Acc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
+                    e);
+      else
+        launch_pair(s, k_pass2_verlet_lds<Vort2Acc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
+                    e);
+      return;
+    }
+  }
   if (SYS == SPC2_CHIMERA)
     launch_pair(s, k_pass2_verlet<Chim2Acc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
                 (const int *)s.wrow, (const int *)s.vlist, s.t, (const int *)s.hid,
@@ -445,6 +714,18 @@ void launch_verlet_fill(Sim &s) {
       hipLaunchKernelGGL(k_verlet_fill<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2,
                          (const int *)s.wrow, s.vlist);
   }
+}
+
+void launch_block_halo(Sim &s) {
+  hipLaunchKernelGGL(k_block_halo, dim3(blocks_for(s.nblk, 64)), dim3(64), 0, s.stream, s.kp, (const int *)s.start,
+                     (const int *)s.t.cell, s.nblk, s.bh_nrun, s.bh_run, s.bh_off, s.bh_max);
+}
+
+void launch_verlet_fill_lds(Sim &s) {
+  const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
+  hipLaunchKernelGGL(k_verlet_fill_lds, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
+                     s.t, r2, (const int *)s.wrow, (const int *)s.bh_nrun, (const int *)s.bh_run,
+                     (const int *)s.bh_off, s.vlist);
 }
 
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt) {

@@ -204,3 +204,91 @@ def test_verlet_fused_stage_tiles_equal_separate_stage_pass(gpu, prec, method):
         b, bb = s.get_state(), s.verlet_stats()[0]
     assert ba == bb and ba >= 3
     assert np.array_equal(a, b), np.abs(a - b).max()
+
+
+def run_layout(params, st, method, dt, steps, monkeypatch, lds, omega=None):
+    """run() with the FP32 pass kernels chosen: lds=False keeps the gather
+    passes (SPC2_VERLET_LDS=0 at creation)."""
+    with monkeypatch.context() as m:
+        if lds:
+            m.delenv("SPC2_VERLET_LDS", raising=False)
+        else:
+            m.setenv("SPC2_VERLET_LDS", "0")
+        with spc2.Sim(params) as s:
+            s.set_state(st)
+            if omega is not None:
+                s.set_omega(omega)
+            s.set_counting(True)
+            s.step(method, dt, 0, 3)
+            layout = s.verlet_layout()
+            s.step(method, dt, 3, steps - 3)
+            return s.get_state(), s.counts(), s.verlet_stats(), layout
+
+
+@pytest.mark.parametrize("system,method", [("chimera", "srk"), ("vortex", "rk4"), ("vortex", "se")])
+def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method):
+    """The LDS-staged FP32 passes gather the same tiles from each workgroup's
+    staged halo, with the same arithmetic and order as the gather passes:
+    trajectories across several rebuilds are bit-identical."""
+    n, dt, steps = 30000, 0.01, 24
+    L = math.sqrt(n / 4000.0)
+    if system == "chimera":
+        rho = 0.1
+        kw = dict(n=n, L=L, rho=rho, v0=0.2 * rho / 8 / dt, sigma=1.0, alpha=1.45, D=0.02, seed=5)
+        om = O.uniform(81, n, -0.5, 0.5)
+    else:
+        rho = 0.05
+        kw = dict(n=n, L=L, rho=rho, D=0.01 if method == "se" else 0.0, seed=6, verlet_skin=0.004, **VORT)
+        om = None
+    st = O.make_state(9, 0 if system == "chimera" else 1, n, L, rho)
+    p = spc2.params(system, "fp32k", "verlet", **kw)
+    a, ca, sa, la = run_layout(p, st, method, dt, steps, monkeypatch, True, om)
+    b, cb, sb, lb = run_layout(p, st, method, dt, steps, monkeypatch, False, om)
+    assert la[0] == 1 and la[1] > 0, la
+    assert lb == (0, 0), lb
+    assert sa == sb and sa[0] >= 2
+    assert np.array_equal(ca, cb), (ca, cb)
+    assert np.array_equal(a, b), np.abs(a - b).max()
+
+
+def test_verlet_lds_vortex_ito_equals_gather(gpu, monkeypatch):
+    """Vortex pass 2 (Ito form) from the staged halo equals the gather pass."""
+    n, rho, D = 40000, 0.05, 0.02
+    L = math.sqrt(n / 4000.0)
+    st = O.make_state(23, 1, n, L, rho)
+    p = spc2.params("vortex", "fp32k", "verlet", n=n, L=L, rho=rho, D=D, **VORT)
+    out = {}
+    for lds in (True, False):
+        with monkeypatch.context() as m:
+            if lds:
+                m.delenv("SPC2_VERLET_LDS", raising=False)
+            else:
+                m.setenv("SPC2_VERLET_LDS", "0")
+            with spc2.Sim(p) as s:
+                s.set_state(st)
+                out[lds] = s.eval_rhs_ito() + (s.verlet_layout(),)
+    assert out[True][2][0] == 1 and out[False][2] == (0, 0)
+    assert np.array_equal(out[True][0], out[False][0]) and np.array_equal(out[True][1], out[False][1])
+
+
+def test_verlet_lds_clustered_state_keeps_gather(gpu):
+    """A dense cluster whose workgroup halo exceeds the LDS capacity keeps the
+    gather passes (FP32 form), with the oracle's pair counts and RHS."""
+    n, rho = 8000, 0.1
+    L = math.sqrt(30000 / 4000.0)
+    st = O.make_state(31, 0, n, L, rho)
+    u = O.uniform(310, 2 * n, 0.0, 0.6)
+    st[0::3] = 1.0 + u[0::2]
+    st[1::3] = 1.0 + u[1::2]
+    kp = O.uniform(311, 3 * n, -1.0, 1.0)
+    kn = O.uniform(312, 3 * n, 5.0, 6.0)
+    ref, rc = O.rhs_k(O.chimera(n, L, rho, 0.3, 1.2, 1.4, 0.05, threads=8), st, kp, 0.5, 0.01, kn)
+    p = spc2.params("chimera", "fp32k", "verlet", n=n, L=L, rho=rho, v0=0.3, sigma=1.2, alpha=1.4, D=0.05)
+    with spc2.Sim(p) as s:
+        s.set_state(st)
+        assert s.verlet_layout() == (0, 0)
+        s.set_counting(True)
+        out = s.eval_rhs(kp, 0.5, 0.01, kn)
+        cnt = s.counts()
+    assert_rhs(out, ref, 0.3, 1.2, TOL["fp32k"], "clustered chimera verlet fp32k")
+    assert np.array_equal(cnt[1:], rc[1:]), (cnt, rc)
