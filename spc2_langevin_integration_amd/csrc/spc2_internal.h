@@ -133,6 +133,11 @@ struct Sim {
   bool vlds = false, vlds_allowed = true;
   int nblk = 0, vlds_cap = 0;
   int *bh_nrun = nullptr, *bh_run = nullptr, *bh_off = nullptr, *bh_max = nullptr;
+  // LDS layout lane slots (vslot[slot] = home) and their wavefront group counts
+  // and offsets; while vlds, wrow holds the slot layout's offsets
+  int *vslot = nullptr, *wmax_s = nullptr, *wrow_s = nullptr;
+  bool vsorted = false;  // the current LDS-layout list uses the lane slots
+  int vsort_mode = -1;   // -1: by list length, 0: never, 1: always (SPC2_VERLET_SORT)
   size_t vlist_cap = 0;
   double *xb = nullptr, *yb = nullptr;  // build positions by home
   long long vbuilds = 0, ventries = 0;
@@ -209,6 +214,7 @@ void launch_verlet_fill(Sim &s);
 constexpr int kVerletRunMax = 16;  // halo runs per pass workgroup (kRunMax of spc2_verlet.hip)
 void launch_block_halo(Sim &s);
 void launch_verlet_fill_lds(Sim &s);
+void launch_verlet_order(Sim &s);
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);
 void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa,
                          double *const k1[3], double *const k2[3], double *const k3[3]);

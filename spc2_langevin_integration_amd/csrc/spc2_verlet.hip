@@ -249,6 +249,32 @@ __global__ __launch_bounds__(64) void k_block_halo(KParams P, const int *start, 
   atomicMax(bh_max, tot);
 }
 
+// Lane slots of the LDS layout: within each pass workgroup (kBlock homes) the
+// homes by decreasing list length (ties by home), so that a wavefront's lists
+// are of similar length and it walks about their mean rather than the longest
+// of 64 independent lists.  The workgroup's halo is the same whichever lane
+// takes a home, but the per-particle reads and writes of the epilogue scatter
+// over the workgroup's homes; the build uses the slots only where lists are
+// long (see verlet_build).  vslot[slot] = home (null: slot = home);
+// wmax_s[w] = wavefront w's group count, the length at its first slot.
+__global__ __launch_bounds__(kBlock) void k_verlet_order(int n, const int *vcnt, int *vslot, int *wmax_s) {
+  __shared__ int c[kBlock];
+  const int base = blockIdx.x * kBlock, m = min(kBlock, n - base);
+  const int me = threadIdx.x;
+  c[me] = me < m ? vcnt[base + me] : 0;
+  __syncthreads();
+  if (me >= m) return;
+  const int mine = c[me];
+  int rank = 0;
+  for (int k = 0; k < m; ++k) {
+    const int ck = c[k];
+    rank += (ck > mine) | ((ck == mine) & (k < me));
+  }
+  const int slot = base + rank;
+  vslot[slot] = base + me;
+  if ((slot & 63) == 0) wmax_s[slot >> 6] = (mine + 3) >> 2;
+}
+
 // LDS index base of build cell c in workgroup b's halo: index = base + home.
 __device__ __forceinline__ int halo_base(const int *run, const int *off, int nrun, const int *start, int c) {
   int k = 0;
@@ -256,17 +282,18 @@ __device__ __forceinline__ int halo_base(const int *run, const int *off, int nru
   return off[k] - start[run[2 * k]];
 }
 
-// Build pass 2 for the LDS layout (FP32 form): as k_verlet_fill, with the
-// partner's index in the workgroup's staged halo in place of its home.
+// Build pass 2 for the LDS layout (FP32 form), per lane slot: as k_verlet_fill,
+// with the partner's index in the workgroup's staged halo in place of its home.
 __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int *start, Tiles t, double rlist2,
-                                                            const int *wrow, const int *bh_nrun, const int *bh_run,
-                                                            const int *bh_off, int *vlist) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const int b = i / kBlock, nrun = bh_nrun[b];
+                                                            const int *vslot, const int *wrow, const int *bh_nrun,
+                                                            const int *bh_run, const int *bh_off, int *vlist) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= P.n) return;
+  const int i = vslot ? vslot[slot] : slot;
+  const int b = slot / kBlock, nrun = bh_nrun[b];
   const int *run = bh_run + (size_t)b * 2 * kRunMax, *off = bh_off + (size_t)b * (kRunMax + 1);
-  const int w = i >> 6, groups = wrow[w + 1] - wrow[w];
-  int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (i & 63);
+  const int w = slot >> 6, groups = wrow[w + 1] - wrow[w];
+  int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (slot & 63);
   const int c = t.cell[i], cx = c % P.nx, cy = c / P.nx;
   const int self = ((halo_base(run, off, nrun, start, c) + i) << 4) | 5;
   int4 buf = make_int4(self, self, self, self);
@@ -499,6 +526,7 @@ __device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, 
 
 template <class Acc, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const int *__restrict__ vcnt,
+                                                             const int *__restrict__ vslot,
                                                              const int *__restrict__ wrow,
                                                              const int *__restrict__ vlist, const int *start,
                                                              const int *bh_nrun, const int *bh_run,
@@ -508,15 +536,16 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const in
   extern __shared__ float4 dyn_lds[];
   const int seg = xcd_segment(blockIdx.x, gridDim.x);
   const Src32 src = stage_halo<1>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap);
-  const int i = seg * kBlock + threadIdx.x;
-  const bool active = i < P.n;
+  const int slot = seg * kBlock + threadIdx.x;
+  const bool active = slot < P.n;
   Pass1Out o{};
   int cnt = 0;
   if (active) {
+    const int i = vslot ? vslot[slot] : slot;
     Acc acc;
     acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P);
     cnt = vcnt[i];
-    walk32<1>(acc, src, (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63), cnt, P);
+    walk32<1>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), cnt, P);
     acc.finish(P, gp[i], o);
     const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
     if (EPI == EPI_K && e.next == 1) {
@@ -531,6 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const in
 
 template <class Acc, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const int *__restrict__ vcnt,
+                                                             const int *__restrict__ vslot,
                                                              const int *__restrict__ wrow,
                                                              const int *__restrict__ vlist, const int *start,
                                                              const int *bh_nrun, const int *bh_run,
@@ -539,8 +569,9 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const in
   extern __shared__ float4 dyn_lds[];
   const int seg = xcd_segment(blockIdx.x, gridDim.x);
   const Src32 src = stage_halo<2>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap);
-  const int i = seg * kBlock + threadIdx.x;
-  if (i >= P.n) return;
+  const int slot = seg * kBlock + threadIdx.x;
+  if (slot >= P.n) return;
+  const int i = vslot ? vslot[slot] : slot;
   Pass2In in{e.aphi_slot[i], e.n_slot[i], e.n2_slot[i], 0.0, 0.0};
   if (EPI == EPI_ITO2) {
     in.in0 = e.a0[i];
@@ -548,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const in
   }
   Acc acc;
   acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P, in);
-  walk32<2>(acc, src, (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63), vcnt[i], P);
+  walk32<2>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
   Pass2Out o;
   acc.finish(P, gp[i], o);
   epilogue2<EPI>(P, e, i, hid[i], o);
@@ -597,12 +628,14 @@ void vpass1_typed(Sim &s, const EpiCtx &e) {
       const size_t sh = (size_t)s.vlds_cap * 20;
       if (SYS == SPC2_CHIMERA)
         launch_pair(s, k_pass1_verlet_lds<ChimAcc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
-                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
+                    (const int *)s.bh_nrun,
                     (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
                     e, cnt);
       else
         launch_pair(s, k_pass1_verlet_lds<VortAcc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
-                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
+                    (const int *)s.bh_nrun,
                     (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
                     e, cnt);
       return;
@@ -646,12 +679,14 @@ void vpass2_typed(Sim &s, const EpiCtx &e) {
       const size_t sh = (size_t)s.vlds_cap * 24;
       if (SYS == SPC2_CHIMERA)
         launch_pair(s, k_pass2_verlet_lds<Chim2Acc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
-                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
+                    (const int *)s.bh_nrun,
                     (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
                     e);
       else
         launch_pair(s, k_pass2_verlet_lds<Vort2Acc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
-                    (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
+                    (const int *)s.bh_nrun,
                     (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
                     e);
       return;
@@ -724,8 +759,14 @@ void launch_block_halo(Sim &s) {
 void launch_verlet_fill_lds(Sim &s) {
   const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
   hipLaunchKernelGGL(k_verlet_fill_lds, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
-                     s.t, r2, (const int *)s.wrow, (const int *)s.bh_nrun, (const int *)s.bh_run,
-                     (const int *)s.bh_off, s.vlist);
+                     s.t, r2, (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow,
+                     (const int *)s.bh_nrun,
+                     (const int *)s.bh_run, (const int *)s.bh_off, s.vlist);
+}
+
+void launch_verlet_order(Sim &s) {
+  hipLaunchKernelGGL(k_verlet_order, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, (const int *)s.vcnt,
+                     s.vslot, s.wmax_s);
 }
 
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt) {

@@ -206,14 +206,19 @@ def test_verlet_fused_stage_tiles_equal_separate_stage_pass(gpu, prec, method):
     assert np.array_equal(a, b), np.abs(a - b).max()
 
 
-def run_layout(params, st, method, dt, steps, monkeypatch, lds, omega=None):
+def run_layout(params, st, method, dt, steps, monkeypatch, lds, omega=None, sort=None):
     """run() with the FP32 pass kernels chosen: lds=False keeps the gather
-    passes (SPC2_VERLET_LDS=0 at creation)."""
+    passes (SPC2_VERLET_LDS=0 at creation); sort = "0" / "1" forces the LDS
+    layout's lane slots off / on (None: by list length)."""
     with monkeypatch.context() as m:
         if lds:
             m.delenv("SPC2_VERLET_LDS", raising=False)
         else:
             m.setenv("SPC2_VERLET_LDS", "0")
+        if sort is None:
+            m.delenv("SPC2_VERLET_SORT", raising=False)
+        else:
+            m.setenv("SPC2_VERLET_SORT", sort)
         with spc2.Sim(params) as s:
             s.set_state(st)
             if omega is not None:
@@ -225,11 +230,13 @@ def run_layout(params, st, method, dt, steps, monkeypatch, lds, omega=None):
             return s.get_state(), s.counts(), s.verlet_stats(), layout
 
 
+@pytest.mark.parametrize("sort", ["0", "1"])
 @pytest.mark.parametrize("system,method", [("chimera", "srk"), ("vortex", "rk4"), ("vortex", "se")])
-def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method):
+def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method, sort):
     """The LDS-staged FP32 passes gather the same tiles from each workgroup's
-    staged halo, with the same arithmetic and order as the gather passes:
-    trajectories across several rebuilds are bit-identical."""
+    staged halo, with the same arithmetic and order as the gather passes,
+    whichever lane takes a particle: trajectories across several rebuilds are
+    bit-identical."""
     n, dt, steps = 30000, 0.01, 24
     L = math.sqrt(n / 4000.0)
     if system == "chimera":
@@ -242,7 +249,7 @@ def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method)
         om = None
     st = O.make_state(9, 0 if system == "chimera" else 1, n, L, rho)
     p = spc2.params(system, "fp32k", "verlet", **kw)
-    a, ca, sa, la = run_layout(p, st, method, dt, steps, monkeypatch, True, om)
+    a, ca, sa, la = run_layout(p, st, method, dt, steps, monkeypatch, True, om, sort)
     b, cb, sb, lb = run_layout(p, st, method, dt, steps, monkeypatch, False, om)
     assert la[0] == 1 and la[1] > 0, la
     assert lb == (0, 0), lb
