@@ -6,7 +6,7 @@ Workload: vortex arrays, deterministic RK4 (D = 0), dt = 0.01, N = 1,048,576 in 
 periodic box at density 4000 (L = sqrt(N / 4000) = 16.19086), rho = 0.05,
 xi_a = 0.01, xi_r = 0.005, v = 0.025, mu+ = mu- = 100, kappa = 20; uniform
 random positions and phases (Philox, seed 42).  Neighbours from a Verlet list
-with skin 0.1 rho, rebuilt before any particle can have moved skin / 2
+with skin 0.125 rho, rebuilt before any particle can have moved skin / 2
 (--neighbour cells: the reference's linked cells rebuilt at every RHS).  Inputs
 are generated on the device and stay resident; a step is one full RK4 step
 (4 RHS evaluations + combine + wrap, plus the list rebuilds that fall in it).
@@ -221,8 +221,8 @@ def main():
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: a few steps of the FP32 workload and a plain device copy of "
                          "COPY_BYTES bytes (the counter calibration), no timing")
-    ap.add_argument("--skin", type=float, default=0.1,
-                    help="Verlet skin as a fraction of rho (0.1: the fastest of 0.1-0.3 on this workload)")
+    ap.add_argument("--skin", type=float, default=0.125,
+                    help="Verlet skin as a fraction of rho (0.125: the fastest of 0.1-0.3 on this workload)")
     ap.add_argument("--error-horizon", type=int, default=100,
                     help="steps after which FP32 and FP64 runs are compared before chaotic decorrelation")
     args = ap.parse_args()

@@ -68,7 +68,7 @@ void free_all(Sim &s) {
   f(s.t2.x); f(s.t2.y); f(s.t2.p); f(s.t2.c); f(s.t2.s); f(s.t2.f4); f(s.gp2);
   f(s.aphi_slot); f(s.n_slot); f(s.n2_slot); f(s.aphi_slot_f);
   f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
-  f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb);
+  f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb); f(s.vmask);
   f(s.bh_nrun); f(s.bh_run); f(s.bh_off); f(s.bh_max); f(s.vslot); f(s.wmax_s); f(s.wrow_s);
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
@@ -182,6 +182,11 @@ spc2_status alloc_all(Sim &s) {
   A(s.bsum, (size_t)scan_blocks(std::max({s.nc, s.nw, 1})) + 1);
   if (s.verlet) {
     A(s.vcnt, n); A(s.wmax, (size_t)s.nw + 1); A(s.wrow, (size_t)s.nw + 1); A(s.xb, n); A(s.yb, n);
+    {  // room for 1.5x the mean candidate count (9 build cells), 1..16 words
+      const double mean = 9.0 * (double)s.n / (double)std::max(s.nc, 1);
+      s.vmask_words = (int)std::min(16.0, std::max(1.0, std::ceil(1.5 * mean / 32.0)));
+      A(s.vmask, (size_t)s.vmask_words * n);
+    }
     if (s.prec == P32K) {
       s.nblk = (s.n + 255) / 256;
       A(s.bh_nrun, (size_t)s.nblk); A(s.bh_run, (size_t)s.nblk * 2 * kVerletRunMax);

@@ -126,6 +126,10 @@ struct Sim {
   // wmax[w] / wrow[w]: wavefront w's group count and first group (see the list
   // layout in spc2_verlet.hip); vlist_cap in ints
   int *vcnt = nullptr, *wmax = nullptr, *wrow = nullptr, *vlist = nullptr;
+  // candidate test bits of the count pass, read back by the fill (words by
+  // word-major [w * n + i]; homes with more candidates are tested again)
+  uint32_t *vmask = nullptr;
+  int vmask_words = 0;
   // LDS-staged FP32 passes: per pass workgroup (kBlock homes) its halo as runs of
   // consecutive build cells (bh_run: first and last cell, bh_off: LDS offsets,
   // bh_nrun: run count or -1), bh_max: the largest halo; vlds when the current

@@ -50,14 +50,14 @@ inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / 
 
 // Visits the candidates of home i in a 3 x 3 stencil of build cells (cells of
 // side >= rlist), ascending home within each cell, in the reference's stencil
-// order; calls f(j, code) for every j != i within rlist.  FP64 forms: min image
-// of the wrapped build positions.  FP32 form: the cell-local FP32 tiles with
-// the pass kernels' offset arithmetic, against rlist^2 (1 + 2^-13), a
-// superset of the FP64 test (the passes test every entry against the cutoffs
-// themselves, and a pair outside rlist at the build stays outside rho while
-// the list is used).
+// order; calls f(j, code, in) for every candidate, in = (j != i and within
+// rlist).  FP64 forms: min image of the wrapped build positions.  FP32 form:
+// the cell-local FP32 tiles with the pass kernels' offset arithmetic, against
+// rlist^2 (1 + 2^-13), a superset of the FP64 test (the passes test every entry
+// against the cutoffs themselves, and a pair outside rlist at the build stays
+// outside rho while the list is used).
 template <int PREC, class F>
-__device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const int *start, const Tiles &t, int i,
+__device__ __forceinline__ void for_each_build_candidate(const KParams &P, const int *start, const Tiles &t, int i,
                                                          double rlist2, F &&f) {
   const int c = t.cell[i];
   const int cx = c % P.nx, cy = c / P.nx;
@@ -77,7 +77,7 @@ __device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const
         for (int j = start[jc]; j < e; ++j) {
           const float4 v = t.f4[j];
           const float dx = v.x + ox, dy = v.y + oy;
-          if ((__builtin_fmaf(dx, dx, dy * dy) <= rl2) & (j != i)) f(j, code);
+          f(j, code, (__builtin_fmaf(dx, dx, dy * dy) <= rl2) & (j != i));
         }
       }
     }
@@ -93,22 +93,80 @@ __device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const
         for (int j = start[jc]; j < e; ++j) {
           double dx = min_image(t.x[j] - xi, P.Lx, P.rLx);
           double dy = min_image(t.y[j] - yi, P.Ly, P.rLy);
-          if ((dx * dx + dy * dy <= rlist2) & (j != i)) f(j, code);
+          f(j, code, (dx * dx + dy * dy <= rlist2) & (j != i));
         }
       }
     }
   }
 }
 
-// Build pass 1: entries per home and each wavefront's group count (its
-// longest list in groups of four).
+// The count pass records each candidate's test as one bit (bit q of word q / 32
+// at mask[(q / 32) * n + i], candidate q in visiting order), up to mwords words;
+// a home with more candidates is tested again by the fill.
+// Visits the entries of home i: calls f(j, code) for every candidate within
+// rlist, from the mask when it holds all candidates, else by testing again.
+template <int PREC, class F>
+__device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const int *start, const Tiles &t, int i,
+                                                         double rlist2, const uint32_t *mask, int mwords, F &&f) {
+  const int c = t.cell[i];
+  const int cx = c % P.nx, cy = c / P.nx;
+  int total = 0;
+  for (int dyo = -1; dyo <= 1; ++dyo) {
+    const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
+    for (int dxo = -1; dxo <= 1; ++dxo) {
+      const int jc = row + adjust_cell(cx + dxo, P.nx);
+      total += start[jc + 1] - start[jc];
+    }
+  }
+  if (total > 32 * mwords) {
+    for_each_build_candidate<PREC>(P, start, t, i, rlist2, [&](int j, int code, bool in) {
+      if (in) f(j, code);
+    });
+    return;
+  }
+  int q0 = 0;
+  for (int dyo = -1; dyo <= 1; ++dyo) {
+    const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
+    for (int dxo = -1; dxo <= 1; ++dxo) {
+      const int jc = row + adjust_cell(cx + dxo, P.nx);
+      const int code = (dxo + 1) | ((dyo + 1) << 2);
+      const int s0 = start[jc], q1 = q0 + start[jc + 1] - s0;
+      for (int q = q0; q < q1;) {
+        const int sh = q & 31, span = min(32 - sh, q1 - q);
+        uint32_t m = mask[(size_t)(q >> 5) * P.n + i] >> sh;
+        if (span < 32) m &= (1u << span) - 1u;
+        while (m) {
+          const int bit = __ffs(m) - 1;
+          m &= m - 1u;
+          f(s0 + (q - q0) + bit, code);
+        }
+        q += span;
+      }
+      q0 = q1;
+    }
+  }
+}
+
+// Build pass 1: entries per home, their candidate bits and each wavefront's
+// group count (its longest list in groups of four).
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *start, Tiles t, double rlist2,
-                                                         int *vcnt, int *wgroups) {
+                                                         uint32_t *mask, int mwords, int *vcnt, int *wgroups) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
   if (i < P.n) {
-    for_each_build_neighbour<PREC>(P, start, t, i, rlist2, [&](int, int) { ++cnt; });
+    uint32_t word = 0;
+    int q = 0;
+    for_each_build_candidate<PREC>(P, start, t, i, rlist2, [&](int, int, bool in) {
+      word |= (uint32_t)in << (q & 31);
+      cnt += in;
+      if ((q & 31) == 31) {
+        if ((q >> 5) < mwords) mask[(size_t)(q >> 5) * P.n + i] = word;
+        word = 0;
+      }
+      ++q;
+    });
+    if ((q & 31) && (q >> 5) < mwords) mask[(size_t)(q >> 5) * P.n + i] = word;
     vcnt[i] = cnt;
   }
   int m = cnt;
@@ -122,7 +180,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *s
 // the column padded to the wavefront's group count.
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *start, Tiles t, double rlist2,
-                                                        const int *wrow, int *vlist) {
+                                                        const uint32_t *mask, int mwords, const int *wrow,
+                                                        int *vlist) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const int w = i >> 6, groups = wrow[w + 1] - wrow[w];
@@ -130,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *st
   const int self = (i << 4) | 5;
   int4 buf = make_int4(self, self, self, self);
   int e = 0;
-  for_each_build_neighbour<PREC>(P, start, t, i, rlist2, [&](int j, int code) {
+  for_each_build_neighbour<PREC>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
     const int ent = (j << 4) | code;
     switch (e & 3) {
       case 0: buf.x = ent; break;
@@ -285,20 +344,25 @@ __device__ __forceinline__ int halo_base(const int *run, const int *off, int nru
 // Build pass 2 for the LDS layout (FP32 form), per lane slot: as k_verlet_fill,
 // with the partner's index in the workgroup's staged halo in place of its home.
 __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int *start, Tiles t, double rlist2,
-                                                            const int *vslot, const int *wrow, const int *bh_nrun,
-                                                            const int *bh_run, const int *bh_off, int *vlist) {
+                                                            const uint32_t *mask, int mwords, const int *vslot,
+                                                            const int *wrow, const int *bh_nrun, const int *bh_run,
+                                                            const int *bh_off, int *vlist) {
+  // the workgroup is one pass workgroup: its halo runs go to LDS once
+  __shared__ int run[2 * kRunMax], off[kRunMax + 1];
+  const int b = blockIdx.x, nrun = bh_nrun[b];
+  if ((int)threadIdx.x < 2 * kRunMax) run[threadIdx.x] = bh_run[(size_t)b * 2 * kRunMax + threadIdx.x];
+  if ((int)threadIdx.x <= kRunMax) off[threadIdx.x] = bh_off[(size_t)b * (kRunMax + 1) + threadIdx.x];
+  __syncthreads();
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= P.n) return;
   const int i = vslot ? vslot[slot] : slot;
-  const int b = slot / kBlock, nrun = bh_nrun[b];
-  const int *run = bh_run + (size_t)b * 2 * kRunMax, *off = bh_off + (size_t)b * (kRunMax + 1);
   const int w = slot >> 6, groups = wrow[w + 1] - wrow[w];
   int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (slot & 63);
   const int c = t.cell[i], cx = c % P.nx, cy = c / P.nx;
   const int self = ((halo_base(run, off, nrun, start, c) + i) << 4) | 5;
   int4 buf = make_int4(self, self, self, self);
   int e = 0, last = -1, base = 0;
-  for_each_build_neighbour<P32K>(P, start, t, i, rlist2, [&](int j, int code) {
+  for_each_build_neighbour<P32K>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
     if (code != last) {
       const int jc = adjust_cell(cy + (code >> 2) - 1, P.ny) * P.nx + adjust_cell(cx + (code & 3) - 1, P.nx);
       base = halo_base(run, off, nrun, start, jc);
@@ -721,15 +785,16 @@ void launch_verlet_count(Sim &s) {
   const dim3 g(blocks_for(s.n)), b(kBlock);
   switch (s.prec) {
     case P64:
-      hipLaunchKernelGGL(k_verlet_count<P64>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vcnt, s.wmax);
+      hipLaunchKernelGGL(k_verlet_count<P64>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
+                         s.vmask_words, s.vcnt, s.wmax);
       break;
     case P64F:
-      hipLaunchKernelGGL(k_verlet_count<P64F>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vcnt,
-                         s.wmax);
+      hipLaunchKernelGGL(k_verlet_count<P64F>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
+                         s.vmask_words, s.vcnt, s.wmax);
       break;
     default:
-      hipLaunchKernelGGL(k_verlet_count<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vcnt,
-                         s.wmax);
+      hipLaunchKernelGGL(k_verlet_count<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
+                         s.vmask_words, s.vcnt, s.wmax);
   }
 }
 
@@ -739,15 +804,15 @@ void launch_verlet_fill(Sim &s) {
   switch (s.prec) {
     case P64:
       hipLaunchKernelGGL(k_verlet_fill<P64>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2,
-                         (const int *)s.wrow, s.vlist);
+                         (const uint32_t *)s.vmask, s.vmask_words, (const int *)s.wrow, s.vlist);
       break;
     case P64F:
       hipLaunchKernelGGL(k_verlet_fill<P64F>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2,
-                         (const int *)s.wrow, s.vlist);
+                         (const uint32_t *)s.vmask, s.vmask_words, (const int *)s.wrow, s.vlist);
       break;
     default:
       hipLaunchKernelGGL(k_verlet_fill<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2,
-                         (const int *)s.wrow, s.vlist);
+                         (const uint32_t *)s.vmask, s.vmask_words, (const int *)s.wrow, s.vlist);
   }
 }
 
@@ -759,8 +824,8 @@ void launch_block_halo(Sim &s) {
 void launch_verlet_fill_lds(Sim &s) {
   const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
   hipLaunchKernelGGL(k_verlet_fill_lds, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
-                     s.t, r2, (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow,
-                     (const int *)s.bh_nrun,
+                     s.t, r2, (const uint32_t *)s.vmask, s.vmask_words,
+                     (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.bh_nrun,
                      (const int *)s.bh_run, (const int *)s.bh_off, s.vlist);
 }
 
