@@ -27,11 +27,11 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32k": 157.3, "fp64": 78.6, "fp64f": 78.6}  # TFLOP/s, MI355X vector FP32 / FP64
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_pass1_verlet<VortAcc<P32K>> on this workload (skin
-# 0.1 rho), from the FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) counter passes
-# calibrated against a 512 MiB device copy, stages 1-3 and 4 weighted 3:1;
-# profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 366.7e6
+# HBM bytes per launch of the FP32 Verlet pair kernel (k_pass1_verlet_lds
+# <VortAcc<P32K>>) on this workload (skin 0.125 rho), from the FETCH_SIZE (x2.000)
+# and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
+# copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
+TRAFFIC_VERLET_FP32K = 376.1e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -123,25 +123,31 @@ def profile_pair_kernel(spc2, w, precision, steps, neighbour):
             "cohesion_per_particle": per[2] / w["n"]}
 
 
-def error_horizon(spc2, w, nb, horizon):
-    """FP32-compensated vs FP64 from the same initial state after `horizon`
-    steps, and the FP64 run against FP64 with the other neighbour structure
-    (same arithmetic, another summation order): the divergence floor of this
-    chaotic system at that horizon."""
+def error_horizon(spc2, w, nb, horizons):
+    """FP32-compensated vs FP64 from the same initial state after each of
+    `horizons` steps, and the FP64 run against FP64 with the other neighbour
+    structure (same arithmetic, another summation order): the divergence floor
+    of this chaotic system at that horizon."""
     import numpy as np
+    horizons = sorted(set(horizons))
     ph = {}
-    for prec, neigh in (("fp64", nb), ("fp32k", nb), ("fp64", "cells" if nb == "verlet" else "verlet")):
+    other = "cells" if nb == "verlet" else "verlet"
+    for prec, neigh in (("fp64", nb), ("fp32k", nb), ("fp64", other)):
         s = make_sim(spc2, w, prec, neigh)
-        s.step(w["method"], w["dt"], 0, horizon)
-        ph[(prec, neigh)] = s.get_state()[2::3]
+        k = 0
+        for h in horizons:
+            s.step(w["method"], w["dt"], k, h - k)
+            k = h
+            ph[(prec, neigh, h)] = s.get_state()[2::3]
         s.close()
-    ref = ph[("fp64", nb)]
     R = lambda p: float(np.abs(np.exp(1j * p).mean()))
     out = {}
-    for key, name in (((("fp32k", nb)), "fp32"), (next(k for k in ph if k[1] != nb), "fp64_reordered")):
-        d = np.abs(ph[key] - ref)
-        out[name] = {"phase_err_max": float(d.max()), "phase_err_rms": float(np.sqrt((d ** 2).mean())),
-                     "order_param_abs_err": abs(R(ph[key]) - R(ref))}
+    for h in horizons:
+        ref = ph[("fp64", nb, h)]
+        for key, name in ((("fp32k", nb, h), "fp32"), (("fp64", other, h), "fp64_reordered")):
+            d = np.abs(ph[key] - ref)
+            out[(name, h)] = {"phase_err_max": float(d.max()), "phase_err_rms": float(np.sqrt((d ** 2).mean())),
+                              "order_param_abs_err": abs(R(ph[key]) - R(ref))}
     return out
 
 
@@ -250,10 +256,12 @@ def main():
     print(f"# pair kernel fp32k: {prof['avg_ms']:.4f} ms/launch, {prof['tflops']:.2f} TFLOP/s, "
           f"{prof['gbs']:.0f} GB/s" + (f"; fp64: {prof64['avg_ms']:.4f} ms/launch" if prof64 else ""),
           file=sys.stderr, flush=True)
-    hz = error_horizon(spc2, w, nb, args.error_horizon) if "fp64" in res and args.error_horizon > 0 else None
+    horizons = [h for h in (args.error_horizon, args.warmup + args.steps) if h > 0]
+    hz = error_horizon(spc2, w, nb, horizons) if "fp64" in res and args.error_horizon > 0 else None
     if hz:
-        print(f"# after {args.error_horizon} steps: fp32 vs fp64 {hz['fp32']}; fp64 reordered {hz['fp64_reordered']}",
-              file=sys.stderr, flush=True)
+        for h in sorted(set(horizons)):
+            print(f"# after {h} steps: fp32 vs fp64 {hz[('fp32', h)]}; fp64 reordered {hz[('fp64_reordered', h)]}",
+                  file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(w)
 
     v = res["fp32k"]
@@ -267,10 +275,11 @@ def main():
             f"after_{args.warmup + args.steps}_steps": {
                 "phase_err_max": float(dphi.max()), "phase_err_rms": float(np.sqrt((dphi ** 2).mean())),
                 "order_param_abs_err": abs(v["R_final"] - res["fp64"]["R_final"])},
-            **({f"after_{args.error_horizon}_steps": hz["fp32"],
-                f"fp64_reordered_after_{args.error_horizon}_steps": hz["fp64_reordered"]} if hz else {})}}
+            **({f"{'fp64_reordered_' if name == 'fp64_reordered' else ''}after_{h}_steps": val
+                for (name, h), val in sorted(hz.items(), key=lambda kv: (kv[0][1], kv[0][0]))
+                if not (name == "fp32" and h == args.warmup + args.steps)} if hz else {})}}
     acc = "VortAcc" if w["system"] == "vortex" else "ChimAcc, Chim2Acc"
-    kname = (f"k_pass1_verlet<{acc}<P32K>>" if hbm else f"k_pass1_lds<{acc}<P32K>>") + \
+    kname = (f"k_pass1_verlet_lds<{acc}<P32K>>" if hbm else f"k_pass1_lds<{acc}<P32K>>") + \
         f" ({w['system']} pair sums, FP32 Kahan)"
     line = {
         "metric": ("particle-steps/s at N=1M (FP32-comp vs FP64, same GPU)" if headline else
