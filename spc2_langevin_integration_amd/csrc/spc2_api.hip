@@ -128,6 +128,11 @@ void fill_kparams_from(KParams &k, const spc2_params &P, int n, int nx, int ny, 
   k.mu_pf = (float)P.mu_plus;
   k.mu_p_over_xa2f = P.xi_a > 0 ? (float)(P.mu_plus / (P.xi_a * P.xi_a)) : 0.0f;
   k.c4f = P.rho > P.xi_a ? (float)(-P.mu_minus * 4.0 / ((P.rho - P.xi_a) * (P.rho - P.xi_a))) : 0.0f;
+  {
+    const double c4 = P.rho > P.xi_a ? -P.mu_minus * 4.0 / ((P.rho - P.xi_a) * (P.rho - P.xi_a)) : 0.0;
+    k.w_af = (float)(c4 * (P.xi_a + P.rho));
+    k.w_bf = (float)(-c4 * P.xi_a * P.rho);
+  }
   k.kappaf = (float)P.kappa;
   k.v0f = (float)P.v0;
   k.Df = (float)P.D;
@@ -251,26 +256,29 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   if (spc2_status st = check_launch("verlet count")) return st;
   // the LDS layout (FP32 form): every pass workgroup's halo, and the largest
   const bool try_lds = s.vlds_allowed && s.bh_max;
+  // lane slots sorted by list length pay off when lists are long enough for the
+  // saved wavefront iterations to outweigh the epilogue's scattered accesses
+  // (measured: 150 entries per particle gain, 40 lose); the mean list length is
+  // estimated from the mean density
+  const double mean_list = (double)s.n / (s.P.Lx * s.P.Ly) * 3.14159265358979 * s.rlist * s.rlist;
+  const bool want_sort = try_lds && (s.vsort_mode == 1 || (s.vsort_mode < 0 && mean_list > 72.0));
   if (try_lds) {
-    launch_verlet_order(s);
-    launch_scan_array(s, s.wmax_s, s.nw, s.wrow_s);
+    if (want_sort) {
+      launch_verlet_order(s);
+      launch_scan_array(s, s.wmax_s, s.nw, s.wrow_s);
+    }
     HIP_TRY(hipMemsetAsync(s.bh_max, 0, sizeof(int), s.stream));
     launch_block_halo(s);
   }
   int groups = 0, groups_s = 0, halo = 0;
   HIP_TRY(hipMemcpyAsync(&groups, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-  if (try_lds) {
-    HIP_TRY(hipMemcpyAsync(&groups_s, s.wrow_s + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    HIP_TRY(hipMemcpyAsync(&halo, s.bh_max, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-  }
+  if (want_sort) HIP_TRY(hipMemcpyAsync(&groups_s, s.wrow_s + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  if (try_lds) HIP_TRY(hipMemcpyAsync(&halo, s.bh_max, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
   const int cap_limit = kLdsBytes / 24;  // pass 2 stages 24 B per halo particle
   s.vlds = try_lds && halo >= 0 && halo <= cap_limit;
   s.vlds_cap = s.vlds ? std::min(cap_limit, std::max(64, (halo + 63) / 64 * 64)) : 0;
-  // lane slots sorted by list length pay off when lists are long enough for the
-  // saved wavefront iterations to outweigh the epilogue's scattered accesses
-  // (measured: 150 entries per particle gain, 38 lose)
-  s.vsorted = s.vlds && (s.vsort_mode == 1 || (s.vsort_mode < 0 && (long long)groups * 64 * 4 > 96LL * s.n));
+  s.vsorted = s.vlds && want_sort;
   if (s.vsorted) {  // the passes and the fill use the lane-slot layout
     std::swap(s.wrow, s.wrow_s);
     groups = groups_s;

@@ -42,6 +42,9 @@ struct KParams {
   double mu_plus, mu_minus, xi_a, xi_r, kappa;
   double outer2;  // max(rho, xi_r)^2: widest test radius
   float xa2f, xr2f, outer2f, xaf, rhof, mu_pf, mu_p_over_xa2f, c4f, kappaf, v0f, Df;
+  // outer W(r) = c4 (r - xi_a)(rho - r) in Horner form r (w_a - c4 r) + w_b,
+  // w_a = c4 (xi_a + rho), w_b = -c4 xi_a rho
+  float w_af, w_bf;
   // FP32 cutoff band: |r^2 - cutoff^2| <= band * cutoff^2 is re-decided in FP64
   float band_rho2f, band_xr2f, band_xa2f;
   // FP32 cohesion pairs closer than this (r^2) take FP64 geometry

@@ -413,7 +413,8 @@ struct VortAcc<P32K> {
     }
     float r = __builtin_amdgcn_sqrtf(r2);
     float w_in = __builtin_fmaf(-P.mu_p_over_xa2f, r2, P.mu_pf);
-    float w_an = P.c4f * (r - P.xaf) * (P.rhof - r);
+    // c4 (r - xi_a)(rho - r) in Horner form (two FMAs; the same FP32 error bound)
+    float w_an = __builtin_fmaf(r, __builtin_fmaf(-P.c4f, r, P.w_af), P.w_bf);
     float w = r2 <= P.xa2f ? w_in : w_an;
     w = inr ? w : 0.0f;
     SW.add_prod(w, v.w);
