@@ -85,6 +85,8 @@ struct Pass1Out {
   double coh;      // vortex cohesion / n2 for vortex_add
   int n;           // partners within rho
   int n2;          // cohesion partners
+  int has_cs;      // cs, sn hold cos, sin of phase (computed by finish)
+  double cs, sn;
 };
 
 // FP32 x offset of the candidate's cell column relative to i's: m < b1 is the
@@ -443,6 +445,9 @@ struct VortAcc<P32K> {
     double sp, cp;
     sincos(phase, &sp, &cp);
     o.phase = phase;
+    o.has_cs = 1;  // the epilogue's velocity uses the same cos, sin
+    o.cs = cp;
+    o.sn = sp;
     o.n = n01;
     o.n2 = n2;
     double coh = n2 > 0 ? -P.kappa * (cp * Ty.value() - sp * Tx.value()) / (double)n2 : 0.0;
@@ -902,7 +907,12 @@ template <int EPI>
 __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e, int h, int id, int slot,
                                               const Pass1Out &o) {
   double sn, cs;
-  sincos(o.phase, &sn, &cs);
+  if (o.has_cs) {
+    sn = o.sn;
+    cs = o.cs;
+  } else {
+    sincos(o.phase, &sn, &cs);
+  }
   double ax = P.v0 * cs, ay = P.v0 * sn;
   EpiState r{ax, ay, 0.0, 0.0, 0.0, 0.0};
   if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
