@@ -233,8 +233,8 @@ def main():
                     help="steps after which FP32 and FP64 runs are compared before chaotic decorrelation")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every GPU box here has one MI355X and this build has no multi-GPU path "
-              "(the spatial-slab sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (a large system would shard by "
+              "spatial slabs of cell rows, with one halo row exchanged per pass)", file=sys.stderr)
         sys.exit(2)
 
     import spc2_langevin_integration_amd as spc2

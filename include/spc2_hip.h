@@ -25,7 +25,7 @@ typedef enum spc2_status {
   SPC2_ERR_INVALID = 1,     /* bad argument or parameter combination */
   SPC2_ERR_HIP = 2,         /* a HIP runtime call failed */
   SPC2_ERR_NOMEM = 3,       /* device allocation failed */
-  SPC2_ERR_UNSUPPORTED = 4, /* valid request this build does not provide */
+  SPC2_ERR_UNSUPPORTED = 4, /* valid request this library does not provide */
 } spc2_status;
 
 typedef enum spc2_system { SPC2_CHIMERA = 0, SPC2_VORTEX = 1 } spc2_system;

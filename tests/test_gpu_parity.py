@@ -129,10 +129,10 @@ def test_cell_list_matches_reference_hash(gpu, golden, name, seed, mode, rho):
 
 
 @pytest.mark.parametrize("n,rho,mode", [
-    (1000, 0.1, 0),          # config 0: L = 1
-    (262144, 0.1, 0),        # config 1: L = sqrt(N / 4000)
-    (1048576, 0.05, 0),      # config 2
-    (1048576, 0.05, 1),      # config 2 size, positions outside the box
+    (1000, 0.1, 0),          # the reference's CPU case: L = 1
+    (262144, 0.1, 0),        # chimera SRK at density 4000: L = sqrt(N / 4000)
+    (1048576, 0.05, 0),      # vortex arrays at N = 1M
+    (1048576, 0.05, 1),      # the same size, positions outside the box
 ])
 def test_cell_list_full_size_bit_exact(gpu, n, rho, mode):
     L = 1.0 if n == 1000 else math.sqrt(n / 4000.0)
