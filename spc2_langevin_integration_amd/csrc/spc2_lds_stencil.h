@@ -13,6 +13,8 @@
 // arithmetic and order.  Compiled with -ffp-contract=off.
 #pragma once
 
+#include <type_traits>
+
 #include "spc2_pair_accumulators.h"
 
 namespace spc2 {
@@ -157,6 +159,32 @@ template <class Acc, class S>
 __device__ __forceinline__ int visit_lds(Acc &acc, const S &src, const SegShared &sh, int ncol, int qi, int ei,
                                          const KParams &P) {
   int cand = 0;
+  if constexpr (std::is_same<S, Src32>::value) {
+    // FP32 form: one loop per stencil cell, its column offset computed once
+    // (the value the per-candidate selection gives inside that cell)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int base = r * ncol + qi;
+      const int lo = sh.off[base - 1], b1 = sh.off[base], b2 = sh.off[base + 1], hi = sh.off[base + 2];
+      acc.row(r - 1, b1, b2, P);
+      cand += hi - lo;
+      const float ol = acc.co.col(-1), oc = acc.co.col(0), orr = acc.co.col(1);
+#pragma unroll 2
+      for (int m = lo; m < b1; ++m) acc.pair_x(src, m, ol, P);
+      if (r == 1) {
+#pragma unroll 2
+        for (int m = b1; m < ei; ++m) acc.pair_x(src, m, oc, P);
+#pragma unroll 2
+        for (int m = ei + 1; m < b2; ++m) acc.pair_x(src, m, oc, P);
+      } else {
+#pragma unroll 2
+        for (int m = b1; m < b2; ++m) acc.pair_x(src, m, oc, P);
+      }
+#pragma unroll 2
+      for (int m = b2; m < hi; ++m) acc.pair_x(src, m, orr, P);
+    }
+    return cand - 1;
+  }
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const int base = r * ncol + qi;

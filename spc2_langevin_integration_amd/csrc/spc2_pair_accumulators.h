@@ -118,6 +118,8 @@ struct ColOffsets {
     float sel = (float)((m >= b2) - (m < b1));
     return __builtin_fmaf(sel, hxf, oM);
   }
+  // ox of stencil column d in {-1, 0, 1} (the value ox gives inside that column)
+  __device__ __forceinline__ float col(int d) const { return __builtin_fmaf((float)d, hxf, oM); }
 };
 
 // =============================================================================
@@ -225,6 +227,11 @@ struct ChimAcc<P32K> {
   __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
     const float4 v = s.f4[m];
     pair_d(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
+  }
+  // pair() with the column offset known (a run of candidates in one stencil column)
+  __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
+    const float4 v = s.f4[m];
+    pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
   }
   // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
   __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float, const KParams &P) {
@@ -397,6 +404,11 @@ struct VortAcc<P32K> {
   __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
     const float4 v = s.f4[m];
     pair_d(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
+  }
+  // pair() with the column offset known (a run of candidates in one stencil column)
+  __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
+    const float4 v = s.f4[m];
+    pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
   }
   // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
   __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float, const KParams &P) {
@@ -586,9 +598,11 @@ struct Chim2Acc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_x(s, m, co.ox(m), P); }
+  // pair() with the column offset known (a run of candidates in one stencil column)
+  __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
     float4 v = s.f4[m];
-    float dx = v.x + co.ox(m), dy = v.y + co.oy;
+    float dx = v.x + ox, dy = v.y + co.oy;
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f;
     if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
@@ -790,9 +804,11 @@ struct Vort2Acc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_x(s, m, co.ox(m), P); }
+  // pair() with the column offset known (a run of candidates in one stencil column)
+  __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
     const float4 v = s.f4[m];
-    pair_t(s, m, v, v.x + co.ox(m), v.y + co.oy, 0.0f, false, P);
+    pair_t(s, m, v, v.x + ox, v.y + co.oy, 0.0f, false, P);
   }
   // a list entry with the partner's tile and pass-1 derivative already loaded
   __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float ajv,
