@@ -200,16 +200,20 @@ spc2_status alloc_all(Sim &s) {
     }
   }
 #undef A
-  HIP_TRY(hipMemset(s.count, 0, sizeof(int) * nc));
-  HIP_TRY(hipMemset(s.omega, 0, sizeof(double) * n));
+  // Cleared on the handle's stream, so that the first kernels on that
+  // (non-blocking) stream see zeros: the cell histogram must start at zero,
+  // since the first binning's arrival ranks index the placement.
+  HIP_TRY(hipMemsetAsync(s.count, 0, sizeof(int) * nc, s.stream));
+  HIP_TRY(hipMemsetAsync(s.omega, 0, sizeof(double) * n, s.stream));
   for (auto &kk : s.k)
-    for (auto &p : kk) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
-  for (auto &p : s.der) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
-  for (auto &p : s.add) HIP_TRY(hipMemset(p, 0, sizeof(double) * n));
-  HIP_TRY(hipMemset(s.counts, 0, sizeof(unsigned long long) * 3 * kCountSlots));
-  HIP_TRY(hipMemset(s.sx, 0, sizeof(double) * n));
-  HIP_TRY(hipMemset(s.sy, 0, sizeof(double) * n));
-  HIP_TRY(hipMemset(s.sp, 0, sizeof(double) * n));
+    for (auto &p : kk) HIP_TRY(hipMemsetAsync(p, 0, sizeof(double) * n, s.stream));
+  for (auto &p : s.der) HIP_TRY(hipMemsetAsync(p, 0, sizeof(double) * n, s.stream));
+  for (auto &p : s.add) HIP_TRY(hipMemsetAsync(p, 0, sizeof(double) * n, s.stream));
+  HIP_TRY(hipMemsetAsync(s.counts, 0, sizeof(unsigned long long) * 3 * kCountSlots, s.stream));
+  HIP_TRY(hipMemsetAsync(s.sx, 0, sizeof(double) * n, s.stream));
+  HIP_TRY(hipMemsetAsync(s.sy, 0, sizeof(double) * n, s.stream));
+  HIP_TRY(hipMemsetAsync(s.sp, 0, sizeof(double) * n, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
   return SPC2_OK;
 }
 

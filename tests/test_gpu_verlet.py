@@ -299,3 +299,23 @@ def test_verlet_lds_clustered_state_keeps_gather(gpu):
         cnt = s.counts()
     assert_rhs(out, ref, 0.3, 1.2, TOL["fp32k"], "clustered chimera verlet fp32k")
     assert np.array_equal(cnt[1:], rc[1:]), (cnt, rc)
+
+
+def test_verlet_fresh_handles_build_sane_lists(gpu):
+    """Handles created one after another in a process (each reusing the
+    previous one's freed device memory) build their first list from a cleared
+    cell histogram: the list length per particle stays at the density's
+    expectation for every handle."""
+    n, rho = 262144, 0.05
+    L = math.sqrt(n / 4000.0)
+    skin = 0.14 * rho
+    expect = 4000.0 * math.pi * (rho + skin) ** 2
+    for k in range(4):
+        p = spc2.params("vortex", "fp32k", "verlet", n=n, L=L, rho=rho, D=0.0, verlet_skin=skin, seed=40 + k,
+                        **VORT)
+        with spc2.Sim(p) as s:
+            s.init_random_state(0.0)
+            builds, entries = s.verlet_stats()
+            s.step("rk4", 0.01, 0, 2)
+        assert builds == 1
+        assert 0.9 * expect < entries / n < 1.1 * expect, (k, entries / n, expect)
