@@ -259,7 +259,7 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   launch_scan_array(s, s.wmax, s.nw, s.wrow);
   if (spc2_status st = check_launch("verlet count")) return st;
   // the LDS layout (FP32 form): every pass workgroup's halo, and the largest
-  const bool try_lds = s.vlds_allowed && s.bh_max;
+  const bool try_lds = s.vlds_allowed && s.bh_max && verlet_lds_unbased(s.P.system);
   // lane slots sorted by list length pay off when lists are long enough for the
   // saved wavefront iterations to outweigh the epilogue's scattered accesses
   // (measured: 150 entries per particle gain, 40 lose); the mean list length is
@@ -279,9 +279,10 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   if (want_sort) HIP_TRY(hipMemcpyAsync(&groups_s, s.wrow_s + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   if (try_lds) HIP_TRY(hipMemcpyAsync(&halo, s.bh_max, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIP_TRY(hipStreamSynchronize(s.stream));
-  const int cap_limit = kLdsBytes / 24;  // pass 2 stages 24 B per halo particle
-  s.vlds = try_lds && halo >= 0 && halo <= cap_limit;
-  s.vlds_cap = s.vlds ? std::min(cap_limit, std::max(64, (halo + 63) / 64 * 64)) : 0;
+  // pass 2 stages 24 B per halo particle, plus one sentinel tile after the halo
+  const int cap_limit = kLdsBytes / 24;
+  s.vlds = try_lds && halo >= 0 && halo < cap_limit;  // halo is INT_MAX when a workgroup cannot be staged
+  s.vlds_cap = s.vlds ? std::min(cap_limit, std::max(64, (halo + 1 + 63) / 64 * 64)) : 0;
   s.vsorted = s.vlds && want_sort;
   if (s.vsorted) {  // the passes and the fill use the lane-slot layout
     std::swap(s.wrow, s.wrow_s);

@@ -220,6 +220,8 @@ void launch_verlet_count(Sim &s);
 void launch_verlet_fill(Sim &s);
 constexpr int kVerletRunMax = 16;  // halo runs per pass workgroup (kRunMax of spc2_verlet.hip)
 void launch_block_halo(Sim &s);
+// the LDS layout's pass kernels of this system start their dynamic LDS at address 0
+bool verlet_lds_unbased(int system);
 void launch_verlet_fill_lds(Sim &s);
 void launch_verlet_order(Sim &s);
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);

@@ -7,6 +7,9 @@
 //       candidates (FP32 cell-local forms only),
 //   pair(src, m, params)     candidate at tile index m,
 //   finish(params, phase, out).
+// The FP32 forms also take Verlet-list entries: pair_c<MODE>(src, m, fx, fy,
+// tile, aj, params) with the partner's tile loaded and its stencil column and
+// row as floats, and flush() at the end of each list group (SumMode).
 // Compiled with -ffp-contract=off.
 #pragma once
 
@@ -107,13 +110,10 @@ struct ColOffsets {
     kx = -hx - xl;
     ky = -hy - yl;
   }
-  // x / y offsets from a list entry's stencil code (dcx + 1) | (dcy + 1) << 2
-  __device__ __forceinline__ float cdx(int code, float hx) const {
-    return __builtin_fmaf((float)(code & 3), hx, kx);
-  }
-  __device__ __forceinline__ float cdy(int code, float hy) const {
-    return __builtin_fmaf((float)((code >> 2) & 3), hy, ky);
-  }
+  // x / y offsets from a list entry's stencil column dcx + 1 and row dcy + 1
+  // (each 0, 1 or 2, as floats)
+  __device__ __forceinline__ float cdx(float fx, float hx) const { return __builtin_fmaf(fx, hx, kx); }
+  __device__ __forceinline__ float cdy(float fy, float hy) const { return __builtin_fmaf(fy, hy, ky); }
   __device__ __forceinline__ float ox(int m) const {
     float sel = (float)((m >= b2) - (m < b1));
     return __builtin_fmaf(sel, hxf, oM);
@@ -121,6 +121,32 @@ struct ColOffsets {
   // ox of stencil column d in {-1, 0, 1} (the value ox gives inside that column)
   __device__ __forceinline__ float col(int d) const { return __builtin_fmaf((float)d, hxf, oM); }
 };
+
+// How an FP32 pair term is summed. The cell walks add each term to the Kahan
+// sums. The Verlet walk sums a list group of four entries in plain FP32
+// partials (the group's first entry starts them) and adds the partials to the
+// Kahan sums once per group (flush()): a compensated sum over groups, whose
+// error bound is the Kahan bound plus 3 ulp of each group's absolute sum.
+enum SumMode { SUM_KAHAN = 0, SUM_GROUP_FIRST = 1, SUM_GROUP = 2 };
+
+template <int MODE>
+__device__ __forceinline__ void group_add(float &p, float x) {
+  p = MODE == SUM_GROUP_FIRST ? x : p + x;
+}
+template <int MODE>
+__device__ __forceinline__ void group_add_prod(float &p, float a, float b) {
+  p = MODE == SUM_GROUP_FIRST ? a * b : __builtin_fmaf(a, b, p);
+}
+
+// A kernel-argument constant copied into a vector register once: an FP32
+// instruction reads at most one scalar register, so keeping a constant that
+// meets another one in the same instruction in a vector register saves a move
+// per use.
+__device__ __forceinline__ float in_vgpr(float x) {
+  float r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
 
 // =============================================================================
 // Chimera pass 1 (ChimeraSystem.cpp:391-424)
@@ -204,6 +230,7 @@ struct ChimAcc<P32K> {
   float xl, yl;
   ColOffsets co;
   Kahan S, C;
+  float ps, pc;  // the current list group's partial sums
   int n, k;
   const double *tx, *ty;
   __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P) {
@@ -214,6 +241,7 @@ struct ChimAcc<P32K> {
     co.set_codes(xl, yl, P.hxf, P.hyf);
     S.init();
     C.init();
+    ps = pc = 0.0f;
     n = 0;
     k = kk;
     tx = t.x;
@@ -234,17 +262,29 @@ struct ChimAcc<P32K> {
     pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
   }
   // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
-  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float, const KParams &P) {
-    pair_d(s, m, v, v.x + co.cdx(code, P.hxf), v.y + co.cdy(code, P.hyf), P);
+  template <int MODE>
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float,
+                                         const KParams &P) {
+    pair_d<MODE>(s, m, v, v.x + co.cdx(fx, P.hxf), v.y + co.cdy(fy, P.hyf), P);
   }
+  __device__ __forceinline__ void flush() {
+    S.add(ps);
+    C.add(pc);
+  }
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
                                          const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f;
     if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
       in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
-    S.add(in ? v.w : 0.0f);
-    C.add(in ? v.z : 0.0f);
+    if constexpr (MODE == SUM_KAHAN) {
+      S.add(in ? v.w : 0.0f);
+      C.add(in ? v.z : 0.0f);
+    } else {
+      group_add<MODE>(ps, in ? v.w : 0.0f);
+      group_add<MODE>(pc, in ? v.z : 0.0f);
+    }
     n += in;
   }
   __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
@@ -377,6 +417,8 @@ struct VortAcc<P32K> {
   float xl, yl;
   ColOffsets co;
   Kahan SW, CW, Tx, Ty;
+  float psw, pcw;    // the current list group's partial sums
+  float mu_pv, wav;  // mu_pf and w_af in vector registers
   int n01, n2, k;
   const double *tx, *ty;
   __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P) {
@@ -389,6 +431,9 @@ struct VortAcc<P32K> {
     CW.init();
     Tx.init();
     Ty.init();
+    psw = pcw = 0.0f;
+    mu_pv = in_vgpr(P.mu_pf);
+    wav = in_vgpr(P.w_af);
     n01 = n2 = 0;
     k = kk;
     tx = t.x;
@@ -411,9 +456,16 @@ struct VortAcc<P32K> {
     pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
   }
   // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
-  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float, const KParams &P) {
-    pair_d(s, m, v, v.x + co.cdx(code, P.hxf), v.y + co.cdy(code, P.hyf), P);
+  template <int MODE>
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float,
+                                         const KParams &P) {
+    pair_d<MODE>(s, m, v, v.x + co.cdx(fx, P.hxf), v.y + co.cdy(fy, P.hyf), P);
   }
+  __device__ __forceinline__ void flush() {
+    SW.add(psw);
+    CW.add(pcw);
+  }
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
                                          const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
@@ -426,13 +478,18 @@ struct VortAcc<P32K> {
       inc = d <= P.xi_r;
     }
     float r = __builtin_amdgcn_sqrtf(r2);
-    float w_in = __builtin_fmaf(-P.mu_p_over_xa2f, r2, P.mu_pf);
+    float w_in = __builtin_fmaf(-P.mu_p_over_xa2f, r2, mu_pv);
     // c4 (r - xi_a)(rho - r) in Horner form (two FMAs; the same FP32 error bound)
-    float w_an = __builtin_fmaf(r, __builtin_fmaf(-P.c4f, r, P.w_af), P.w_bf);
+    float w_an = __builtin_fmaf(r, __builtin_fmaf(-P.c4f, r, wav), P.w_bf);
     float w = r2 <= P.xa2f ? w_in : w_an;
     w = inr ? w : 0.0f;
-    SW.add_prod(w, v.w);
-    CW.add_prod(w, v.z);
+    if constexpr (MODE == SUM_KAHAN) {
+      SW.add_prod(w, v.w);
+      CW.add_prod(w, v.z);
+    } else {
+      group_add_prod<MODE>(psw, w, v.w);
+      group_add_prod<MODE>(pcw, w, v.z);
+    }
     n01 += inr;
     if (inc) {
       if (r2 >= P.close2f) {
@@ -573,6 +630,7 @@ struct Chim2Acc<P32K> {
   ColOffsets co;
   double ai, n, in0, in1;
   Kahan C, S, Pc, Qs;
+  float pc, ps, ppc, pqs;  // the current list group's partial sums
   int k;
   const double *tx, *ty;
   __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P, const Pass2In &in) {
@@ -589,6 +647,7 @@ struct Chim2Acc<P32K> {
     S.init();
     Pc.init();
     Qs.init();
+    pc = ps = ppc = pqs = 0.0f;
     k = kk;
     tx = t.x;
     ty = t.y;
@@ -610,21 +669,36 @@ struct Chim2Acc<P32K> {
     sum(v, in, in ? s.aj[m] : 0.0f);
   }
   // a list entry with the partner's tile and pass-1 derivative already loaded
-  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float ajv,
+  template <int MODE>
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float ajv,
                                          const KParams &P) {
-    float dx = v.x + co.cdx(code, P.hxf), dy = v.y + co.cdy(code, P.hyf);
+    float dx = v.x + co.cdx(fx, P.hxf), dy = v.y + co.cdy(fy, P.hyf);
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f;
     if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
       in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
-    sum(v, in, in ? ajv : 0.0f);
+    sum<MODE>(v, in, in ? ajv : 0.0f);
   }
+  __device__ __forceinline__ void flush() {
+    C.add(pc);
+    S.add(ps);
+    Pc.add(ppc);
+    Qs.add(pqs);
+  }
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void sum(const float4 &v, bool in, float aj) {
     float cz = in ? v.z : 0.0f, sw = in ? v.w : 0.0f;
-    C.add(cz);
-    S.add(sw);
-    Pc.add(aj * cz);
-    Qs.add(aj * sw);
+    if constexpr (MODE == SUM_KAHAN) {
+      C.add(cz);
+      S.add(sw);
+      Pc.add(aj * cz);
+      Qs.add(aj * sw);
+    } else {
+      group_add<MODE>(pc, cz);
+      group_add<MODE>(ps, sw);
+      group_add_prod<MODE>(ppc, aj, cz);
+      group_add_prod<MODE>(pqs, aj, sw);
+    }
   }
   __device__ __forceinline__ void finish(const KParams &P, double phase, Pass2Out &o) const {
     double sa, ca;
@@ -811,10 +885,13 @@ struct Vort2Acc<P32K> {
     pair_t(s, m, v, v.x + ox, v.y + co.oy, 0.0f, false, P);
   }
   // a list entry with the partner's tile and pass-1 derivative already loaded
-  __device__ __forceinline__ void pair_c(const Src32 &s, int m, int code, const float4 &v, float ajv,
+  // (few entries reach the sums, which stay per entry: flush() has nothing to add)
+  template <int MODE>
+  __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float ajv,
                                          const KParams &P) {
-    pair_t(s, m, v, v.x + co.cdx(code, P.hxf), v.y + co.cdy(code, P.hyf), ajv, true, P);
+    pair_t(s, m, v, v.x + co.cdx(fx, P.hxf), v.y + co.cdy(fy, P.hyf), ajv, true, P);
   }
+  __device__ __forceinline__ void flush() {}
   __device__ __forceinline__ void pair_t(const Src32 &s, int m, const float4 &v, float dx, float dy, float ajv,
                                          bool have_aj, const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
@@ -1024,8 +1101,10 @@ __device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int
 // Pair counters: workgroup totals added to one of kCountSlots slot triples
 // (blockIdx modulo kCountSlots), so that the atomics of a launch spread over
 // many addresses; the host sums the slots.  Called by every thread.
-__device__ __forceinline__ void count_add(unsigned long long *counts, long long a, long long b, long long c) {
-  __shared__ long long part[3][kPairBlock / 64];
+// count_add_lds keeps the workgroup's partials in LDS the caller provides;
+// count_add declares its own.
+__device__ __forceinline__ void count_add_lds(long long (*part)[kPairBlock / 64], unsigned long long *counts,
+                                              long long a, long long b, long long c) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
     a += __shfl_xor(a, d, 64);
@@ -1045,6 +1124,11 @@ __device__ __forceinline__ void count_add(unsigned long long *counts, long long 
     for (int q = 0; q < kPairBlock / 64; ++q) v += part[threadIdx.x][q];
     atomicAdd(&counts[3 * (blockIdx.x % kCountSlots) + threadIdx.x], (unsigned long long)v);
   }
+}
+
+__device__ __forceinline__ void count_add(unsigned long long *counts, long long a, long long b, long long c) {
+  __shared__ long long part[3][kPairBlock / 64];
+  count_add_lds(part, counts, a, b, c);
 }
 
 

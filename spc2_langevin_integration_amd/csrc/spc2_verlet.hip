@@ -43,10 +43,11 @@ inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / 
 // four per lane, group g of lane p at int4 (wrow[w] + g) * 64 + p, so that a
 // wavefront reads 1 KiB per group load and a lane takes four entries per load.
 // Entry = (j << 4) | code, code = (dcx + 1) | (dcy + 1) << 2 for the stencil
-// cell (dcx, dcy) of partner j.  Every lane's column runs to the wavefront's
-// group count, padded with a valid entry (the home itself, never summed: the
-// passes stop at the count), and kVerletSpareGroups valid groups follow the
-// list, so the passes may read one group ahead without bounds.
+// cell (dcx, dcy) of partner j (the LDS layout's entries are EntLds).  Every
+// lane's column runs to the wavefront's group count, padded with a valid entry
+// (the home itself, never summed: the passes stop at the count; in the LDS
+// layout the halo's sentinel tile), and kVerletSpareGroups valid groups follow
+// the list, so the passes may read one group ahead without bounds.
 
 // Visits the candidates of home i in a 3 x 3 stencil of build cells (cells of
 // side >= rlist), ascending home within each cell, in the reference's stencil
@@ -334,6 +335,54 @@ __global__ __launch_bounds__(kBlock) void k_verlet_order(int n, const int *vcnt,
   if ((slot & 63) == 0) wmax_s[slot >> 6] = (mine + 3) >> 2;
 }
 
+// FP32 list entry formats.
+//   Gather lists: (j << 4) | (dcx + 1) | (dcy + 1) << 2, j a home, so that j << 4,
+//   the tile's byte offset, is one mask from the entry.  Entries past the count
+//   are skipped.
+//   LDS lists: (h << 16) | (dcy + 1) << 8 | (dcx + 1), h an index into the
+//   workgroup's staged halo (below 2^12, so that h << 4 is ent >> 12), with the
+//   stencil column and row one byte each, which converts to float in one
+//   instruction.  Padding entries name the sentinel tile staged after the halo,
+//   far from every particle, so that every entry of a group is summed unguarded.
+struct EntGather {
+  static constexpr bool kGuard = true;
+  __device__ static uint32_t off(int e) { return (uint32_t)e & ~15u; }
+  __device__ static float4 tile(const char *f4b, uint32_t off) { return *(const float4 *)(f4b + off); }
+  __device__ static int index(int e) { return e >> 4; }
+  __device__ static float fx(int e) { return (float)(e & 3); }
+  __device__ static float fy(int e) { return (float)((e >> 2) & 3); }
+};
+struct EntLds {
+  static constexpr bool kGuard = false;
+  __device__ static uint32_t off(int e) { return (uint32_t)e >> 12; }
+  // the staged tiles start at LDS address 0 (the passes have no static LDS), so
+  // the byte offset is the LDS address: no base is added per entry
+  __device__ static float4 tile(const char *, uint32_t off) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f x = *(const __attribute__((address_space(3))) v4f *)(uintptr_t)off;
+    return make_float4(x.x, x.y, x.z, x.w);
+  }
+  __device__ static int index(int e) { return (int)((uint32_t)e >> 16); }
+  __device__ static float fx(int e) { return (float)((uint32_t)e & 255u); }
+  __device__ static float fy(int e) { return (float)(((uint32_t)e >> 8) & 255u); }
+  // entry for halo index h and stencil code (dcx + 1) | (dcy + 1) << 2
+  __host__ __device__ static int make(int h, int code) { return (h << 16) | ((code >> 2) << 8) | (code & 3); }
+};
+constexpr float kSentinelCoord = 1.0e6f;  // sentinel tile x and y: r^2 ~ 2e12, beyond every cutoff
+static_assert(kLdsBytes / 20 < (1 << 12), "staged halo indices must fit EntLds's 12 bits");
+
+// An LDS pass workgroup's dynamic LDS: the staged tiles (16 B each, from
+// address 0), their homes (4 B), for pass 2 the pass-1 derivatives (4 B), then
+// this tail with the halo's runs and the pass-1 count partials.
+struct LdsTail {
+  int gst[kRunMax];      // first home of each run
+  int off[kRunMax + 1];  // LDS index of each run's first tile; off[n] = halo size
+  int n;                 // run count
+  long long part[3][kBlock / 64];
+};
+static_assert(sizeof(LdsTail) % 8 == 0, "LdsTail keeps the 8-byte alignment of its offset");
+constexpr size_t lds_pass_bytes(int cap, int pass) { return (size_t)cap * (pass == 2 ? 24 : 20) + sizeof(LdsTail); }
+
 // LDS index base of build cell c in workgroup b's halo: index = base + home.
 __device__ __forceinline__ int halo_base(const int *run, const int *off, int nrun, const int *start, int c) {
   int k = 0;
@@ -342,7 +391,8 @@ __device__ __forceinline__ int halo_base(const int *run, const int *off, int nru
 }
 
 // Build pass 2 for the LDS layout (FP32 form), per lane slot: as k_verlet_fill,
-// with the partner's index in the workgroup's staged halo in place of its home.
+// with EntLds entries (the partner's index in the workgroup's staged halo in
+// place of its home) and the halo's sentinel tile as padding.
 __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int *start, Tiles t, double rlist2,
                                                             const uint32_t *mask, int mwords, const int *vslot,
                                                             const int *wrow, const int *bh_nrun, const int *bh_run,
@@ -359,8 +409,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
   const int w = slot >> 6, groups = wrow[w + 1] - wrow[w];
   int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (slot & 63);
   const int c = t.cell[i], cx = c % P.nx, cy = c / P.nx;
-  const int self = ((halo_base(run, off, nrun, start, c) + i) << 4) | 5;
-  int4 buf = make_int4(self, self, self, self);
+  const int pad = EntLds::make(off[nrun], 5);  // the sentinel tile after the halo
+  int4 buf = make_int4(pad, pad, pad, pad);
   int e = 0, last = -1, base = 0;
   for_each_build_neighbour<P32K>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
     if (code != last) {
@@ -368,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
       base = halo_base(run, off, nrun, start, jc);
       last = code;
     }
-    const int ent = ((base + j) << 4) | code;
+    const int ent = EntLds::make(base + j, code);
     switch (e & 3) {
       case 0: buf.x = ent; break;
       case 1: buf.y = ent; break;
@@ -376,11 +426,11 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
       default:
         buf.w = ent;
         dst[(size_t)(e >> 2) * 64] = buf;
-        buf = make_int4(self, self, self, self);
+        buf = make_int4(pad, pad, pad, pad);
     }
     ++e;
   });
-  for (int g = (e + 3) >> 2; g < groups; ++g) dst[(size_t)g * 64] = make_int4(self, self, self, self);
+  for (int g = (e + 3) >> 2; g < groups; ++g) dst[(size_t)g * 64] = make_int4(pad, pad, pad, pad);
   if (e & 3) dst[(size_t)(e >> 2) * 64] = buf;
 }
 
@@ -446,9 +496,9 @@ __device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, con
 // FP32 list walk, a group of four entries in flight: the next group's entries
 // and the current group's partner tiles (and pass-2 derivatives) are requested
 // before the group is summed, in list order, so one memory latency covers four
-// entries.  An entry's upper bits (j << 4) are the byte offset of tile j, so a
-// gather is one mask from the entry, addressed from the uniform tile base.
-template <int PASS, class Acc>
+// entries.  Each group is summed into the accumulator's partials and flushed
+// into its compensated sums.
+template <int PASS, class F, class Acc>
 __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P) {
   if (cnt <= 0) return;
   const char *f4b = (const char *)src.f4;
@@ -460,16 +510,19 @@ __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l
     float a[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const uint32_t off = (uint32_t)e[u] & ~15u;
-      v[u] = *(const float4 *)(f4b + off);
+      const uint32_t off = F::off(e[u]);
+      v[u] = F::tile(f4b, off);
       a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
     }
     l += 64;
     g = *l;  // the next group (a spare group follows the list)
+    acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[0]), F::fx(e[0]), F::fy(e[0]), v[0], a[0], P);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (u == 0 || q + u < cnt) acc.pair_c(src, e[u] >> 4, e[u] & 15, v[u], a[u], P);
+    for (int u = 1; u < 4; ++u) {
+      if (!F::kGuard || q + u < cnt)
+        acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
     }
+    acc.flush();
   }
 }
 
@@ -516,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet(KParams P, const int *_
     cnt = vcnt[i];
     const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
     if constexpr (PREC == P32K)
-      walk32<1>(acc, src, l, cnt, P);
+      walk32<1, EntGather>(acc, src, l, cnt, P);
     else
       walk64(acc, src, l, cnt, P);
     acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
@@ -550,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   const int cnt = vcnt[i];
   const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
   if constexpr (PREC == P32K)
-    walk32<2>(acc, src, l, cnt, P);
+    walk32<2, EntGather>(acc, src, l, cnt, P);
   else
     walk64(acc, src, l, cnt, P);
   Pass2Out o;
@@ -558,31 +611,38 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   epilogue2<EPI>(P, e, i, hid[i], o);
 }
 
-// Stages workgroup seg's halo: FP32 tiles, their homes (for the FP64 re-checks)
-// and, for pass 2, the pass-1 phi derivatives.  Returns the LDS sources.
+// Stages workgroup seg's halo: FP32 tiles, their homes (for the FP64 re-checks),
+// for pass 2 the pass-1 phi derivatives, and the sentinel tile after the halo.
+// Returns the LDS sources; *tail is the dynamic area's LdsTail.
 template <int PASS>
 __device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, const int *bh_nrun,
                                             const int *bh_run, const int *bh_off, int seg, const Tiles &t,
-                                            const EpiCtx &e, void *dyn, int cap) {
-  __shared__ int s_gst[kRunMax], s_off[kRunMax + 1], s_n;
-  if (threadIdx.x == 0) s_n = bh_nrun[seg];
-  if (threadIdx.x <= kRunMax) {
-    const int k = threadIdx.x, nrun = bh_nrun[seg];
-    if (k <= nrun) s_off[k] = bh_off[(size_t)seg * (kRunMax + 1) + k];
-    if (k < nrun) s_gst[k] = start[bh_run[(size_t)seg * 2 * kRunMax + 2 * k]];
-  }
-  __syncthreads();
-  const int nrun = s_n, total = s_off[nrun];
+                                            const EpiCtx &e, void *dyn, int cap, LdsTail **tail) {
   float4 *f4l = (float4 *)dyn;
   int *slot = (int *)(f4l + cap);
   float *ajl = (float *)(slot + cap);
+  LdsTail *tl = (LdsTail *)(ajl + (PASS == 2 ? cap : 0));
+  *tail = tl;
+  if (threadIdx.x == 0) tl->n = bh_nrun[seg];
+  if (threadIdx.x <= kRunMax) {
+    const int k = threadIdx.x, nrun = bh_nrun[seg];
+    if (k <= nrun) tl->off[k] = bh_off[(size_t)seg * (kRunMax + 1) + k];
+    if (k < nrun) tl->gst[k] = start[bh_run[(size_t)seg * 2 * kRunMax + 2 * k]];
+  }
+  __syncthreads();
+  const int nrun = tl->n, total = tl->off[nrun];
   for (int el = threadIdx.x; el < total; el += kBlock) {
     int k = 0;
-    while (k + 1 < nrun && s_off[k + 1] <= el) ++k;
-    const int g = s_gst[k] + (el - s_off[k]);
+    while (k + 1 < nrun && tl->off[k + 1] <= el) ++k;
+    const int g = tl->gst[k] + (el - tl->off[k]);
     f4l[el] = t.f4[g];
     slot[el] = g;
     if (PASS == 2) ajl[el] = e.aphi_slot_f[g];
+  }
+  if (threadIdx.x == 0) {  // the sentinel tile that padding entries name (total < cap)
+    f4l[total] = make_float4(kSentinelCoord, kSentinelCoord, 0.0f, 0.0f);
+    slot[total] = 0;
+    if (PASS == 2) ajl[total] = 0.0f;
   }
   __syncthreads();
   return Src32{f4l, slot, PASS == 2 ? ajl : nullptr};
@@ -599,7 +659,8 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const in
                                                              unsigned long long *counts) {
   extern __shared__ float4 dyn_lds[];
   const int seg = xcd_segment(blockIdx.x, gridDim.x);
-  const Src32 src = stage_halo<1>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap);
+  LdsTail *tl;
+  const Src32 src = stage_halo<1>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap, &tl);
   const int slot = seg * kBlock + threadIdx.x;
   const bool active = slot < P.n;
   Pass1Out o{};
@@ -609,7 +670,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const in
     Acc acc;
     acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P);
     cnt = vcnt[i];
-    walk32<1>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), cnt, P);
+    walk32<1, EntLds>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), cnt, P);
     acc.finish(P, gp[i], o);
     const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
     if (EPI == EPI_K && e.next == 1) {
@@ -619,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const in
       write_stage<P32K>(P, e.tn, e.gpn, i, r.x, r.y, r.p);
     }
   }
-  if (counts) count_add(counts, cnt, active ? o.n : 0, active ? o.n2 : 0);
+  if (counts) count_add_lds(tl->part, counts, cnt, active ? o.n : 0, active ? o.n2 : 0);
 }
 
 template <class Acc, int EPI>
@@ -632,7 +693,8 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const in
                                                              const double *gp, EpiCtx e) {
   extern __shared__ float4 dyn_lds[];
   const int seg = xcd_segment(blockIdx.x, gridDim.x);
-  const Src32 src = stage_halo<2>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap);
+  LdsTail *tl;
+  const Src32 src = stage_halo<2>(P, start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap, &tl);
   const int slot = seg * kBlock + threadIdx.x;
   if (slot >= P.n) return;
   const int i = vslot ? vslot[slot] : slot;
@@ -643,7 +705,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const in
   }
   Acc acc;
   acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P, in);
-  walk32<2>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
+  walk32<2, EntLds>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
   Pass2Out o;
   acc.finish(P, gp[i], o);
   epilogue2<EPI>(P, e, i, hid[i], o);
@@ -689,7 +751,7 @@ void vpass1_typed(Sim &s, const EpiCtx &e) {
     if (s.vlds) {
       KParams kp = s.kp;
       kp.lds_cap = s.vlds_cap;
-      const size_t sh = (size_t)s.vlds_cap * 20;
+      const size_t sh = lds_pass_bytes(s.vlds_cap, 1);
       if (SYS == SPC2_CHIMERA)
         launch_pair(s, k_pass1_verlet_lds<ChimAcc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
                     (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
@@ -740,7 +802,7 @@ void vpass2_typed(Sim &s, const EpiCtx &e) {
     if (s.vlds) {
       KParams kp = s.kp;
       kp.lds_cap = s.vlds_cap;
-      const size_t sh = (size_t)s.vlds_cap * 24;
+      const size_t sh = lds_pass_bytes(s.vlds_cap, 2);
       if (SYS == SPC2_CHIMERA)
         launch_pair(s, k_pass2_verlet_lds<Chim2Acc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
                     (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
@@ -778,7 +840,36 @@ void vpass2_sys(Sim &s, int epi, const EpiCtx &e) {
   }
 }
 
+// True when every LDS pass kernel of one system has no static LDS, so that its
+// dynamic area starts at LDS address 0, where EntLds::tile reads the staged
+// tiles.
+template <class A1, class A2>
+bool lds_passes_unbased() {
+  const void *kern[] = {
+      reinterpret_cast<const void *>(&k_pass1_verlet_lds<A1, EPI_K>),
+      reinterpret_cast<const void *>(&k_pass1_verlet_lds<A1, EPI_ITO1>),
+      reinterpret_cast<const void *>(&k_pass1_verlet_lds<A1, EPI_SE>),
+      reinterpret_cast<const void *>(&k_pass1_verlet_lds<A1, EPI_RK4_FINAL>),
+      reinterpret_cast<const void *>(&k_pass2_verlet_lds<A2, EPI_ITO2>),
+      reinterpret_cast<const void *>(&k_pass2_verlet_lds<A2, EPI_SRK>),
+  };
+  for (const void *k : kern) {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, k) != hipSuccess || a.sharedSizeBytes != 0) return false;
+  }
+  return true;
+}
+
 }  // namespace
+
+bool verlet_lds_unbased(int system) {
+  if (system == SPC2_CHIMERA) {
+    static const bool ok = lds_passes_unbased<ChimAcc<P32K>, Chim2Acc<P32K>>();
+    return ok;
+  }
+  static const bool ok = lds_passes_unbased<VortAcc<P32K>, Vort2Acc<P32K>>();
+  return ok;
+}
 
 void launch_verlet_count(Sim &s) {
   const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
