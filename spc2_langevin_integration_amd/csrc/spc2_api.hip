@@ -70,6 +70,7 @@ void free_all(Sim &s) {
   f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
   f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb); f(s.vmask);
   f(s.bh_nrun); f(s.bh_run); f(s.bh_off); f(s.bh_max); f(s.vslot); f(s.wmax_s); f(s.wrow_s);
+  f(s.wmax8); f(s.wrow8);
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
 
@@ -197,6 +198,7 @@ spc2_status alloc_all(Sim &s) {
       A(s.bh_nrun, (size_t)s.nblk); A(s.bh_run, (size_t)s.nblk * 2 * kVerletRunMax);
       A(s.bh_off, (size_t)s.nblk * (kVerletRunMax + 1)); A(s.bh_max, 1);
       A(s.vslot, n); A(s.wmax_s, (size_t)s.nw + 1); A(s.wrow_s, (size_t)s.nw + 1);
+      A(s.wmax8, (size_t)s.nw + 1); A(s.wrow8, (size_t)s.nw + 1);
     }
   }
 #undef A
@@ -266,32 +268,37 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   // estimated from the mean density
   const double mean_list = (double)s.n / (s.P.Lx * s.P.Ly) * 3.14159265358979 * s.rlist * s.rlist;
   const bool want_sort = try_lds && (s.vsort_mode == 1 || (s.vsort_mode < 0 && mean_list > 72.0));
+  int *wrow_l = want_sort ? s.wrow_s : s.wrow8;  // the LDS layout's offsets (groups of eight)
   if (try_lds) {
     if (want_sort) {
       launch_verlet_order(s);
       launch_scan_array(s, s.wmax_s, s.nw, s.wrow_s);
+    } else {
+      launch_scan_array(s, s.wmax8, s.nw, s.wrow8);
     }
     HIP_TRY(hipMemsetAsync(s.bh_max, 0, sizeof(int), s.stream));
     launch_block_halo(s);
   }
-  int groups = 0, groups_s = 0, halo = 0;
+  int groups = 0, groups_l = 0, halo = 0;
   HIP_TRY(hipMemcpyAsync(&groups, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-  if (want_sort) HIP_TRY(hipMemcpyAsync(&groups_s, s.wrow_s + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-  if (try_lds) HIP_TRY(hipMemcpyAsync(&halo, s.bh_max, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  if (try_lds) {
+    HIP_TRY(hipMemcpyAsync(&groups_l, wrow_l + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipMemcpyAsync(&halo, s.bh_max, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  }
   HIP_TRY(hipStreamSynchronize(s.stream));
-  // pass 2 stages 24 B per halo particle, plus one sentinel tile after the halo
-  const int cap_limit = kLdsBytes / 24;
+  // pass 2 stages 20 B per halo particle, plus one sentinel tile after the halo
+  const int cap_limit = kLdsBytes / 20;
   s.vlds = try_lds && halo >= 0 && halo < cap_limit;  // halo is INT_MAX when a workgroup cannot be staged
   s.vlds_cap = s.vlds ? std::min(cap_limit, std::max(64, (halo + 1 + 63) / 64 * 64)) : 0;
   s.vsorted = s.vlds && want_sort;
-  if (s.vsorted) {  // the passes and the fill use the lane-slot layout
-    std::swap(s.wrow, s.wrow_s);
-    groups = groups_s;
+  if (s.vlds) {  // the passes and the fill use the LDS layout's offsets
+    std::swap(s.wrow, want_sort ? s.wrow_s : s.wrow8);
+    groups = groups_l;
   }
-  // groups = sum over wavefronts of their longest list in groups of four; more
-  // than 4096 entries per particle means a skin or density the list is not
-  // meant for
-  if (groups < 0 || (long long)groups * 4 > 4096LL * s.nw)
+  // groups = sum over wavefronts of their longest list in groups of four (LDS
+  // layout: eight); more than 4096 entries per particle means a skin or density
+  // the list is not meant for
+  if (groups < 0 || (long long)groups * (s.vlds ? 8 : 4) > 4096LL * s.nw)
     return fail(SPC2_ERR_INVALID, "Verlet list of " + std::to_string(groups) + " groups for " +
                                       std::to_string(s.nw) + " wavefronts: skin or density too large for a neighbour list");
   const size_t need = ((size_t)groups + kVerletSpareGroups) * 256;

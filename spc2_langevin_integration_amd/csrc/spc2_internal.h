@@ -136,12 +136,15 @@ struct Sim {
   // LDS-staged FP32 passes: per pass workgroup (kBlock homes) its halo as runs of
   // consecutive build cells (bh_run: first and last cell, bh_off: LDS offsets,
   // bh_nrun: run count or -1), bh_max: the largest halo; vlds when the current
-  // list holds halo indices
+  // list holds halo indices (eight 16-bit entries per group: wmax8 / wrow8 are
+  // the wavefront group counts and offsets in groups of eight)
   bool vlds = false, vlds_allowed = true;
   int nblk = 0, vlds_cap = 0;
   int *bh_nrun = nullptr, *bh_run = nullptr, *bh_off = nullptr, *bh_max = nullptr;
+  int *wmax8 = nullptr, *wrow8 = nullptr;
   // LDS layout lane slots (vslot[slot] = home) and their wavefront group counts
-  // and offsets; while vlds, wrow holds the slot layout's offsets
+  // and offsets (groups of eight); while vlds, wrow holds the offsets of the
+  // layout in use
   int *vslot = nullptr, *wmax_s = nullptr, *wrow_s = nullptr;
   bool vsorted = false;  // the current LDS-layout list uses the lane slots
   int vsort_mode = -1;   // -1: by list length, 0: never, 1: always (SPC2_VERLET_SORT)

@@ -25,13 +25,25 @@ namespace {
 constexpr int kPairBlock = 256;
 
 // ---- tile sources -------------------------------------------------------------
-// Src32 reads the FP32 tile (cell-local x, y and cos, sin of the phase); `slot`
-// maps an LDS index to its global slot (null: the index is the slot).
+// Src32 reads the FP32 tile (cell-local x, y and cos, sin of the phase).  The
+// global slot of tile index m: slot[m] when `slot` is set, else, when the tiles
+// are a staged halo of `rn` runs (run r's tiles from index roff[r] hold slots
+// rgst[r], rgst[r] + 1, ...), found from the runs, else m itself.
 struct Src32 {
   const float4 *f4;
   const int *slot;
   const float *aj;
-  __device__ __forceinline__ int gslot(int m) const { return slot ? slot[m] : m; }
+  const int *rgst = nullptr, *roff = nullptr;
+  int rn = 0;
+  __device__ __forceinline__ int gslot(int m) const {
+    if (slot) return slot[m];
+    if (rgst) {
+      int k = 0;
+      while (k + 1 < rn && roff[k + 1] <= m) ++k;
+      return rgst[k] + (m - roff[k]);
+    }
+    return m;
+  }
 };
 struct Src64 {
   const double *x, *y, *p, *c, *s, *aj;

@@ -149,10 +149,12 @@ __device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const
 }
 
 // Build pass 1: entries per home, their candidate bits and each wavefront's
-// group count (its longest list in groups of four).
+// group count (its longest list in groups of four; wgroups8, when set, in
+// groups of eight, for the LDS layout).
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *start, Tiles t, double rlist2,
-                                                         uint32_t *mask, int mwords, int *vcnt, int *wgroups) {
+                                                         uint32_t *mask, int mwords, int *vcnt, int *wgroups,
+                                                         int *wgroups8) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
   if (i < P.n) {
@@ -174,7 +176,10 @@ __global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *s
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
   const int w = i >> 6;
-  if ((threadIdx.x & 63) == 0 && w * 64 < P.n) wgroups[w] = (m + 3) >> 2;
+  if ((threadIdx.x & 63) == 0 && w * 64 < P.n) {
+    wgroups[w] = (m + 3) >> 2;
+    if (wgroups8) wgroups8[w] = (m + 7) >> 3;
+  }
 }
 
 // Build pass 2: the entries, in the order of the count, four per store, then
@@ -316,7 +321,8 @@ __global__ __launch_bounds__(64) void k_block_halo(KParams P, const int *start, 
 // takes a home, but the per-particle reads and writes of the epilogue scatter
 // over the workgroup's homes; the build uses the slots only where lists are
 // long (see verlet_build).  vslot[slot] = home (null: slot = home);
-// wmax_s[w] = wavefront w's group count, the length at its first slot.
+// wmax_s[w] = wavefront w's group count (groups of eight), from the length at
+// its first slot.
 __global__ __launch_bounds__(kBlock) void k_verlet_order(int n, const int *vcnt, int *vslot, int *wmax_s) {
   __shared__ int c[kBlock];
   const int base = blockIdx.x * kBlock, m = min(kBlock, n - base);
@@ -332,20 +338,20 @@ __global__ __launch_bounds__(kBlock) void k_verlet_order(int n, const int *vcnt,
   }
   const int slot = base + rank;
   vslot[slot] = base + me;
-  if ((slot & 63) == 0) wmax_s[slot >> 6] = (mine + 3) >> 2;
+  if ((slot & 63) == 0) wmax_s[slot >> 6] = (mine + 7) >> 3;
 }
 
 // FP32 list entry formats.
 //   Gather lists: (j << 4) | (dcx + 1) | (dcy + 1) << 2, j a home, so that j << 4,
-//   the tile's byte offset, is one mask from the entry.  Entries past the count
-//   are skipped.
-//   LDS lists: (h << 16) | (dcy + 1) << 8 | (dcx + 1), h an index into the
-//   workgroup's staged halo (below 2^12, so that h << 4 is ent >> 12), with the
-//   stencil column and row one byte each, which converts to float in one
-//   instruction.  Padding entries name the sentinel tile staged after the halo,
-//   far from every particle, so that every entry of a group is summed unguarded.
+//   the tile's byte offset, is one mask from the entry; four entries per lane
+//   and group.  Entries past the count are skipped.
+//   LDS lists: 16-bit entries (h << 4) | (dcx + 1) | (dcy + 1) << 2, h an index
+//   into the workgroup's staged halo (below 2^12), so that h << 4, the staged
+//   tile's LDS address, is one mask from the entry; eight entries per lane and
+//   group, two to a 32-bit word (the first in the low half).  Padding entries
+//   name the sentinel tile staged after the halo, far from every particle, so
+//   that every entry of a group is summed unguarded.
 struct EntGather {
-  static constexpr bool kGuard = true;
   __device__ static uint32_t off(int e) { return (uint32_t)e & ~15u; }
   __device__ static float4 tile(const char *f4b, uint32_t off) { return *(const float4 *)(f4b + off); }
   __device__ static int index(int e) { return e >> 4; }
@@ -353,27 +359,27 @@ struct EntGather {
   __device__ static float fy(int e) { return (float)((e >> 2) & 3); }
 };
 struct EntLds {
-  static constexpr bool kGuard = false;
-  __device__ static uint32_t off(int e) { return (uint32_t)e >> 12; }
+  __device__ static uint32_t off(uint32_t e) { return e & 0xfff0u; }
   // the staged tiles start at LDS address 0 (the passes have no static LDS), so
   // the byte offset is the LDS address: no base is added per entry
-  __device__ static float4 tile(const char *, uint32_t off) {
+  __device__ static float4 tile(uint32_t off) {
     typedef float v4f __attribute__((ext_vector_type(4)));
     const v4f x = *(const __attribute__((address_space(3))) v4f *)(uintptr_t)off;
     return make_float4(x.x, x.y, x.z, x.w);
   }
-  __device__ static int index(int e) { return (int)((uint32_t)e >> 16); }
-  __device__ static float fx(int e) { return (float)((uint32_t)e & 255u); }
-  __device__ static float fy(int e) { return (float)(((uint32_t)e >> 8) & 255u); }
+  __device__ static int index(uint32_t e) { return (int)((e >> 4) & 0xfffu); }
+  __device__ static float fx(uint32_t e) { return (float)(e & 3u); }
+  __device__ static float fy(uint32_t e) { return (float)((e >> 2) & 3u); }
   // entry for halo index h and stencil code (dcx + 1) | (dcy + 1) << 2
-  __host__ __device__ static int make(int h, int code) { return (h << 16) | ((code >> 2) << 8) | (code & 3); }
+  __host__ __device__ static uint32_t make(int h, int code) { return ((uint32_t)h << 4) | (uint32_t)code; }
 };
 constexpr float kSentinelCoord = 1.0e6f;  // sentinel tile x and y: r^2 ~ 2e12, beyond every cutoff
 static_assert(kLdsBytes / 20 < (1 << 12), "staged halo indices must fit EntLds's 12 bits");
 
 // An LDS pass workgroup's dynamic LDS: the staged tiles (16 B each, from
-// address 0), their homes (4 B), for pass 2 the pass-1 derivatives (4 B), then
-// this tail with the halo's runs and the pass-1 count partials.
+// address 0), for pass 2 the pass-1 derivatives (4 B each), then this tail with
+// the halo's runs (which also give a staged tile's home) and the pass-1 count
+// partials.
 struct LdsTail {
   int gst[kRunMax];      // first home of each run
   int off[kRunMax + 1];  // LDS index of each run's first tile; off[n] = halo size
@@ -381,7 +387,7 @@ struct LdsTail {
   long long part[3][kBlock / 64];
 };
 static_assert(sizeof(LdsTail) % 8 == 0, "LdsTail keeps the 8-byte alignment of its offset");
-constexpr size_t lds_pass_bytes(int cap, int pass) { return (size_t)cap * (pass == 2 ? 24 : 20) + sizeof(LdsTail); }
+constexpr size_t lds_pass_bytes(int cap, int pass) { return (size_t)cap * (pass == 2 ? 20 : 16) + sizeof(LdsTail); }
 
 // LDS index base of build cell c in workgroup b's halo: index = base + home.
 __device__ __forceinline__ int halo_base(const int *run, const int *off, int nrun, const int *start, int c) {
@@ -392,7 +398,7 @@ __device__ __forceinline__ int halo_base(const int *run, const int *off, int nru
 
 // Build pass 2 for the LDS layout (FP32 form), per lane slot: as k_verlet_fill,
 // with EntLds entries (the partner's index in the workgroup's staged halo in
-// place of its home) and the halo's sentinel tile as padding.
+// place of its home), eight per group, and the halo's sentinel tile as padding.
 __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int *start, Tiles t, double rlist2,
                                                             const uint32_t *mask, int mwords, const int *vslot,
                                                             const int *wrow, const int *bh_nrun, const int *bh_run,
@@ -409,8 +415,10 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
   const int w = slot >> 6, groups = wrow[w + 1] - wrow[w];
   int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (slot & 63);
   const int c = t.cell[i], cx = c % P.nx, cy = c / P.nx;
-  const int pad = EntLds::make(off[nrun], 5);  // the sentinel tile after the halo
-  int4 buf = make_int4(pad, pad, pad, pad);
+  const uint32_t pad = EntLds::make(off[nrun], 5);  // the sentinel tile after the halo
+  const int pad2 = (int)(pad | pad << 16);
+  uint32_t word = 0;
+  int4 buf = make_int4(pad2, pad2, pad2, pad2);
   int e = 0, last = -1, base = 0;
   for_each_build_neighbour<P32K>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
     if (code != last) {
@@ -418,20 +426,26 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
       base = halo_base(run, off, nrun, start, jc);
       last = code;
     }
-    const int ent = EntLds::make(base + j, code);
-    switch (e & 3) {
-      case 0: buf.x = ent; break;
-      case 1: buf.y = ent; break;
-      case 2: buf.z = ent; break;
-      default:
-        buf.w = ent;
-        dst[(size_t)(e >> 2) * 64] = buf;
-        buf = make_int4(pad, pad, pad, pad);
+    const uint32_t ent = EntLds::make(base + j, code);
+    if ((e & 1) == 0) {
+      word = ent | pad << 16;
+    } else {
+      word = (word & 0xffffu) | ent << 16;
+    }
+    switch ((e >> 1) & 3) {
+      case 0: buf.x = (int)word; break;
+      case 1: buf.y = (int)word; break;
+      case 2: buf.z = (int)word; break;
+      default: buf.w = (int)word;
+    }
+    if ((e & 7) == 7) {
+      dst[(size_t)(e >> 3) * 64] = buf;
+      buf = make_int4(pad2, pad2, pad2, pad2);
     }
     ++e;
   });
-  for (int g = (e + 3) >> 2; g < groups; ++g) dst[(size_t)g * 64] = make_int4(pad, pad, pad, pad);
-  if (e & 3) dst[(size_t)(e >> 2) * 64] = buf;
+  for (int g = (e + 7) >> 3; g < groups; ++g) dst[(size_t)g * 64] = make_int4(pad2, pad2, pad2, pad2);
+  if (e & 7) dst[(size_t)(e >> 3) * 64] = buf;
 }
 
 // Per-RHS stage: s + (k_coef dt) k by home, the FP64 tile (conditionally
@@ -493,34 +507,79 @@ __device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, con
   acc.pair(src, j, P);
 }
 
-// FP32 list walk, a group of four entries in flight: the next group's entries
-// and the current group's partner tiles (and pass-2 derivatives) are requested
-// before the group is summed, in list order, so one memory latency covers four
-// entries.  Each group is summed into the accumulator's partials and flushed
-// into its compensated sums.
-template <int PASS, class F, class Acc>
+// FP32 list walk over a gather list, a group of four entries in flight: the
+// next group's entries and the current group's partner tiles (and pass-2
+// derivatives) are requested before the group is summed, in list order, so one
+// memory latency covers four entries.  Every eight entries are summed into the
+// accumulator's partials and flushed into its compensated sums, as walk_lds
+// does, so that both layouts give the same sums.
+template <int PASS, class Acc>
 __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P) {
+  using F = EntGather;
   if (cnt <= 0) return;
   const char *f4b = (const char *)src.f4;
   const char *ajb = (const char *)src.aj;
   int4 g = *l;
-  for (int q = 0; q < cnt; q += 4) {
-    const int e[4] = {g.x, g.y, g.z, g.w};
-    float4 v[4];
-    float a[4];
+  for (int q = 0; q < cnt; q += 8) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t off = F::off(e[u]);
-      v[u] = F::tile(f4b, off);
-      a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && q + 4 >= cnt) break;
+      const int e[4] = {g.x, g.y, g.z, g.w};
+      float4 v[4];
+      float a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t off = F::off(e[u]);
+        v[u] = F::tile(f4b, off);
+        a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
+      }
+      l += 64;
+      g = *l;  // the next group (a spare group follows the list)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (h == 0 && u == 0)
+          acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+        else if (q + 4 * h + u < cnt)
+          acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+      }
     }
+    acc.flush();
+  }
+}
+
+// FP32 list walk over an LDS list: groups of eight 16-bit entries, taken as two
+// halves of four whose staged tiles (and pass-2 derivatives) are requested
+// before the half is summed; the next group's entries are requested before the
+// current group is summed.  Sentinel padding is summed like any entry (it adds
+// zero), and each group's partials are flushed once.
+template <int PASS, class Acc>
+__device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P) {
+  using F = EntLds;
+  if (cnt <= 0) return;
+  const char *ajb = (const char *)src.aj;
+  int4 g = *l;
+  for (int q = 0; q < cnt; q += 8) {
+    const uint32_t w[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
     l += 64;
     g = *l;  // the next group (a spare group follows the list)
-    acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[0]), F::fx(e[0]), F::fy(e[0]), v[0], a[0], P);
 #pragma unroll
-    for (int u = 1; u < 4; ++u) {
-      if (!F::kGuard || q + u < cnt)
-        acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t e[4] = {w[2 * h] & 0xffffu, w[2 * h] >> 16, w[2 * h + 1] & 0xffffu, w[2 * h + 1] >> 16};
+      float4 v[4];
+      float a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t off = F::off(e[u]);
+        v[u] = F::tile(off);
+        a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (h == 0 && u == 0)
+          acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+        else
+          acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+      }
     }
     acc.flush();
   }
@@ -569,7 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet(KParams P, const int *_
     cnt = vcnt[i];
     const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
     if constexpr (PREC == P32K)
-      walk32<1, EntGather>(acc, src, l, cnt, P);
+      walk32<1>(acc, src, l, cnt, P);
     else
       walk64(acc, src, l, cnt, P);
     acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
@@ -603,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   const int cnt = vcnt[i];
   const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
   if constexpr (PREC == P32K)
-    walk32<2, EntGather>(acc, src, l, cnt, P);
+    walk32<2>(acc, src, l, cnt, P);
   else
     walk64(acc, src, l, cnt, P);
   Pass2Out o;
@@ -611,16 +670,16 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   epilogue2<EPI>(P, e, i, hid[i], o);
 }
 
-// Stages workgroup seg's halo: FP32 tiles, their homes (for the FP64 re-checks),
-// for pass 2 the pass-1 phi derivatives, and the sentinel tile after the halo.
-// Returns the LDS sources; *tail is the dynamic area's LdsTail.
+// Stages workgroup seg's halo: FP32 tiles, for pass 2 the pass-1 phi
+// derivatives, and the sentinel tile after the halo.  Returns the LDS sources
+// (a tile's home, for the FP64 re-checks, comes from the runs); *tail is the
+// dynamic area's LdsTail.
 template <int PASS>
 __device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, const int *bh_nrun,
                                             const int *bh_run, const int *bh_off, int seg, const Tiles &t,
                                             const EpiCtx &e, void *dyn, int cap, LdsTail **tail) {
   float4 *f4l = (float4 *)dyn;
-  int *slot = (int *)(f4l + cap);
-  float *ajl = (float *)(slot + cap);
+  float *ajl = (float *)(f4l + cap);
   LdsTail *tl = (LdsTail *)(ajl + (PASS == 2 ? cap : 0));
   *tail = tl;
   if (threadIdx.x == 0) tl->n = bh_nrun[seg];
@@ -636,16 +695,14 @@ __device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, 
     while (k + 1 < nrun && tl->off[k + 1] <= el) ++k;
     const int g = tl->gst[k] + (el - tl->off[k]);
     f4l[el] = t.f4[g];
-    slot[el] = g;
     if (PASS == 2) ajl[el] = e.aphi_slot_f[g];
   }
   if (threadIdx.x == 0) {  // the sentinel tile that padding entries name (total < cap)
     f4l[total] = make_float4(kSentinelCoord, kSentinelCoord, 0.0f, 0.0f);
-    slot[total] = 0;
     if (PASS == 2) ajl[total] = 0.0f;
   }
   __syncthreads();
-  return Src32{f4l, slot, PASS == 2 ? ajl : nullptr};
+  return Src32{f4l, nullptr, PASS == 2 ? ajl : nullptr, tl->gst, tl->off, nrun};
 }
 
 template <class Acc, int EPI>
@@ -670,7 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const in
     Acc acc;
     acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P);
     cnt = vcnt[i];
-    walk32<1, EntLds>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), cnt, P);
+    walk_lds<1>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), cnt, P);
     acc.finish(P, gp[i], o);
     const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
     if (EPI == EPI_K && e.next == 1) {
@@ -705,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const in
   }
   Acc acc;
   acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P, in);
-  walk32<2, EntLds>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
+  walk_lds<2>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
   Pass2Out o;
   acc.finish(P, gp[i], o);
   epilogue2<EPI>(P, e, i, hid[i], o);
@@ -877,15 +934,15 @@ void launch_verlet_count(Sim &s) {
   switch (s.prec) {
     case P64:
       hipLaunchKernelGGL(k_verlet_count<P64>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
-                         s.vmask_words, s.vcnt, s.wmax);
+                         s.vmask_words, s.vcnt, s.wmax, s.wmax8);
       break;
     case P64F:
       hipLaunchKernelGGL(k_verlet_count<P64F>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
-                         s.vmask_words, s.vcnt, s.wmax);
+                         s.vmask_words, s.vcnt, s.wmax, s.wmax8);
       break;
     default:
       hipLaunchKernelGGL(k_verlet_count<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
-                         s.vmask_words, s.vcnt, s.wmax);
+                         s.vmask_words, s.vcnt, s.wmax, s.wmax8);
   }
 }
 
