@@ -340,6 +340,25 @@ spc2_status verlet_measure(Sim &s) {
 // state: stage positions are within drift + reach of the build positions, and
 // every pair within rho at the stage was within rho + skin at the build while
 // that stays <= skin / 2.
+// Largest distance one SRK step can move a particle when the noise comes from
+// the built-in generator (StochasticRungeKuttaStepperPtr.hpp:52-66 update):
+//   v0 dt                                    the drift,
+// + sqrt(2 D) v0 |dZ|, dZ = h1 (U1 + U2 / sqrt(3)), |U| <= 8.6 (Box-Muller's
+//   radius at its smallest uniform, 2^-53, is 8.572),
+// + dt^2 / 2 v0 (|a_phi| + D), a_phi the pass-1 phase derivative: |a_phi| <=
+//   |sigma| (chimera: sigma times a mean of sines) or max(|mu+|, |mu-|) +
+//   |kappa| (vortex arrays: a mean of W sin, |W| <= max(|mu+|, |mu-|), plus the
+//   cohesion mean), with 0.1% for rounding.
+// Supplied noise has no such bound: its steps are measured instead.
+double srk_step_reach(const Sim &s, double dt) {
+  const double v0 = std::fabs(s.P.v0), D = std::fabs(s.P.D);
+  const double a_phi = 1.001 * (s.P.system == SPC2_CHIMERA
+                                    ? std::fabs(s.P.sigma)
+                                    : std::max(std::fabs(s.P.mu_plus), std::fabs(s.P.mu_minus)) + std::fabs(s.P.kappa));
+  const double dZ = 0.5 * dt * std::sqrt(dt) * 8.6 * (1.0 + 1.0 / std::sqrt(3.0));
+  return v0 * (dt + std::sqrt(2.0 * D) * dZ + 0.5 * dt * dt * (a_phi + D));
+}
+
 spc2_status verlet_ensure(Sim &s, double reach, bool k_xy_live) {
   const double half = 0.5 * s.skin * (1.0 - 1e-9);
   if (s.vbuilt && s.drift + reach <= half) return SPC2_OK;
@@ -396,8 +415,9 @@ void maybe_rehome(Sim &s) {
   s.steps_since_rehome = 0;
 }
 
-// Verlet: the next pass-1 launch also writes the next stage's tiles (mode 1:
-// s + kcdt k of that launch; 2: the state it writes).
+// Verlet: the next pass launch also writes the next stage's tiles (mode 1:
+// s + kcdt k of that pass-1 launch; 2: the state the launch writes, by pass 1
+// of SE and RK4's last stage or by SRK's pass 2).
 void next_stage(Sim &s, int mode, double kcdt) {
   s.vnext = mode;
   s.vnext_kcdt = kcdt;
@@ -674,10 +694,11 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
     StepArgs sa = step_args(s, dt, (uint64_t)(step0 + k));
     sa.wrap = (flags & SPC2_STEP_NO_WRAP) ? 0 : 1;
     if (noise && method != SPC2_RK4) sa.noise = s.noise + (size_t)k * npp * (size_t)s.n;
-    // Verlet SE and RK4: each pass-1 launch also writes the tiles of the next
-    // stage (the next step's first stage after the update), so a separate
-    // stage pass is needed only after a list build or another call
-    const bool fused = s.verlet && method != SPC2_SRK;
+    // Verlet: each pass-1 launch of SE and RK4, and SRK's pass-2 launch, also
+    // writes the tiles of the next stage (the next step's first stage after the
+    // update), so a separate stage pass is needed only after a list build or
+    // another call
+    const bool fused = s.verlet;
     if (fused && !s.tile_fresh) build(s, nullptr, 0.0);
     if (method == SPC2_SE) {
       if (fused) {
@@ -687,9 +708,10 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
       }
       rhs_pass1(s, EPI_SE, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
     } else if (method == SPC2_SRK) {
-      build(s, nullptr, 0.0);
+      if (!fused) build(s, nullptr, 0.0);
       sa.zero_in = 1;  // derivative zero-filled each step (StochasticRungeKuttaStepperPtr.hpp:43-46)
       rhs_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
+      if (fused) next_stage(s, 2, 0.0);
       rhs_pass2(s, EPI_SRK, nullptr, nullptr, sa);
     } else {
       // RungeKutta4StepperPtr.cpp:25-43: k1 = f(s, k1, 0), k2 = f(s, k1, 1/2),
@@ -717,12 +739,13 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
     }
     if (fused) s.tile_fresh = true;
     if (s.verlet) {
-      // SE and RK4 move a particle by at most v0 dt per step; SRK adds noise to
-      // the positions, so its displacement is measured before the next use
-      if (method == SPC2_SRK)
+      // SE and RK4 move a particle by at most v0 dt per step, SRK by at most
+      // srk_step_reach with generated noise; with supplied noise SRK's
+      // displacement is measured before the next use
+      if (method == SPC2_SRK && sa.noise)
         s.drift = std::numeric_limits<double>::infinity();
       else
-        s.drift += std::fabs(s.P.v0) * dt * (1.0 + 1e-12);
+        s.drift += (method == SPC2_SRK ? srk_step_reach(s, dt) : std::fabs(s.P.v0) * dt) * (1.0 + 1e-12);
     }
     maybe_rehome(s);
     if (spc2_status st = check_launch("step")) return st;

@@ -1079,7 +1079,8 @@ __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e,
 }
 
 template <int EPI>
-__device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int h, int id, const Pass2Out &o) {
+__device__ __forceinline__ EpiState epilogue2(const KParams &P, const EpiCtx &e, int h, int id, const Pass2Out &o) {
+  EpiState r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (EPI == EPI_ITO2) {
     e.a0[h] = o.a0;
     e.a1[h] = o.a1;
@@ -1107,7 +1108,11 @@ __device__ __forceinline__ void epilogue2(const KParams &P, const EpiCtx &e, int
     e.sx[h] = x;
     e.sy[h] = y;
     e.sp[h] = p;
+    r.x = x;
+    r.y = y;
+    r.p = p;
   }
+  return r;
 }
 
 // Pair counters: workgroup totals added to one of kCountSlots slot triples

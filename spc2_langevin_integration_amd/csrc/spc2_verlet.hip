@@ -667,7 +667,8 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
     walk64(acc, src, l, cnt, P);
   Pass2Out o;
   acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
-  epilogue2<EPI>(P, e, i, hid[i], o);
+  const EpiState r = epilogue2<EPI>(P, e, i, hid[i], o);
+  if (EPI == EPI_SRK && e.next == 2) write_stage<PREC>(P, e.tn, e.gpn, i, r.x, r.y, r.p);  // the next step's stage
 }
 
 // Stages workgroup seg's halo: FP32 tiles, for pass 2 the pass-1 phi
@@ -765,7 +766,8 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const in
   walk_lds<2>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
   Pass2Out o;
   acc.finish(P, gp[i], o);
-  epilogue2<EPI>(P, e, i, hid[i], o);
+  const EpiState r = epilogue2<EPI>(P, e, i, hid[i], o);
+  if (EPI == EPI_SRK && e.next == 2) write_stage<P32K>(P, e.tn, e.gpn, i, r.x, r.y, r.p);  // the next step's stage
 }
 
 // Largest squared displacement since the build (min image), two-level and
@@ -998,9 +1000,11 @@ void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const doubl
   }
 }
 
-void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
-                         double *const k2[3], double *const k3[3]) {
-  EpiCtx e = make_ctx(s, ox, oy, op, sa, k1, k2, k3);
+namespace {
+// A pass launch that writes the next stage's tiles (s.vnext, set by
+// next_stage): the launch context names the second tile set, and afterwards the
+// written tiles become the current ones (cells are shared).
+int take_next(Sim &s, EpiCtx &e) {
   const int next = s.vnext;
   s.vnext = 0;
   if (next) {
@@ -1009,31 +1013,42 @@ void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, co
     e.kcdt_n = s.vnext_kcdt;
     e.next = next;
   }
+  return next;
+}
+void swap_tiles(Sim &s) {
+  std::swap(s.t.x, s.t2.x);
+  std::swap(s.t.y, s.t2.y);
+  std::swap(s.t.p, s.t2.p);
+  std::swap(s.t.c, s.t2.c);
+  std::swap(s.t.s, s.t2.s);
+  std::swap(s.t.f4, s.t2.f4);
+  std::swap(s.gp, s.gp2);
+}
+}  // namespace
+
+void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
+                         double *const k2[3], double *const k3[3]) {
+  EpiCtx e = make_ctx(s, ox, oy, op, sa, k1, k2, k3);
+  const int next = take_next(s, e);
   switch (s.prec) {
     case P64: vpass1_sys<P64>(s, epi, e); break;
     case P64F: vpass1_sys<P64F>(s, epi, e); break;
     default: vpass1_sys<P32K>(s, epi, e);
   }
-  if (next) {  // the written tiles become the current ones (cells are shared)
-    std::swap(s.t.x, s.t2.x);
-    std::swap(s.t.y, s.t2.y);
-    std::swap(s.t.p, s.t2.p);
-    std::swap(s.t.c, s.t2.c);
-    std::swap(s.t.s, s.t2.s);
-    std::swap(s.t.f4, s.t2.f4);
-    std::swap(s.gp, s.gp2);
-  }
+  if (next) swap_tiles(s);
 }
 
 void launch_verlet_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa) {
   EpiCtx e = make_ctx(s, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
   e.a0 = a0;
   e.a1 = a1;
+  const int next = take_next(s, e);
   switch (s.prec) {
     case P64: vpass2_sys<P64>(s, epi, e); break;
     case P64F: vpass2_sys<P64F>(s, epi, e); break;
     default: vpass2_sys<P32K>(s, epi, e);
   }
+  if (next) swap_tiles(s);
 }
 
 void launch_drift(Sim &s, double *out) {
