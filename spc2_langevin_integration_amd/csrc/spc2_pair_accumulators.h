@@ -1000,12 +1000,14 @@ __device__ __forceinline__ void wrap_state(const KParams &P, double &x, double &
   y = y - floor(y * P.rLy) * P.Ly;
 }
 
-// Final phi derivative given the in/out value `old` (k_next or derivative entry
-// before the call): the reference writes it only when partners exist.
-__device__ __forceinline__ double phi_out(const Pass1Out &o, double old) {
+// Final phi derivative given the in/out value (k_next or derivative entry
+// before the call, *old, or 0 when old is null): the reference writes it only
+// when partners exist, so the value is read only for a particle without them.
+__device__ __forceinline__ double phi_out(const Pass1Out &o, const double *old) {
   if (o.written) return o.aphi;
-  if (o.vortex_add) return old + o.coh;
-  return old;
+  const double v = old ? *old : 0.0;
+  if (o.vortex_add) return v + o.coh;
+  return v;
 }
 
 template <int EPI>
@@ -1021,7 +1023,7 @@ __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e,
   double ax = P.v0 * cs, ay = P.v0 * sn;
   EpiState r{ax, ay, 0.0, 0.0, 0.0, 0.0};
   if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
-    double ap = phi_out(o, e.sa.zero_in ? 0.0 : e.op[h]);
+    double ap = phi_out(o, e.sa.zero_in ? nullptr : e.op + h);
     r.ap = ap;
     e.ox[h] = ax;
     e.oy[h] = ay;
@@ -1051,7 +1053,7 @@ __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e,
   } else if (EPI == EPI_SE) {
     // StochasticEulerStepperPtr.hpp:44-53 with a zero-filled derivative and the
     // natural frequency in the drift as (omega + a_phi) dt
-    double ap = phi_out(o, 0.0);
+    double ap = phi_out(o, nullptr);
     double xi;
     if (e.sa.noise) {
       xi = e.sa.noise[id];  // materialised noise is indexed by particle id

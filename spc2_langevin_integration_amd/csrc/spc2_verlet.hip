@@ -549,8 +549,8 @@ __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l
 
 // FP32 list walk over an LDS list: groups of eight 16-bit entries, taken as two
 // halves of four whose staged tiles (and pass-2 derivatives) are requested
-// before the half is summed; the next group's entries are requested before the
-// current group is summed.  Sentinel padding is summed like any entry (it adds
+// before the half is summed; the lane's next group of entries is requested
+// before the current group is summed.  Sentinel padding is summed like any entry (it adds
 // zero), and each group's partials are flushed once.
 template <int PASS, class Acc>
 __device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P) {
@@ -561,7 +561,7 @@ __device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 
   for (int q = 0; q < cnt; q += 8) {
     const uint32_t w[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
     l += 64;
-    g = *l;  // the next group (a spare group follows the list)
+    if (q + 8 < cnt) g = *l;  // the lane's next group, if it has one
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t e[4] = {w[2 * h] & 0xffffu, w[2 * h] >> 16, w[2 * h + 1] & 0xffffu, w[2 * h + 1] >> 16};
