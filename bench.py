@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # <VortAcc<P32K>>) on this workload (skin 0.125 rho), from the FETCH_SIZE (x2.000)
 # and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
 # copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 376.2e6
+TRAFFIC_VERLET_FP32K = 257.4e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -83,6 +83,36 @@ def time_precision(spc2, w, precision, steps, warmup, neighbour):
             "phases": phases, "list_builds": builds}
 
 
+def fixed_bytes_per_particle(w, precision, neighbour):
+    """Algorithmic HBM bytes per particle and pair-kernel launch, beside the
+    list: what each launch must read and write once per particle (neighbour
+    tiles are counted once, as the particle's own tile; their reuse comes
+    from LDS and cache).
+
+    Vortex RK4, every stage: own tile (FP32: x, y, cos, sin 16 B + FP64 stage
+    phase 8 B; FP64: 24 B), list count 4 B, k written 24 B.  The last stage
+    (a quarter of the launches) also reads k1..k3 (72 B) and reads and writes
+    the state (48 B).  Verlet lists fuse the next stage's tiles into the
+    epilogue: written (FP32: 16 B tile + 16 B FP64 position + 8 B phase; FP64:
+    24 B + 8 B), with the build cell (4 B) and, in stages 1-3, the state
+    (24 B) read.
+
+    Chimera SRK, the mean of its two passes: pass 1 reads the own tile 24 B,
+    count and id 8 B and writes the derivative 24 B and the pass-2 inputs 28 B
+    (84 B); pass 2 reads the tile 24 B, count and id 8 B, its pass-1 inputs
+    24 B and the staged pass-1 derivative 4 B, omega 8 B, the state 24 B, and
+    writes the state 24 B and (Verlet) the next step's tiles 40 B with the
+    build cell 4 B read (160 B)."""
+    fp32 = precision == "fp32k"
+    if w["system"] == "chimera":
+        return (84 + 160) / 2.0
+    own = 24
+    b = own + 4 + 24 + (72 + 48) / 4.0
+    if neighbour == "verlet":
+        b += (40 if fp32 else 32) + 4 + 24 * 3 / 4.0
+    return b
+
+
 def profile_pair_kernel(spc2, w, precision, steps, neighbour):
     """HIP-event timing of the pair kernels on the handle's stream, with pair counts."""
     s = make_sim(spc2, w, precision, neighbour)
@@ -94,6 +124,7 @@ def profile_pair_kernel(spc2, w, precision, steps, neighbour):
     s.set_counting(True)             # counting pass over the same number of launches
     s.step(w["method"], w["dt"], 2 + steps, steps)
     cnt = s.counts()
+    lds = neighbour == "verlet" and s.verlet_layout()[0] == 1
     s.close()
     per = cnt / launches
     if w["system"] == "chimera":
@@ -107,23 +138,14 @@ def profile_pair_kernel(spc2, w, precision, steps, neighbour):
         flops = (FLOP_CANDIDATE * per[0] + FLOP_VORTEX_PAIR * per[1] + FLOP_COHESION * per[2] +
                  FLOP_EPILOGUE * w["n"])
     avg_s = ms / launches / 1e3
-    # algorithmic HBM bytes per launch, per particle: own tile (FP32 16 B; FP64
-    # 24 B) + FP64 stage phase 8 B + home->id and counts 8 B + k written 24 B;
-    # the last RK4 stage also reads k1..k3 (72 B) and reads and writes the state
-    # (48 B), a quarter of the launches; Verlet: 4 B per list entry (neighbour
-    # tiles are re-read from cache, not HBM), the phi derivative read back (in/out,
-    # 8 B) and the next stage's tiles written by the epilogue (FP32: tile 16 B +
-    # FP64 position 16 B + phase 8 B; FP64: 24 B + phase 8 B) from the state
-    # (24 B, stages 1-3). Chimera: both SRK passes read the whole list.
-    own = 16 if precision == "fp32k" else 24
-    list_b = 4.0 * per[0] if neighbour == "verlet" else 0.0
-    fused = ((8 + (40 if precision == "fp32k" else 32) + 24 * 3 / 4.0)
-             if neighbour == "verlet" and w["method"] == "rk4" else 0.0)
-    bytes_per = w["n"] * (own + 8 + 8 + 24 + (72 + 48) / 4.0 + fused) + list_b
+    # list: 2 B per entry in the LDS layout (16-bit halo indices), 4 B in the
+    # gather layout, once per launch (both SRK passes read it)
+    entry_b = (2.0 if lds else 4.0) if neighbour == "verlet" else 0.0
+    bytes_per = w["n"] * fixed_bytes_per_particle(w, precision, neighbour) + entry_b * per[0]
     return {"avg_ms": ms / launches, "launches": launches, "flops_per_launch": flops,
             "tflops": flops / avg_s / 1e12, "bytes_per_launch": bytes_per, "gbs": bytes_per / avg_s / 1e9,
             "candidates_per_particle": per[0] / w["n"], "neighbours_per_particle": per[1] / w["n"],
-            "cohesion_per_particle": per[2] / w["n"]}
+            "cohesion_per_particle": per[2] / w["n"], "lds_layout": lds}
 
 
 def error_horizon(spc2, w, nb, horizons):
