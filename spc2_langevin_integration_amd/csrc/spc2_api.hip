@@ -142,6 +142,9 @@ void fill_kparams_from(KParams &k, const spc2_params &P, int n, int nx, int ny, 
   k.band_rho2f = (float)(1e-5 * k.rho2);
   k.band_xr2f = (float)(1e-5 * P.xi_r * P.xi_r);
   k.band_xa2f = (float)(1e-5 * P.xi_a * P.xi_a);
+  k.rho2f_lo = k.rho2f - k.band_rho2f;
+  k.rho2f_hi = k.rho2f + k.band_rho2f;
+  k.xr2f_hi = k.xr2f + k.band_xr2f;
   const double close = 1e-2 * std::min(k.hx, k.hy);
   k.close2f = (float)(close * close);
 }

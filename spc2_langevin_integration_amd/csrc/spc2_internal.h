@@ -47,6 +47,7 @@ struct KParams {
   float w_af, w_bf;
   // FP32 cutoff band: |r^2 - cutoff^2| <= band * cutoff^2 is re-decided in FP64
   float band_rho2f, band_xr2f, band_xa2f;
+  float rho2f_lo, rho2f_hi, xr2f_hi;  // rho2f -/+ band_rho2f, xr2f + band_xr2f
   // FP32 cohesion pairs closer than this (r^2) take FP64 geometry
   float close2f;
   // pair-kernel decomposition: row segments of seg_w cells, segs per row,

@@ -287,9 +287,8 @@ struct ChimAcc<P32K> {
   __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
                                          const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool in = r2 <= P.rho2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
-      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    bool in = r2 <= P.rho2f_hi;  // outside the band, the same as r2 <= rho2f
+    if ((r2 >= P.rho2f_lo) & in) in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
     if constexpr (MODE == SUM_KAHAN) {
       S.add(in ? v.w : 0.0f);
       C.add(in ? v.z : 0.0f);
@@ -481,14 +480,11 @@ struct VortAcc<P32K> {
   __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
                                          const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool inr = r2 <= P.rho2f, inc = r2 <= P.xr2f;
-    // one branch for either band (no short circuit: both tests are cheaper than a branch)
-    if ((__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f) | (__builtin_fabsf(r2 - P.xr2f) <= P.band_xr2f)) {
-      // the reference tests sqrt(r^2) <= cutoff in FP64 (VortexArraysSystemPtr.cpp:473-488)
-      double d = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy));
-      inr = d <= P.rho;
-      inc = d <= P.xi_r;
-    }
+    // Outside the rho band the FP32 test decides (r2 <= rho2 + band there means
+    // r2 <= rho2 - band); inside it the reference's FP64 test sqrt(r^2) <= rho
+    // does (VortexArraysSystemPtr.cpp:473-488).
+    bool inr = r2 <= P.rho2f_hi;
+    if ((r2 >= P.rho2f_lo) & inr) inr = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy)) <= P.rho;
     float r = __builtin_amdgcn_sqrtf(r2);
     float w_in = __builtin_fmaf(-P.mu_p_over_xa2f, r2, mu_pv);
     // c4 (r - xi_a)(rho - r) in Horner form (two FMAs; the same FP32 error bound)
@@ -503,23 +499,28 @@ struct VortAcc<P32K> {
       group_add_prod<MODE>(pcw, w, v.z);
     }
     n01 += inr;
-    if (inc) {
-      if (r2 >= P.close2f) {
-        float ir = __builtin_amdgcn_rsqf(r2);
-        Tx.add_prod(dx, ir);
-        Ty.add_prod(dy, ir);
-      } else {
-        double ddx, ddy;
-        exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
-        double dr = sqrt(ddx * ddx + ddy * ddy);
-        if (dr > 0.0) {
-          Tx.add((float)(ddx / dr));
-          Ty.add((float)(ddy / dr));
+    if (r2 <= P.xr2f_hi) {  // within xi_r or its band: decided as for rho
+      bool inc = r2 <= P.xr2f;
+      if (r2 >= P.xr2f - P.band_xr2f)
+        inc = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy)) <= P.xi_r;
+      if (inc) {
+        if (r2 >= P.close2f) {
+          float ir = __builtin_amdgcn_rsqf(r2);
+          Tx.add_prod(dx, ir);
+          Ty.add_prod(dy, ir);
         } else {
-          Tx.add(1.0f);  // atan2(+0, +0) = 0
+          double ddx, ddy;
+          exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
+          double dr = sqrt(ddx * ddx + ddy * ddy);
+          if (dr > 0.0) {
+            Tx.add((float)(ddx / dr));
+            Ty.add((float)(ddy / dr));
+          } else {
+            Tx.add(1.0f);  // atan2(+0, +0) = 0
+          }
         }
+        ++n2;
       }
-      ++n2;
     }
   }
   __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
@@ -675,9 +676,8 @@ struct Chim2Acc<P32K> {
     float4 v = s.f4[m];
     float dx = v.x + ox, dy = v.y + co.oy;
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool in = r2 <= P.rho2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
-      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    bool in = r2 <= P.rho2f_hi;  // outside the band, the same as r2 <= rho2f
+    if ((r2 >= P.rho2f_lo) & in) in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
     sum(v, in, in ? s.aj[m] : 0.0f);
   }
   // a list entry with the partner's tile and pass-1 derivative already loaded
@@ -686,9 +686,8 @@ struct Chim2Acc<P32K> {
                                          const KParams &P) {
     float dx = v.x + co.cdx(fx, P.hxf), dy = v.y + co.cdy(fy, P.hyf);
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    bool in = r2 <= P.rho2f;
-    if (__builtin_fabsf(r2 - P.rho2f) <= P.band_rho2f)
-      in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    bool in = r2 <= P.rho2f_hi;  // outside the band, the same as r2 <= rho2f
+    if ((r2 >= P.rho2f_lo) & in) in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
     sum<MODE>(v, in, in ? ajv : 0.0f);
   }
   __device__ __forceinline__ void flush() {
