@@ -688,7 +688,7 @@ struct Chim2Acc<P32K> {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f_hi;  // outside the band, the same as r2 <= rho2f
     if ((r2 >= P.rho2f_lo) & in) in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
-    sum<MODE>(v, in, in ? ajv : 0.0f);
+    sum<MODE>(v, in, ajv);  // aj multiplies cos, sin terms that are zero outside rho
   }
   __device__ __forceinline__ void flush() {
     C.add(pc);
