@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # <VortAcc<P32K>>) on this workload (skin 0.125 rho), from the FETCH_SIZE (x2.000)
 # and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
 # copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 257.4e6
+TRAFFIC_VERLET_FP32K = 257.5e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -148,6 +148,12 @@ def profile_pair_kernel(spc2, w, precision, steps, neighbour):
             "cohesion_per_particle": per[2] / w["n"], "lds_layout": lds}
 
 
+def phase_diff(a, b):
+    """|a - b| as angles, in [0, pi]."""
+    import numpy as np
+    return np.abs(np.mod(a - b + np.pi, 2 * np.pi) - np.pi)
+
+
 def error_horizon(spc2, w, nb, horizons):
     """FP32-compensated vs FP64 from the same initial state after each of
     `horizons` steps, and the FP64 run against FP64 with the other neighbour
@@ -170,8 +176,7 @@ def error_horizon(spc2, w, nb, horizons):
     for h in horizons:
         ref = ph[("fp64", nb, h)]
         for key, name in ((("fp32k", nb, h), "fp32"), (("fp64", other, h), "fp64_reordered")):
-            # phases are kept in [0, 2 pi): compare them on the circle
-            d = np.abs(np.mod(ph[key] - ref + np.pi, 2 * np.pi) - np.pi)
+            d = phase_diff(ph[key], ref)
             out[(name, h)] = {"phase_err_max": float(d.max()), "phase_err_rms": float(np.sqrt((d ** 2).mean())),
                               "order_param_abs_err": abs(R(ph[key]) - R(ref))}
     return out
@@ -296,7 +301,7 @@ def main():
     err = {}
     if "fp64" in res:
         import numpy as np
-        dphi = np.abs(v["phases"] - res["fp64"]["phases"])
+        dphi = phase_diff(v["phases"], res["fp64"]["phases"])
         err = {"errors_fp32_vs_fp64": {
             f"after_{args.warmup + args.steps}_steps": {
                 "phase_err_max": float(dphi.max()), "phase_err_rms": float(np.sqrt((dphi ** 2).mean())),

@@ -125,6 +125,27 @@ def test_verlet_trajectory_matches_cells_across_rebuilds(gpu, method):
     assert np.abs(periodic_diff(a, b, L)).max() < 1e-9
 
 
+def test_verlet_srk_noise_bound_keeps_neighbour_sets(gpu):
+    """SRK with the built-in noise rebuilds the list on a bound of each step's
+    displacement whose noise term (sqrt(2 D) v0 |dZ|, |dZ| bounded through
+    Box-Muller's largest radius) here outweighs the drift v0 dt.  The
+    neighbour sets must stay those of the linked cells at every RHS."""
+    n, rho, dt, steps = 30000, 0.1, 0.01, 60
+    L = math.sqrt(n / 4000.0)
+    v0 = 0.2 * rho / 8 / dt
+    st = O.make_state(14, 0, n, L, rho)
+    om = O.uniform(81, n, -0.5, 0.5)
+    kw = dict(n=n, L=L, rho=rho, v0=v0, sigma=1.0, alpha=1.45, D=5.0, seed=17)
+    a, ca, _ = run(spc2.params("chimera", "fp64", "cells", **kw), st, "srk", dt, steps, om, chunks=(7, 20))
+    b, cb, (builds, _) = run(spc2.params("chimera", "fp64", "verlet", **kw), st, "srk", dt, steps, om,
+                             chunks=(7, 20))
+    assert np.array_equal(ca[1:], cb[1:]), (ca, cb)
+    # the bound per step is about 3.2 v0 dt here (noise term 2.2 v0 dt): a
+    # rebuild at least every third step
+    assert builds >= steps // 3
+    assert np.abs(periodic_diff(a, b, L)).max() < 1e-9
+
+
 def test_verlet_vortex_rk4_fp32_matches_fp64_cells(gpu):
     """The bench workload at a reduced N: vortex RK4 with Verlet lists, FP64
     against FP64 linked cells (the same neighbour sets at every RHS), and
