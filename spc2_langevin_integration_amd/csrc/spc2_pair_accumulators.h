@@ -135,10 +135,10 @@ struct ColOffsets {
 };
 
 // How an FP32 pair term is summed. The cell walks add each term to the Kahan
-// sums. The Verlet walk sums a list group of four entries in plain FP32
-// partials (the group's first entry starts them) and adds the partials to the
-// Kahan sums once per group (flush()): a compensated sum over groups, whose
-// error bound is the Kahan bound plus 3 ulp of each group's absolute sum.
+// sums. The Verlet walks sum every eight list entries in plain FP32 partials
+// (the group's first entry starts them) and add the partials to the Kahan sums
+// once per group (flush()): a compensated sum over groups, whose error bound is
+// the Kahan bound plus 7 ulp of each group's absolute sum.
 enum SumMode { SUM_KAHAN = 0, SUM_GROUP_FIRST = 1, SUM_GROUP = 2 };
 
 template <int MODE>

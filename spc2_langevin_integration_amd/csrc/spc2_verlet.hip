@@ -39,11 +39,12 @@ constexpr int kBlock = 256;
 
 inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
-// List layout: entries of the 64 homes of wavefront w are stored in groups of
-// four per lane, group g of lane p at int4 (wrow[w] + g) * 64 + p, so that a
-// wavefront reads 1 KiB per group load and a lane takes four entries per load.
-// Entry = (j << 4) | code, code = (dcx + 1) | (dcy + 1) << 2 for the stencil
-// cell (dcx, dcy) of partner j (the LDS layout's entries are EntLds).  Every
+// List layout: entries of the 64 homes of wavefront w are stored one int4 per
+// lane and group, group g of lane p at int4 (wrow[w] + g) * 64 + p, so that a
+// wavefront reads 1 KiB per group load: four 32-bit entries per group in the
+// gather layout, eight 16-bit ones in the LDS layout (EntLds).  Gather entry =
+// (j << 4) | code, code = (dcx + 1) | (dcy + 1) << 2 for the stencil cell
+// (dcx, dcy) of partner j.  Every
 // lane's column runs to the wavefront's group count, padded with a valid entry
 // (the home itself, never summed: the passes stop at the count; in the LDS
 // layout the halo's sentinel tile), and kVerletSpareGroups valid groups follow
