@@ -418,8 +418,16 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
   const int c = t.cell[i], cx = c % P.nx, cy = c / P.nx;
   const uint32_t pad = EntLds::make(off[nrun], 5);  // the sentinel tile after the halo
   const int pad2 = (int)(pad | pad << 16);
-  uint32_t word = 0;
-  int4 buf = make_int4(pad2, pad2, pad2, pad2);
+  // the group being formed, a 128-bit shift register of 16-bit entries: each
+  // entry enters at the top and the others move down 16 bits (one funnel shift
+  // per word), so that after eight entries word k holds entries 2k, 2k + 1
+  uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+  const auto push = [&](uint32_t ent) {
+    r0 = __builtin_amdgcn_alignbit(r1, r0, 16);
+    r1 = __builtin_amdgcn_alignbit(r2, r1, 16);
+    r2 = __builtin_amdgcn_alignbit(r3, r2, 16);
+    r3 = __builtin_amdgcn_alignbit(ent, r3, 16);
+  };
   int e = 0, last = -1, base = 0;
   for_each_build_neighbour<P32K>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
     if (code != last) {
@@ -427,26 +435,15 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
       base = halo_base(run, off, nrun, start, jc);
       last = code;
     }
-    const uint32_t ent = EntLds::make(base + j, code);
-    if ((e & 1) == 0) {
-      word = ent | pad << 16;
-    } else {
-      word = (word & 0xffffu) | ent << 16;
-    }
-    switch ((e >> 1) & 3) {
-      case 0: buf.x = (int)word; break;
-      case 1: buf.y = (int)word; break;
-      case 2: buf.z = (int)word; break;
-      default: buf.w = (int)word;
-    }
-    if ((e & 7) == 7) {
-      dst[(size_t)(e >> 3) * 64] = buf;
-      buf = make_int4(pad2, pad2, pad2, pad2);
-    }
+    push(EntLds::make(base + j, code));
+    if ((e & 7) == 7) dst[(size_t)(e >> 3) * 64] = make_int4((int)r0, (int)r1, (int)r2, (int)r3);
     ++e;
   });
+  if (e & 7) {  // the last group, completed with sentinel entries
+    for (int k = e & 7; k < 8; ++k) push(pad);
+    dst[(size_t)(e >> 3) * 64] = make_int4((int)r0, (int)r1, (int)r2, (int)r3);
+  }
   for (int g = (e + 7) >> 3; g < groups; ++g) dst[(size_t)g * 64] = make_int4(pad2, pad2, pad2, pad2);
-  if (e & 7) dst[(size_t)(e >> 3) * 64] = buf;
 }
 
 // Per-RHS stage: s + (k_coef dt) k by home, the FP64 tile (conditionally
