@@ -194,23 +194,28 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *st
   const int w = i >> 6, groups = wrow[w + 1] - wrow[w];
   int4 *dst = (int4 *)vlist + (size_t)wrow[w] * 64 + (i & 63);
   const int self = (i << 4) | 5;
-  int4 buf = make_int4(self, self, self, self);
+  // the group being formed: entries enter at the top and move down a word
+  // (register moves instead of a switch on the entry's position)
+  int r0 = self, r1 = self, r2 = self, r3 = self;
   int e = 0;
   for_each_build_neighbour<PREC>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
-    const int ent = (j << 4) | code;
-    switch (e & 3) {
-      case 0: buf.x = ent; break;
-      case 1: buf.y = ent; break;
-      case 2: buf.z = ent; break;
-      default:
-        buf.w = ent;
-        dst[(size_t)(e >> 2) * 64] = buf;
-        buf = make_int4(self, self, self, self);
-    }
+    r0 = r1;
+    r1 = r2;
+    r2 = r3;
+    r3 = (j << 4) | code;
+    if ((e & 3) == 3) dst[(size_t)(e >> 2) * 64] = make_int4(r0, r1, r2, r3);
     ++e;
   });
+  if (e & 3) {  // the last group, completed with the home itself (never summed)
+    for (int k = e & 3; k < 4; ++k) {
+      r0 = r1;
+      r1 = r2;
+      r2 = r3;
+      r3 = self;
+    }
+    dst[(size_t)(e >> 2) * 64] = make_int4(r0, r1, r2, r3);
+  }
   for (int g = (e + 3) >> 2; g < groups; ++g) dst[(size_t)g * 64] = make_int4(self, self, self, self);
-  if (e & 3) dst[(size_t)(e >> 2) * 64] = buf;
 }
 
 // ---- LDS-staged passes (FP32 form) -----------------------------------------------
