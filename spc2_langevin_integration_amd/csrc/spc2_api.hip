@@ -260,11 +260,16 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   launch_rehome(s, k_xy_live);  // slot k becomes home k: tiles, cells and start[] are now by home
   HIP_TRY(hipMemcpyAsync(s.xb, s.t.x, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
   HIP_TRY(hipMemcpyAsync(s.yb, s.t.y, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
-  launch_verlet_count(s);
+  // the LDS layout (FP32 form): every pass workgroup's halo, and the largest
+  // (the count pass also reads its candidates from the halos that fit its LDS)
+  const bool try_lds = s.vlds_allowed && s.bh_max && verlet_lds_unbased(s.P.system);
+  if (try_lds) {
+    HIP_TRY(hipMemsetAsync(s.bh_max, 0, sizeof(int), s.stream));
+    launch_block_halo(s);
+  }
+  launch_verlet_count(s, try_lds);
   launch_scan_array(s, s.wmax, s.nw, s.wrow);
   if (spc2_status st = check_launch("verlet count")) return st;
-  // the LDS layout (FP32 form): every pass workgroup's halo, and the largest
-  const bool try_lds = s.vlds_allowed && s.bh_max && verlet_lds_unbased(s.P.system);
   // lane slots sorted by list length pay off when lists are long enough for the
   // saved wavefront iterations to outweigh the epilogue's scattered accesses
   // (measured: 150 entries per particle gain, 40 lose); the mean list length is
@@ -279,8 +284,6 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
     } else {
       launch_scan_array(s, s.wmax8, s.nw, s.wrow8);
     }
-    HIP_TRY(hipMemsetAsync(s.bh_max, 0, sizeof(int), s.stream));
-    launch_block_halo(s);
   }
   int groups = 0, groups_l = 0, halo = 0;
   HIP_TRY(hipMemcpyAsync(&groups, s.wrow + s.nw, sizeof(int), hipMemcpyDeviceToHost, s.stream));

@@ -220,7 +220,8 @@ void launch_pass2(Sim &s, int epi, double *a0, double *a1, const StepArgs &sa);
 // valid groups (64 lanes x 4 entries) kept after a Verlet list for the
 // passes' read-ahead
 constexpr int kVerletSpareGroups = 2;
-void launch_verlet_count(Sim &s);
+// halo: the workgroups' halo runs are current (launch_block_halo ran since the re-homing)
+void launch_verlet_count(Sim &s, bool halo);
 void launch_verlet_fill(Sim &s);
 constexpr int kVerletRunMax = 16;  // halo runs per pass workgroup (kRunMax of spc2_verlet.hip)
 void launch_block_halo(Sim &s);

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "spc2_device.h"
 #include "spc2_internal.h"
@@ -38,6 +39,7 @@ namespace {
 constexpr int kBlock = 256;
 
 inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+constexpr int kRunMax = kVerletRunMax;
 
 // List layout: entries of the 64 homes of wavefront w are stored one int4 per
 // lane and group, group g of lane p at int4 (wrow[w] + g) * 64 + p, so that a
@@ -58,9 +60,18 @@ inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / 
 // rlist^2 (1 + 2^-13), a superset of the FP64 test (the passes test every entry
 // against the cutoffs themselves, and a pair outside rlist at the build stays
 // outside rho while the list is used).
-template <int PREC, class F>
+// FP32 tiles of the candidates: TilesGlobal reads tile j from memory; the
+// count pass reads them from its workgroup's staged halo (HaloTiles), where the
+// tiles of build cell jc sit at base(jc) + j.
+struct TilesGlobal {
+  const float4 *f4;
+  __device__ int base(int) const { return 0; }
+  __device__ float4 at(int k) const { return f4[k]; }
+};
+
+template <int PREC, class F, class TS = TilesGlobal>
 __device__ __forceinline__ void for_each_build_candidate(const KParams &P, const int *start, const Tiles &t, int i,
-                                                         double rlist2, F &&f) {
+                                                         double rlist2, F &&f, const TS &ts = TS{nullptr}) {
   const int c = t.cell[i];
   const int cx = c % P.nx, cy = c / P.nx;
   if constexpr (PREC == P32K) {
@@ -75,9 +86,10 @@ __device__ __forceinline__ void for_each_build_candidate(const KParams &P, const
         const int code = (dxo + 1) | ((dyo + 1) << 2);
         const float ox = __builtin_fmaf((float)(dxo + 1), P.hxf, kx);
         const int e = start[jc + 1];
+        const int tb = ts.base(jc);
 #pragma unroll 4
         for (int j = start[jc]; j < e; ++j) {
-          const float4 v = t.f4[j];
+          const float4 v = std::is_same<TS, TilesGlobal>::value ? t.f4[j] : ts.at(tb + j);
           const float dx = v.x + ox, dy = v.y + oy;
           f(j, code, (__builtin_fmaf(dx, dx, dy * dy) <= rl2) & (j != i));
         }
@@ -149,27 +161,72 @@ __device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const
   }
 }
 
+// LDS index base of build cell c in workgroup b's halo: index = base + home.
+__device__ __forceinline__ int halo_base(const int *run, const int *off, int nrun, const int *start, int c) {
+  int k = 0;
+  while (k + 1 < nrun && run[2 * k + 2] <= c) ++k;
+  return off[k] - start[run[2 * k]];
+}
+
+// The tiles of a workgroup's staged halo, for the candidate loop.
+struct HaloTiles {
+  const float4 *tiles;
+  const int *run, *off, *start;
+  int nrun;
+  __device__ int base(int c) const { return halo_base(run, off, nrun, start, c); }
+  __device__ float4 at(int k) const { return tiles[k]; }
+};
+
 // Build pass 1: entries per home, their candidate bits and each wavefront's
 // group count (its longest list in groups of four; wgroups8, when set, in
-// groups of eight, for the LDS layout).
+// groups of eight, for the LDS layout).  The FP32 form reads the candidates'
+// tiles from the workgroup's halo staged in LDS when the halo runs are given
+// (bh_nrun) and the halo fits kCountHalo tiles, else from memory: the same
+// values either way.
+constexpr int kCountHalo = 1536;  // 24 KiB of LDS per count workgroup
+
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_count(KParams P, const int *start, Tiles t, double rlist2,
                                                          uint32_t *mask, int mwords, int *vcnt, int *wgroups,
-                                                         int *wgroups8) {
+                                                         int *wgroups8, const int *bh_nrun, const int *bh_run,
+                                                         const int *bh_off) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
-  if (i < P.n) {
-    uint32_t word = 0;
-    int q = 0;
-    for_each_build_candidate<PREC>(P, start, t, i, rlist2, [&](int, int, bool in) {
-      word |= (uint32_t)in << (q & 31);
-      cnt += in;
-      if ((q & 31) == 31) {
-        if ((q >> 5) < mwords) mask[(size_t)(q >> 5) * P.n + i] = word;
-        word = 0;
+  uint32_t word = 0;
+  int q = 0;
+  const auto bit = [&](int, int, bool in) {
+    word |= (uint32_t)in << (q & 31);
+    cnt += in;
+    if ((q & 31) == 31) {
+      if ((q >> 5) < mwords) mask[(size_t)(q >> 5) * P.n + i] = word;
+      word = 0;
+    }
+    ++q;
+  };
+  bool staged = false;
+  if constexpr (PREC == P32K) {
+    __shared__ float4 halo[kCountHalo];
+    __shared__ int run[2 * kRunMax], off[kRunMax + 1];
+    const int b = blockIdx.x, nrun = bh_nrun ? bh_nrun[b] : -1;
+    if (nrun >= 0) {  // uniform over the workgroup
+      if ((int)threadIdx.x < 2 * kRunMax) run[threadIdx.x] = bh_run[(size_t)b * 2 * kRunMax + threadIdx.x];
+      if ((int)threadIdx.x <= kRunMax) off[threadIdx.x] = bh_off[(size_t)b * (kRunMax + 1) + threadIdx.x];
+      __syncthreads();
+      const int total = off[nrun];
+      staged = total <= kCountHalo;
+      if (staged) {
+        for (int el = threadIdx.x; el < total; el += kBlock) {
+          int k = 0;
+          while (k + 1 < nrun && off[k + 1] <= el) ++k;
+          halo[el] = t.f4[start[run[2 * k]] + (el - off[k])];
+        }
+        __syncthreads();
+        if (i < P.n) for_each_build_candidate<PREC>(P, start, t, i, rlist2, bit, HaloTiles{halo, run, off, start, nrun});
       }
-      ++q;
-    });
+    }
+  }
+  if (!staged && i < P.n) for_each_build_candidate<PREC>(P, start, t, i, rlist2, bit);
+  if (i < P.n) {
     if ((q & 31) && (q >> 5) < mwords) mask[(size_t)(q >> 5) * P.n + i] = word;
     vcnt[i] = cnt;
   }
@@ -229,7 +286,6 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *st
 // home.  The passes stage the halo tiles in LDS with coalesced reads and gather
 // from LDS.  A build where a workgroup spans more than two cell rows, or whose
 // largest halo does not fit the LDS capacity, keeps the gather layout.
-constexpr int kRunMax = kVerletRunMax;
 
 // Runs of halo cells of workgroup b in increasing cell order: cells
 // [run0[r], run1[r]], LDS offset roff[r] (roff[nrun] = halo size).  nrun = -1:
@@ -394,13 +450,6 @@ struct LdsTail {
 };
 static_assert(sizeof(LdsTail) % 8 == 0, "LdsTail keeps the 8-byte alignment of its offset");
 constexpr size_t lds_pass_bytes(int cap, int pass) { return (size_t)cap * (pass == 2 ? 20 : 16) + sizeof(LdsTail); }
-
-// LDS index base of build cell c in workgroup b's halo: index = base + home.
-__device__ __forceinline__ int halo_base(const int *run, const int *off, int nrun, const int *start, int c) {
-  int k = 0;
-  while (k + 1 < nrun && run[2 * k + 2] <= c) ++k;
-  return off[k] - start[run[2 * k]];
-}
 
 // Build pass 2 for the LDS layout (FP32 form), per lane slot: as k_verlet_fill,
 // with EntLds entries (the partner's index in the workgroup's staged halo in
@@ -933,21 +982,25 @@ bool verlet_lds_unbased(int system) {
   return ok;
 }
 
-void launch_verlet_count(Sim &s) {
+void launch_verlet_count(Sim &s, bool halo) {
+  const int *hn = halo ? (const int *)s.bh_nrun : nullptr;
   const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
   const dim3 g(blocks_for(s.n)), b(kBlock);
   switch (s.prec) {
     case P64:
       hipLaunchKernelGGL(k_verlet_count<P64>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
-                         s.vmask_words, s.vcnt, s.wmax, s.wmax8);
+                         s.vmask_words, s.vcnt, s.wmax, s.wmax8, hn,
+                         (const int *)s.bh_run, (const int *)s.bh_off);
       break;
     case P64F:
       hipLaunchKernelGGL(k_verlet_count<P64F>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
-                         s.vmask_words, s.vcnt, s.wmax, s.wmax8);
+                         s.vmask_words, s.vcnt, s.wmax, s.wmax8, hn,
+                         (const int *)s.bh_run, (const int *)s.bh_off);
       break;
     default:
       hipLaunchKernelGGL(k_verlet_count<P32K>, g, b, 0, s.stream, s.kp, (const int *)s.start, s.t, r2, s.vmask,
-                         s.vmask_words, s.vcnt, s.wmax, s.wmax8);
+                         s.vmask_words, s.vcnt, s.wmax, s.wmax8, hn,
+                         (const int *)s.bh_run, (const int *)s.bh_off);
   }
 }
 
