@@ -91,8 +91,10 @@ struct Sim {
   // Particle arrays (state, omega, k, derivative, stage) are stored in "home"
   // order h: hid[h] is the particle id at home h, hpos[id] its home.  Homes are
   // re-assigned every rehome_every steps to the current cell-sorted order so
-  // that per-particle reads and writes stay coalesced.
+  // that per-particle reads and writes stay coalesced.  hpos is rebuilt from
+  // hid only when a transfer by id needs it (hpos_stale after a re-homing).
   int *hid = nullptr, *hpos = nullptr;
+  bool hpos_stale = false;
   // cell list (per home: cell_of, local; per slot: tmp/tmpid while placing,
   // then slot_h / slot_id = home / id of the particle in each cell-sorted slot)
   int *cell_of = nullptr, *count = nullptr, *local = nullptr, *start = nullptr, *tmp = nullptr,

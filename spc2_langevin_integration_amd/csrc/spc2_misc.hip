@@ -132,16 +132,20 @@ struct PermuteSet {
 
 // The first `count` arrays of the set move (the rest are dead at the call).
 __global__ __launch_bounds__(kBlock) void k_permute(int n, const int *slot_h, PermuteSet ps, int count,
-                                                    const int *hid_in, int *hid_out, int *hpos) {
+                                                    const int *hid_in, int *hid_out) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int h = slot_h[k];
 #pragma unroll
   for (int a = 0; a < kPermuteArrays; ++a)
     if (a < count) ps.dst[a][k] = ps.src[a][h];
-  const int id = hid_in[h];
-  hid_out[k] = id;
-  hpos[id] = k;
+  hid_out[k] = hid_in[h];
+}
+
+// hpos[hid[h]] = h: the home of every particle id.
+__global__ __launch_bounds__(kBlock) void k_hpos(int n, const int *hid, int *hpos) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < n) hpos[hid[h]] = h;
 }
 
 __global__ __launch_bounds__(kBlock) void k_noise(int n, int per_particle, uint64_t seed, uint64_t step,
@@ -176,25 +180,38 @@ void launch_init_state(Sim &s, double omega_std) {
                      omega_std, s.hid, s.sx, s.sy, s.sp, s.omega);
 }
 
+namespace {
+void ensure_hpos(Sim &s) {
+  if (!s.hpos_stale) return;
+  hipLaunchKernelGGL(k_hpos, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, (const int *)s.hid, s.hpos);
+  s.hpos_stale = false;
+}
+}  // namespace
+
 void launch_to_home(Sim &s, const double *src_id, double *dst_home) {
+  ensure_hpos(s);
   hipLaunchKernelGGL(k_to_home, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_id, dst_home);
 }
 
 void launch_from_home(Sim &s, const double *src_home, double *dst_id) {
+  ensure_hpos(s);
   hipLaunchKernelGGL(k_from_home, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_home, dst_id);
 }
 
 void launch_from_home_int(Sim &s, const int *src_home, int *dst_id) {
+  ensure_hpos(s);
   hipLaunchKernelGGL(k_from_home_int, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_home,
                      dst_id);
 }
 
 void launch_snapshot(Sim &s, float *out) {
+  ensure_hpos(s);
   hipLaunchKernelGGL(k_snapshot, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, s.sx, s.sy, s.sp,
                      out);
 }
 
 void launch_identity_homes(Sim &s) {
+  s.hpos_stale = false;
   hipLaunchKernelGGL(k_iota, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hid, s.hpos);
 }
 
@@ -214,9 +231,10 @@ void launch_rehome(Sim &s, bool k_xy_live) {
     ps.dst[a] = s.spare[a];
   }
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.slot_h, ps, count, s.hid,
-                     s.spare_i, s.hpos);
+                     s.spare_i);
   for (int a = 0; a < count; ++a) std::swap(*arrs[a], s.spare[a]);
   std::swap(s.hid, s.spare_i);
+  s.hpos_stale = true;  // a scatter by id, deferred to the next transfer by id
 }
 
 void launch_noise(Sim &s, int per_particle, uint64_t step, double *out) {
