@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # <VortAcc<P32K>>) on this workload (skin 0.125 rho), from the FETCH_SIZE (x2.000)
 # and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
 # copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 257.5e6
+TRAFFIC_VERLET_FP32K = 253.2e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -90,12 +90,12 @@ def fixed_bytes_per_particle(w, precision, neighbour):
     from LDS and cache).
 
     Vortex RK4, every stage: own tile (FP32: x, y, cos, sin 16 B + FP64 stage
-    phase 8 B; FP64: 24 B), list count 4 B, k written 24 B.  The last stage
-    (a quarter of the launches) also reads k1..k3 (72 B) and reads and writes
-    the state (48 B).  Verlet lists fuse the next stage's tiles into the
-    epilogue: written (FP32: 16 B tile + 16 B FP64 position + 8 B phase; FP64:
-    24 B + 8 B), with the build cell (4 B) and, in stages 1-3, the state
-    (24 B) read.
+    phase 8 B; FP64: 24 B), list count 4 B, k written (stages 1-3: 24 B; the
+    last stage writes only k_4's phi, 8 B).  The last stage (a quarter of the
+    launches) also reads k1..k3 (72 B) and reads and writes the state (48 B).
+    Verlet lists fuse the next stage's tiles into the epilogue: written (FP32:
+    16 B tile + 16 B FP64 position + 8 B phase; FP64: 24 B + 8 B), with the
+    build cell (4 B) and, in stages 1-3, the state (24 B) read.
 
     Chimera SRK, the mean of its two passes: pass 1 reads the own tile 24 B,
     count and id 8 B and writes the derivative 24 B and the pass-2 inputs 28 B
@@ -107,7 +107,7 @@ def fixed_bytes_per_particle(w, precision, neighbour):
     if w["system"] == "chimera":
         return (84 + 160) / 2.0
     own = 24
-    b = own + 4 + 24 + (72 + 48) / 4.0
+    b = own + 4 + (24 * 3 + 8) / 4.0 + (72 + 48) / 4.0
     if neighbour == "verlet":
         b += (40 if fp32 else 32) + 4 + 24 * 3 / 4.0
     return b

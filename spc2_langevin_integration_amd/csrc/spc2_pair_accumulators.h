@@ -1024,8 +1024,12 @@ __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e,
   if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
     double ap = phi_out(o, e.sa.zero_in ? nullptr : e.op + h);
     r.ap = ap;
-    e.ox[h] = ax;
-    e.oy[h] = ay;
+    if (EPI != EPI_RK4_FINAL) {
+      // k_4's x / y components are consumed here by the combine and read by
+      // nothing else; only its phi component carries over (in/out above)
+      e.ox[h] = ax;
+      e.oy[h] = ay;
+    }
     e.op[h] = ap;
     if (EPI == EPI_ITO1) {
       e.aphi_slot[slot] = ap;
