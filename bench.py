@@ -90,27 +90,28 @@ def fixed_bytes_per_particle(w, precision, neighbour):
     from LDS and cache).
 
     Vortex RK4, every stage: own tile (FP32: x, y, cos, sin 16 B + FP64 stage
-    phase 8 B; FP64: 24 B), list count 4 B, k written (stages 1-3: 24 B; the
-    last stage writes only k_4's phi, 8 B).  The last stage (a quarter of the
-    launches) also reads k1..k3 (72 B) and reads and writes the state (48 B).
+    phase 8 B; FP64: 24 B), list count 4 B, k written (linked cells: 24 B in
+    stages 1-3, k_4's phi 8 B in the last; Verlet steps write only k's phi,
+    8 B, since the combine rebuilds k_1..k_3's x / y from the stage phases).
+    The last stage (a quarter of the launches) also reads and writes the state
+    (48 B) and reads k1..k3 (72 B; Verlet: their phi components, 24 B).
     Verlet lists fuse the next stage's tiles into the epilogue: written (FP32:
     16 B tile + 16 B FP64 position + 8 B phase; FP64: 24 B + 8 B), with the
     build cell (4 B) and, in stages 1-3, the state (24 B) read.
 
     Chimera SRK, the mean of its two passes: pass 1 reads the own tile 24 B,
-    count and id 8 B and writes the derivative 24 B and the pass-2 inputs 28 B
-    (84 B); pass 2 reads the tile 24 B, count and id 8 B, its pass-1 inputs
-    24 B and the staged pass-1 derivative 4 B, omega 8 B, the state 24 B, and
-    writes the state 24 B and (Verlet) the next step's tiles 40 B with the
-    build cell 4 B read (160 B)."""
+    count and id 8 B and writes the pass-2 inputs 28 B (60 B; the derivative
+    itself is not written, nothing in the step reads it); pass 2 reads the
+    tile 24 B, count and id 8 B, its pass-1 inputs 24 B and the staged pass-1
+    derivative 4 B, omega 8 B, the state 24 B, and writes the state 24 B and
+    (Verlet) the next step's tiles 40 B with the build cell 4 B read (160 B)."""
     fp32 = precision == "fp32k"
     if w["system"] == "chimera":
-        return (84 + 160) / 2.0
+        return (60 + 160) / 2.0
     own = 24
-    b = own + 4 + (24 * 3 + 8) / 4.0 + (72 + 48) / 4.0
     if neighbour == "verlet":
-        b += (40 if fp32 else 32) + 4 + 24 * 3 / 4.0
-    return b
+        return own + 4 + 8 + (24 + 48) / 4.0 + (40 if fp32 else 32) + 4 + 24 * 3 / 4.0
+    return own + 4 + (24 * 3 + 8) / 4.0 + (72 + 48) / 4.0
 
 
 def profile_pair_kernel(spc2, w, precision, steps, neighbour):

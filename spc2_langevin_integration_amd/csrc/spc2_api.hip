@@ -554,6 +554,7 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
   if (const char *env = std::getenv("SPC2_REHOME_EVERY")) s.rehome_every = std::max(1, std::atoi(env));
   if (const char *env = std::getenv("SPC2_VERLET_LDS")) s.vlds_allowed = std::atoi(env) != 0;
   if (const char *env = std::getenv("SPC2_VERLET_SORT")) s.vsort_mode = std::atoi(env) != 0 ? 1 : 0;
+  if (const char *env = std::getenv("SPC2_RK4_STORE_KXY")) s.rk4_store_kxy = std::atoi(env) != 0;
   hipError_t e = hipGetDevice(&s.device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -716,7 +717,9 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
     } else if (method == SPC2_SRK) {
       if (!fused) build(s, nullptr, 0.0);
       sa.zero_in = 1;  // derivative zero-filled each step (StochasticRungeKuttaStepperPtr.hpp:43-46)
-      rhs_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);
+      // the derivative itself is read by nothing in the step (pass 2 takes a_phi
+      // per slot and its own velocity), so pass 1 does not write it
+      rhs_pass1(s, EPI_ITO1, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
       if (fused) next_stage(s, 2, 0.0);
       rhs_pass2(s, EPI_SRK, nullptr, nullptr, sa);
     } else {
@@ -724,14 +727,23 @@ spc2_status spc2_step_ex(spc2_sim_t h, int32_t method, double dt, int64_t step0,
       // k3 = f(s, k2, 1/2), k4 = f(s, k3, 1), stage = s + (c * dt) * k
       double *const *k1 = s.k[0], *const *k2 = s.k[1], *const *k3 = s.k[2], *const *k4 = s.k[3];
       if (fused) {  // the first stage s + 0 k1 is the state itself
+        // The epilogues form the next stage from k in registers, so k_1..k_3's
+        // x / y components need not be stored: the combine rebuilds them from
+        // the stage phases p, p + (dt / 2) k1_phi and p + (dt / 2) k2_phi, which
+        // needs these stage coefficients to stay 0.5 dt (SPC2_RK4_STORE_KXY=1
+        // stores and reads them instead).
+        const bool kxy = s.rk4_store_kxy;
+        double *const k1o[3] = {kxy ? k1[0] : nullptr, kxy ? k1[1] : nullptr, k1[2]};
+        double *const k2o[3] = {kxy ? k2[0] : nullptr, kxy ? k2[1] : nullptr, k2[2]};
+        double *const k3o[3] = {kxy ? k3[0] : nullptr, kxy ? k3[1] : nullptr, k3[2]};
         next_stage(s, 1, 0.5 * dt);
-        rhs_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);
+        rhs_pass1(s, EPI_K, k1o[0], k1o[1], k1o[2], sa, nullptr, nullptr, nullptr);
         next_stage(s, 1, 0.5 * dt);
-        rhs_pass1(s, EPI_K, k2[0], k2[1], k2[2], sa, nullptr, nullptr, nullptr);
+        rhs_pass1(s, EPI_K, k2o[0], k2o[1], k2o[2], sa, nullptr, nullptr, nullptr);
         next_stage(s, 1, 1.0 * dt);
-        rhs_pass1(s, EPI_K, k3[0], k3[1], k3[2], sa, nullptr, nullptr, nullptr);
+        rhs_pass1(s, EPI_K, k3o[0], k3o[1], k3o[2], sa, nullptr, nullptr, nullptr);
         next_stage(s, 2, 0.0);
-        rhs_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1, k2, k3);
+        rhs_pass1(s, EPI_RK4_FINAL, k4[0], k4[1], k4[2], sa, k1o, k2o, k3o);
       } else {
         build(s, k1, 0.0 * dt);
         rhs_pass1(s, EPI_K, k1[0], k1[1], k1[2], sa, nullptr, nullptr, nullptr);

@@ -151,6 +151,9 @@ struct Sim {
   int *vslot = nullptr, *wmax_s = nullptr, *wrow_s = nullptr;
   bool vsorted = false;  // the current LDS-layout list uses the lane slots
   int vsort_mode = -1;   // -1: by list length, 0: never, 1: always (SPC2_VERLET_SORT)
+  // fused RK4 steps store k_1..k_3's x / y components instead of rebuilding
+  // them in the combine (SPC2_RK4_STORE_KXY)
+  bool rk4_store_kxy = false;
   size_t vlist_cap = 0;
   double *xb = nullptr, *yb = nullptr;  // build positions by home
   long long vbuilds = 0, ventries = 0;

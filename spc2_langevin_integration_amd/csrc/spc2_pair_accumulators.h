@@ -1022,15 +1022,16 @@ __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e,
   double ax = P.v0 * cs, ay = P.v0 * sn;
   EpiState r{ax, ay, 0.0, 0.0, 0.0, 0.0};
   if (EPI == EPI_K || EPI == EPI_ITO1 || EPI == EPI_RK4_FINAL) {
-    double ap = phi_out(o, e.sa.zero_in ? nullptr : e.op + h);
+    double ap = phi_out(o, e.sa.zero_in || !e.op ? nullptr : e.op + h);
     r.ap = ap;
-    if (EPI != EPI_RK4_FINAL) {
-      // k_4's x / y components are consumed here by the combine and read by
-      // nothing else; only its phi component carries over (in/out above)
+    // Null outputs are not written: k_4's x / y components (the combine below
+    // consumes them), a fused RK4 step's k_1..k_3 x / y (rebuilt from the stage
+    // phases by the combine) and an SRK step's derivative (read by nothing).
+    if (EPI != EPI_RK4_FINAL && e.ox) {
       e.ox[h] = ax;
       e.oy[h] = ay;
     }
-    e.op[h] = ap;
+    if (e.op) e.op[h] = ap;
     if (EPI == EPI_ITO1) {
       e.aphi_slot[slot] = ap;
       e.aphi_slot_f[slot] = (float)ap;
@@ -1042,9 +1043,27 @@ __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e,
       // x, y and (...) / 6 * dt for phi, evaluated left to right
       const double dt = e.sa.dt;
       double x = e.sx[h], y = e.sy[h], p = e.sp[h];
-      x += (e.k1x[h] + 2.0 * e.k2x[h] + 2.0 * e.k3x[h] + ax) * dt / 6.0;
-      y += (e.k1y[h] + 2.0 * e.k2y[h] + 2.0 * e.k3y[h] + ay) * dt / 6.0;
-      p += (e.k1p[h] + 2.0 * e.k2p[h] + 2.0 * e.k3p[h] + ap) / 6.0 * dt;
+      const double k1p = e.k1p[h], k2p = e.k2p[h], k3p = e.k3p[h];
+      double k1x, k1y, k2x, k2y, k3x, k3y;
+      if (e.k1x) {
+        k1x = e.k1x[h], k1y = e.k1y[h], k2x = e.k2x[h], k2y = e.k2y[h], k3x = e.k3x[h], k3y = e.k3y[h];
+      } else {
+        // k_i = v0 (cos, sin) of stage i's phase: the state's for stage 1,
+        // p + (dt / 2) k_1 and p + (dt / 2) k_2 for stages 2 and 3, formed with
+        // the operations and operands of the epilogues that wrote those stages
+        // (contraction off), so that each k_i is the same bit for bit
+        const double hdt = 0.5 * dt;
+        double sn_i, cs_i;
+        sincos(p, &sn_i, &cs_i);
+        k1x = P.v0 * cs_i, k1y = P.v0 * sn_i;
+        sincos(p + hdt * k1p, &sn_i, &cs_i);
+        k2x = P.v0 * cs_i, k2y = P.v0 * sn_i;
+        sincos(p + hdt * k2p, &sn_i, &cs_i);
+        k3x = P.v0 * cs_i, k3y = P.v0 * sn_i;
+      }
+      x += (k1x + 2.0 * k2x + 2.0 * k3x + ax) * dt / 6.0;
+      y += (k1y + 2.0 * k2y + 2.0 * k3y + ay) * dt / 6.0;
+      p += (k1p + 2.0 * k2p + 2.0 * k3p + ap) / 6.0 * dt;
       if (e.sa.wrap) wrap_state(P, x, y);
       e.sx[h] = x;
       e.sy[h] = y;

@@ -228,6 +228,38 @@ def test_verlet_fused_stage_tiles_equal_separate_stage_pass(gpu, prec, method):
     assert np.array_equal(a, b), np.abs(a - b).max()
 
 
+@pytest.mark.parametrize("system,prec", [("vortex", "fp32k"), ("vortex", "fp64"), ("chimera", "fp32k"),
+                                         ("chimera", "fp64f")])
+def test_verlet_rk4_combine_rebuilt_k_equals_stored_k(gpu, monkeypatch, system, prec):
+    """Fused RK4 steps do not store k_1..k_3's x / y components; the combine
+    rebuilds them as v0 (cos, sin) of the stage phases.  The trajectory must
+    equal, bit for bit and across list rebuilds, the one whose combine reads
+    them from memory (SPC2_RK4_STORE_KXY=1)."""
+    n, rho, dt, steps = 30000, 0.05, 0.01, 30
+    L = math.sqrt(n / 4000.0)
+    if system == "vortex":
+        kw = dict(n=n, L=L, rho=rho, D=0.0, seed=4, verlet_skin=0.004, **VORT)
+    else:
+        kw = dict(n=n, L=L, rho=rho, v0=0.02, sigma=1.0, alpha=1.45, D=0.0, seed=4, verlet_skin=0.004)
+    st = O.make_state(12, 1, n, L, rho)
+    om = O.uniform(81, n, -0.5, 0.5) if system == "chimera" else None
+    p = spc2.params(system, prec, "verlet", **kw)
+    out = []
+    for store in ("0", "1"):
+        with monkeypatch.context() as m:
+            m.setenv("SPC2_RK4_STORE_KXY", store)
+            with spc2.Sim(p) as s:
+                s.set_state(st)
+                if om is not None:
+                    s.set_omega(om)
+                s.step("rk4", dt, 0, 7)
+                s.step("rk4", dt, 7, steps - 7)
+                out.append((s.get_state(), s.verlet_stats()[0]))
+    (a, ba), (b, bb) = out
+    assert ba == bb and ba >= 3
+    assert np.array_equal(a, b), np.abs(a - b).max()
+
+
 def run_layout(params, st, method, dt, steps, monkeypatch, lds, omega=None, sort=None):
     """run() with the FP32 pass kernels chosen: lds=False keeps the gather
     passes (SPC2_VERLET_LDS=0 at creation); sort = "0" / "1" forces the LDS
