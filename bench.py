@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # <VortAcc<P32K>>) on this workload (skin 0.125 rho), from the FETCH_SIZE (x2.000)
 # and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
 # copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 253.2e6
+TRAFFIC_VERLET_FP32K = 228.0e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
