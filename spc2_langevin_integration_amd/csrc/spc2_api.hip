@@ -71,6 +71,8 @@ void free_all(Sim &s) {
   f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb); f(s.vmask);
   f(s.bh_nrun); f(s.bh_run); f(s.bh_off); f(s.bh_max); f(s.vslot); f(s.wmax_s); f(s.wrow_s);
   f(s.wmax8); f(s.wrow8);
+  for (auto &kk : s.kev)
+    for (auto &p : kk) f(p);
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
 }
 
@@ -630,32 +632,42 @@ spc2_status spc2_set_omega(spc2_sim_t h, const double *omega) {
 spc2_status spc2_eval_rhs(spc2_sim_t h, const double *k_prev, double k_coef, double dt, double *k_next) {
   if (check_handle(h) || !k_prev || !k_next) return fail(SPC2_ERR_INVALID, "null argument");
   Sim &s = h->s;
-  // k_prev -> k[1] (scratch), k_next (in/out) -> k[0]
-  if (spc2_status st = upload_aos(s, k_prev, 3, s.k[1], 3)) return st;
-  if (spc2_status st = upload_aos(s, k_next, 3, s.k[0], 3)) return st;
+  // The caller's k buffers go to scratch of their own: s.k holds the RK4
+  // stepper's k_1..k_4, whose phi components carry over between its steps.
+  for (auto &kk : s.kev)
+    for (auto &p : kk)
+      if (!p) HIP_TRY(dalloc(&p, (size_t)s.n));
+  if (spc2_status st = upload_aos(s, k_prev, 3, s.kev[1], 3)) return st;
   if (s.verlet) {
     // reach of the stage: |k_coef dt| max_i |(k_x, k_y)_i|
     double kmax = 0.0;
     for (int64_t i = 0; i < s.n; ++i) kmax = std::max(kmax, std::hypot(k_prev[3 * i], k_prev[3 * i + 1]));
     const double reach = std::fabs(k_coef * dt) * kmax * (1.0 + 1e-12);
-    spc2_status st = reach <= 0.5 * s.skin * (1.0 - 1e-9) ? verlet_ensure(s, reach, true)
-                                                           : verlet_build(s, s.k[1], k_coef * dt, true);
+    const long long builds = s.vbuilds;
+    spc2_status st = reach <= 0.5 * s.skin * (1.0 - 1e-9) ? verlet_ensure(s, reach, false)
+                                                           : verlet_build(s, s.kev[1], k_coef * dt, false);
     if (st) return st;
+    // a build re-homes the particles: k_prev again, in the new home order
+    if (s.vbuilds != builds)
+      if (spc2_status st2 = upload_aos(s, k_prev, 3, s.kev[1], 3)) return st2;
   }
-  build(s, s.k[1], k_coef * dt);
+  if (spc2_status st = upload_aos(s, k_next, 3, s.kev[0], 3)) return st;
+  build(s, s.kev[1], k_coef * dt);
   StepArgs sa = step_args(s, dt, 0);
-  rhs_pass1(s, EPI_K, s.k[0][0], s.k[0][1], s.k[0][2], sa, nullptr, nullptr, nullptr);
+  rhs_pass1(s, EPI_K, s.kev[0][0], s.kev[0][1], s.kev[0][2], sa, nullptr, nullptr, nullptr);
   if (spc2_status st = check_launch("eval_rhs")) return st;
-  return download_aos(s, k_next, 3, s.k[0], 3);
+  return download_aos(s, k_next, 3, s.kev[0], 3);
 }
 
 spc2_status spc2_eval_rhs_ito(spc2_sim_t h, double *derivative, double *additional) {
   if (check_handle(h) || !derivative || !additional) return fail(SPC2_ERR_INVALID, "null argument");
   Sim &s = h->s;
+  // the list first: a build re-homes the particles, so the in/out arrays are
+  // uploaded after it, in the home order the passes use
+  if (s.verlet)
+    if (spc2_status st = verlet_ensure(s, 0.0, false)) return st;
   if (spc2_status st = upload_aos(s, derivative, 3, s.der, 3)) return st;
   if (spc2_status st = upload_aos(s, additional, 2, s.add, 2)) return st;
-  if (s.verlet)
-    if (spc2_status st = verlet_ensure(s, 0.0, true)) return st;
   build(s, nullptr, 0.0);
   StepArgs sa = step_args(s, 0.0, 0);
   rhs_pass1(s, EPI_ITO1, s.der[0], s.der[1], s.der[2], sa, nullptr, nullptr, nullptr);

@@ -86,6 +86,8 @@ struct Sim {
   double *k[4][3] = {};
   // Ito outputs
   double *der[3] = {}, *add[2] = {};
+  // k_prev / k_next of the k-form evaluation entry point, allocated at its first call
+  double *kev[2][3] = {};
   // stage positions (wrapped for binning when linked), by particle
   double *gx = nullptr, *gy = nullptr, *gp = nullptr;
   // Particle arrays (state, omega, k, derivative, stage) are stored in "home"

@@ -260,6 +260,55 @@ def test_verlet_rk4_combine_rebuilt_k_equals_stored_k(gpu, monkeypatch, system, 
     assert np.array_equal(a, b), np.abs(a - b).max()
 
 
+@pytest.mark.parametrize("neighbour", ["cells", "verlet"])
+def test_eval_rhs_between_rk4_steps_leaves_the_stepper_alone(gpu, neighbour):
+    """A k-form evaluation with the caller's buffers between RK4 steps must not
+    touch the stepper's k_1..k_4, whose phi components a particle without
+    partners keeps from one step to the next (RungeKutta4StepperPtr.cpp:25-43
+    holds them as stepper members).  Sparse system: about 45% of the particles
+    have no partner within rho."""
+    n, rho, dt = 4000, 0.05, 0.01
+    L = math.sqrt(n / 100.0)
+    st, om = O.init_state(21, n, L, L, 0.5)
+    kw = dict(n=n, L=L, rho=rho, v0=0.02, sigma=1.0, alpha=1.45, D=0.0, seed=3)
+    p = spc2.params("chimera", "fp64", neighbour, **kw)
+    out = []
+    for interleave in (False, True):
+        with spc2.Sim(p) as s:
+            s.set_state(st)
+            s.set_omega(om)
+            s.step("rk4", dt, 0, 3)
+            if interleave:  # small stage reach: no list rebuild inside the call
+                s.eval_rhs(O.uniform(22, 3 * n, -0.01, 0.01), 0.5, dt, O.uniform(23, 3 * n, 5.0, 6.0))
+            s.step("rk4", dt, 3, 3)
+            out.append(s.get_state())
+    assert np.array_equal(out[0], out[1]), np.abs(out[0] - out[1]).max()
+
+
+def test_verlet_ito_eval_with_a_rebuild_inside_keeps_in_out_arrays(gpu):
+    """eval_rhs_ito's derivative and additional arrays are in/out.  When the call
+    rebuilds the Verlet list (here: the previous k-form evaluation built it from a
+    stage more than skin / 2 away), the particles are re-homed; the in/out values
+    must still reach their own particles."""
+    n, rho = 30000, 0.1
+    L = math.sqrt(n / 4000.0)
+    st = O.make_state(31, 1, n, L, rho)
+    kw = dict(n=n, L=L, rho=rho, v0=0.3, sigma=1.2, alpha=1.4, D=0.05)
+    p = spc2.params("chimera", "fp64", "verlet", **kw)
+    d_in, a_in = O.uniform(32, 3 * n, 1.0, 2.0), O.uniform(33, 2 * n, -3.0, 3.0)
+    with spc2.Sim(p) as s:
+        s.set_state(st)
+        s.eval_rhs(O.uniform(34, 3 * n, -1.0, 1.0), 1.0, 0.02, np.zeros(3 * n))  # reach above skin / 2
+        b0 = s.verlet_stats()[0]
+        d, a = s.eval_rhs_ito(d_in, a_in)
+        assert s.verlet_stats()[0] == b0 + 1  # the call rebuilt the list
+    with spc2.Sim(p) as s:
+        s.set_state(st)
+        d_ref, a_ref = s.eval_rhs_ito(d_in, a_in)
+    assert np.abs(d - d_ref).max() <= 1e-12 * np.abs(d_ref).max()
+    assert np.abs(a - a_ref).max() <= 1e-12 * np.abs(a_ref).max()
+
+
 def run_layout(params, st, method, dt, steps, monkeypatch, lds, omega=None, sort=None):
     """run() with the FP32 pass kernels chosen: lds=False keeps the gather
     passes (SPC2_VERLET_LDS=0 at creation); sort = "0" / "1" forces the LDS
