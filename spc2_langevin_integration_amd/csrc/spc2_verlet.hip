@@ -460,6 +460,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
                                                             const int *bh_off, int *vlist) {
   // the workgroup is one pass workgroup: its halo runs go to LDS once
   __shared__ int run[2 * kRunMax], off[kRunMax + 1];
+  __shared__ int qend[9][kBlock];  // per lane and stencil cell, below
+  __shared__ uint32_t ebase[9][kBlock];
   const int b = blockIdx.x, nrun = bh_nrun[b];
   if ((int)threadIdx.x < 2 * kRunMax) run[threadIdx.x] = bh_run[(size_t)b * 2 * kRunMax + threadIdx.x];
   if ((int)threadIdx.x <= kRunMax) off[threadIdx.x] = bh_off[(size_t)b * (kRunMax + 1) + threadIdx.x];
@@ -482,17 +484,58 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
     r2 = __builtin_amdgcn_alignbit(r3, r2, 16);
     r3 = __builtin_amdgcn_alignbit(ent, r3, 16);
   };
-  int e = 0, last = -1, base = 0;
-  for_each_build_neighbour<P32K>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
-    if (code != last) {
-      const int jc = adjust_cell(cy + (code >> 2) - 1, P.ny) * P.nx + adjust_cell(cx + (code & 3) - 1, P.nx);
-      base = halo_base(run, off, nrun, start, jc);
-      last = code;
-    }
-    push(EntLds::make(base + j, code));
+  int e = 0;
+  const auto emit = [&](uint32_t ent) {
+    push(ent);
     if ((e & 7) == 7) dst[(size_t)(e >> 3) * 64] = make_int4((int)r0, (int)r1, (int)r2, (int)r3);
     ++e;
-  });
+  };
+  // Per build cell k of the stencil, in visiting order: the end of its
+  // candidates in that order (qend) and the entry of candidate q less q << 4
+  // (ebase: the halo index base and the stencil code; the sum is exact modulo
+  // 2^32 and its low four bits stay the code)
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int dyo = k / 3 - 1, dxo = k % 3 - 1;
+    const int jc = adjust_cell(cy + dyo, P.ny) * P.nx + adjust_cell(cx + dxo, P.nx);
+    const int s0 = start[jc];
+    ebase[k][threadIdx.x] = EntLds::make(halo_base(run, off, nrun, start, jc) + s0 - total,
+                                         (dxo + 1) | ((dyo + 1) << 2));
+    total += start[jc + 1] - s0;
+    qend[k][threadIdx.x] = total;
+  }
+  if (total > 32 * mwords) {  // the count pass kept no bits for this home: test again
+    int last = -1, base = 0;
+    for_each_build_candidate<P32K>(P, start, t, i, rlist2, [&](int j, int code, bool in) {
+      if (!in) return;
+      if (code != last) {
+        const int jc = adjust_cell(cy + (code >> 2) - 1, P.ny) * P.nx + adjust_cell(cx + (code & 3) - 1, P.nx);
+        base = halo_base(run, off, nrun, start, jc);
+        last = code;
+      }
+      emit(EntLds::make(base + j, code));
+    });
+  } else {
+    // the candidate bits as one stream, word by word; candidate q lies in the
+    // first cell whose qend exceeds it (cells only advance)
+    int k = 0, qe = qend[0][threadIdx.x];
+    uint32_t eb = ebase[0][threadIdx.x];
+    for (int q0 = 0; q0 < total; q0 += 32) {
+      uint32_t m = mask[(size_t)(q0 >> 5) * P.n + i];
+      if (total - q0 < 32) m &= (1u << (total - q0)) - 1u;
+      while (m) {
+        const int q = q0 + __ffs(m) - 1;
+        m &= m - 1u;
+        while (q >= qe) {
+          ++k;
+          qe = qend[k][threadIdx.x];
+          eb = ebase[k][threadIdx.x];
+        }
+        emit(eb + ((uint32_t)q << 4));
+      }
+    }
+  }
   if (e & 7) {  // the last group, completed with sentinel entries
     for (int k = e & 7; k < 8; ++k) push(pad);
     dst[(size_t)(e >> 3) * 64] = make_int4((int)r0, (int)r1, (int)r2, (int)r3);
