@@ -119,18 +119,24 @@ __device__ __forceinline__ void for_each_build_candidate(const KParams &P, const
 // a home with more candidates is tested again by the fill.
 // Visits the entries of home i: calls f(j, code) for every candidate within
 // rlist, from the mask when it holds all candidates, else by testing again.
+// The mask's bits are read as one stream in visiting order; qend / jbase (the
+// caller's LDS, a column per lane) hold, per stencil cell, the end of its
+// candidates in that order and the home of candidate q less q.
 template <int PREC, class F>
 __device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const int *start, const Tiles &t, int i,
-                                                         double rlist2, const uint32_t *mask, int mwords, F &&f) {
+                                                         double rlist2, const uint32_t *mask, int mwords,
+                                                         int (*qend)[kBlock], int (*jbase)[kBlock], F &&f) {
   const int c = t.cell[i];
   const int cx = c % P.nx, cy = c / P.nx;
+  const int me = threadIdx.x;
   int total = 0;
-  for (int dyo = -1; dyo <= 1; ++dyo) {
-    const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
-    for (int dxo = -1; dxo <= 1; ++dxo) {
-      const int jc = row + adjust_cell(cx + dxo, P.nx);
-      total += start[jc + 1] - start[jc];
-    }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int jc = adjust_cell(cy + k / 3 - 1, P.ny) * P.nx + adjust_cell(cx + k % 3 - 1, P.nx);
+    const int s0 = start[jc];
+    jbase[k][me] = s0 - total;
+    total += start[jc + 1] - s0;
+    qend[k][me] = total;
   }
   if (total > 32 * mwords) {
     for_each_build_candidate<PREC>(P, start, t, i, rlist2, [&](int j, int code, bool in) {
@@ -138,25 +144,20 @@ __device__ __forceinline__ void for_each_build_neighbour(const KParams &P, const
     });
     return;
   }
-  int q0 = 0;
-  for (int dyo = -1; dyo <= 1; ++dyo) {
-    const int row = adjust_cell(cy + dyo, P.ny) * P.nx;
-    for (int dxo = -1; dxo <= 1; ++dxo) {
-      const int jc = row + adjust_cell(cx + dxo, P.nx);
-      const int code = (dxo + 1) | ((dyo + 1) << 2);
-      const int s0 = start[jc], q1 = q0 + start[jc + 1] - s0;
-      for (int q = q0; q < q1;) {
-        const int sh = q & 31, span = min(32 - sh, q1 - q);
-        uint32_t m = mask[(size_t)(q >> 5) * P.n + i] >> sh;
-        if (span < 32) m &= (1u << span) - 1u;
-        while (m) {
-          const int bit = __ffs(m) - 1;
-          m &= m - 1u;
-          f(s0 + (q - q0) + bit, code);
-        }
-        q += span;
+  int k = 0, qe = qend[0][me], jb = jbase[0][me], code = 0;  // cell k's stencil code (k % 3) | (k / 3) << 2
+  for (int q0 = 0; q0 < total; q0 += 32) {
+    uint32_t m = mask[(size_t)(q0 >> 5) * P.n + i];
+    if (total - q0 < 32) m &= (1u << (total - q0)) - 1u;
+    while (m) {
+      const int q = q0 + __ffs(m) - 1;
+      m &= m - 1u;
+      while (q >= qe) {
+        ++k;
+        qe = qend[k][me];
+        jb = jbase[k][me];
+        code = (k % 3) | ((k / 3) << 2);
       }
-      q0 = q1;
+      f(jb + q, code);
     }
   }
 }
@@ -246,6 +247,7 @@ template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *start, Tiles t, double rlist2,
                                                         const uint32_t *mask, int mwords, const int *wrow,
                                                         int *vlist) {
+  __shared__ int qend[9][kBlock], jbase[9][kBlock];  // per lane and stencil cell
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const int w = i >> 6, groups = wrow[w + 1] - wrow[w];
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill(KParams P, const int *st
   // (register moves instead of a switch on the entry's position)
   int r0 = self, r1 = self, r2 = self, r3 = self;
   int e = 0;
-  for_each_build_neighbour<PREC>(P, start, t, i, rlist2, mask, mwords, [&](int j, int code) {
+  for_each_build_neighbour<PREC>(P, start, t, i, rlist2, mask, mwords, qend, jbase, [&](int j, int code) {
     r0 = r1;
     r1 = r2;
     r2 = r3;
