@@ -58,6 +58,7 @@ def lib():
         L.orc_num_cells.restype = C.c_int32
         L.orc_cell_list.argtypes = [sp, C.c_int32, C.c_int32, dp, ip, ip, ip, C.POINTER(C.c_int64)]
         L.orc_rhs_k.argtypes = [sp, dp, dp, C.c_double, C.c_double, dp, C.c_void_p]
+        L.orc_rhs_k_scaled.argtypes = [sp, dp, dp, C.c_double, C.c_double, dp, C.c_void_p, C.c_void_p]
         L.orc_rhs_ito.argtypes = [sp, dp, dp, dp, C.c_void_p]
         L.orc_step_se.argtypes = [sp, dp, C.c_double, C.c_void_p, dp, C.c_int32]
         L.orc_step_srk.argtypes = [sp, dp, C.c_double, C.c_void_p, dp]
@@ -71,7 +72,7 @@ def lib():
         L.orc_mt_seed.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_mt_uniform.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_double, dp]
         L.orc_mt_normals.argtypes = [C.c_void_p, C.c_int64, dp]
-        for f in ("orc_cell_list", "orc_rhs_k", "orc_rhs_ito", "orc_step_se", "orc_step_srk", "orc_step_rk4"):
+        for f in ("orc_cell_list", "orc_rhs_k", "orc_rhs_k_scaled", "orc_rhs_ito", "orc_step_se", "orc_step_srk", "orc_step_rk4"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -138,14 +139,17 @@ def cell_list(sysp, pos, ncx=None, ncy=None):
     return cell_of, start, srt, clamped.value
 
 
-def rhs_k(sysp, state, k_prev, k_coef, dt, k_next):
+def rhs_k(sysp, state, k_prev, k_coef, dt, k_next, term_scale=False):
+    """k-form RHS; with term_scale=True also returns each particle's mean
+    absolute phase-derivative term (0 without partners)."""
     counts = np.zeros(3, np.int64)
     k_next = np.array(k_next, np.float64, copy=True)
     kp = k_next if k_prev is k_next else np.ascontiguousarray(k_prev, np.float64)
-    rc = lib().orc_rhs_k(C.byref(sysp), np.ascontiguousarray(state, np.float64), kp, k_coef, dt, k_next,
-                         counts.ctypes.data_as(C.c_void_p))
+    ts = np.zeros(sysp.n, np.float64) if term_scale else None
+    rc = lib().orc_rhs_k_scaled(C.byref(sysp), np.ascontiguousarray(state, np.float64), kp, k_coef, dt, k_next,
+                                counts.ctypes.data_as(C.c_void_p), _ptr(ts))
     assert rc == 0
-    return k_next, counts
+    return (k_next, counts, ts) if term_scale else (k_next, counts)
 
 
 def rhs_ito(sysp, state, derivative=None, additional=None):

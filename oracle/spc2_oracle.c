@@ -236,6 +236,9 @@ typedef struct rhs_ctx {
   double *out;         /* k_next or derivative (AoS, stride 3) */
   double *sumA, *card; /* chimera: alignment_force_, neighborhood_cardinality_ */
   double *card3;       /* vortex: neighborhood_cardinality_[i][0..2] */
+  /* per-particle term scale (may be NULL): (1/n) sum |term| of the phase
+   * derivative, the normalisation the parity bars use */
+  double *tscale;
   /* pass 2 */
   double *additional;
   int64_t *cnt;        /* per-thread counts are summed under no lock: per-slice arrays */
@@ -293,7 +296,7 @@ static void chimera_pass1(void *vp, int64_t lo, int64_t hi) {
   for (int64_t i = lo; i < hi; ++i) {
     double xi, yi, phii;
     read_pos(c, i, &xi, &yi, &phii);
-    double A = 0.0, n = 0.0;
+    double A = 0.0, n = 0.0, T = 0.0;
     FOR_EACH_PARTNER(c, i, xi, yi, {
       double xj, yj, phij, dx, dy;
       read_pos(c, j, &xj, &yj, &phij);
@@ -301,11 +304,14 @@ static void chimera_pass1(void *vp, int64_t lo, int64_t hi) {
       double dr2 = dx * dx + dy * dy;
       ++cand;
       if (dr2 <= rho2) {
-        A += sin(phij - phii - s->alpha);
+        double sn = sin(phij - phii - s->alpha);
+        A += sn;
+        T += fabs(sn);
         n += 1.0;
         ++hits;
       }
     });
+    if (c->tscale) c->tscale[i] = n != 0.0 ? fabs(s->sigma) * T / n : 0.0;
     c->sumA[i] = A;
     c->card[i] = n;
     c->out[KS * i] = s->v0 * cos(phii);
@@ -369,7 +375,7 @@ static void vortex_pass1(void *vp, int64_t lo, int64_t hi) {
   for (int64_t i = lo; i < hi; ++i) {
     double xi, yi, phii;
     read_pos(c, i, &xi, &yi, &phii);
-    double A = 0.0, B = 0.0, C = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    double A = 0.0, B = 0.0, C = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0, TA = 0.0, TC = 0.0;
     FOR_EACH_PARTNER(c, i, xi, yi, {
       double xj, yj, phij, dx, dy;
       read_pos(c, j, &xj, &yj, &phij);
@@ -377,20 +383,27 @@ static void vortex_pass1(void *vp, int64_t lo, int64_t hi) {
       double dr = sqrt(dx * dx + dy * dy);
       ++cand;
       if (dr <= s->xi_a) {
-        A += vortex_w(s, dr) * sin(phij - phii);
+        double t = vortex_w(s, dr) * sin(phij - phii);
+        A += t;
+        TA += fabs(t);
         n0 += 1.0;
         ++hits;
       } else if (dr <= s->rho) {
-        B += vortex_w(s, dr) * sin(phij - phii);
+        double t = vortex_w(s, dr) * sin(phij - phii);
+        B += t;
+        TA += fabs(t);
         n1 += 1.0;
         ++hits;
       }
       if (dr <= s->xi_r) {
-        C += -s->kappa * sin(atan2(dy, dx) - phii);
+        double t = -s->kappa * sin(atan2(dy, dx) - phii);
+        C += t;
+        TC += fabs(t);
         n2 += 1.0;
         ++coh;
       }
     });
+    if (c->tscale) c->tscale[i] = ((n0 + n1) > 0 ? TA / (n0 + n1) : 0.0) + (n2 > 0 ? TC / n2 : 0.0);
     c->card3[3 * i] = n0;
     c->card3[3 * i + 1] = n1;
     c->card3[3 * i + 2] = n2;
@@ -478,6 +491,11 @@ static void free_cells(cell_bufs *b) {
  * k_next is in/out: its phi entry is left as is when no partner is found. */
 int orc_rhs_k(const orc_sys *s, const double *state, const double *k_prev, double k_coef, double dt, double *k_next,
               int64_t *counts) {
+  return orc_rhs_k_scaled(s, state, k_prev, k_coef, dt, k_next, counts, NULL);
+}
+
+int orc_rhs_k_scaled(const orc_sys *s, const double *state, const double *k_prev, double k_coef, double dt,
+                     double *k_next, int64_t *counts, double *tscale) {
   int64_t n = s->n;
   int64_t cnt[3] = {0, 0, 0};
   rhs_ctx c;
@@ -487,6 +505,7 @@ int orc_rhs_k(const orc_sys *s, const double *state, const double *k_prev, doubl
   c.use_ll = use_linked_list(s);
   c.out = k_next;
   c.cnt = cnt;
+  c.tscale = tscale;
   double *stage = NULL;
   cell_bufs cb;
   memset(&cb, 0, sizeof(cb));

@@ -50,6 +50,12 @@ int orc_cell_list(const orc_sys *s, int32_t ncx, int32_t ncy, const double *pos_
  * counts[2] = cohesion pairs (vortex r <= xi_r); may be NULL. */
 int orc_rhs_k(const orc_sys *s, const double *state, const double *k_prev, double k_coef, double dt, double *k_next,
               int64_t *counts);
+/* orc_rhs_k that also writes each particle's term scale (may be NULL): the
+ * mean absolute term of its phase derivative, sigma (1/n) sum |sin| (chimera)
+ * or (1/(n0+n1)) sum |W sin| + (1/n2) sum |kappa sin| (vortex arrays); 0 for a
+ * particle without partners. */
+int orc_rhs_k_scaled(const orc_sys *s, const double *state, const double *k_prev, double k_coef, double dt,
+                     double *k_next, int64_t *counts, double *tscale);
 int orc_rhs_ito(const orc_sys *s, const double *state, double *derivative, double *additional, int64_t *counts);
 
 /* ---- steppers (state updated in place, engine wrap applied after) ------ */

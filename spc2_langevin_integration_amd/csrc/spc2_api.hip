@@ -527,6 +527,16 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
       delete h;
       return fail(SPC2_ERR_INVALID, "Verlet lists support n < 2^27");
     }
+    if (p->system == SPC2_VORTEX && p->rho <= 0.25 * p->Lx && p->rho <= 0.25 * p->Ly) {
+      // the reference finds cohesion partners only within its 3 x 3 stencil of
+      // cells of side L / int(L / rho) (VortexArraysSystemPtr.cpp:35-36, :449-489);
+      // a list reaching further would add partners the reference never sees
+      const double hx = p->Lx / (int)(p->Lx / p->rho), hy = p->Ly / (int)(p->Ly / p->rho);
+      if (p->xi_r > std::min(hx, hy)) {
+        delete h;
+        return fail(SPC2_ERR_INVALID, "Verlet lists need xi_r <= the linked cells' side L / int(L / rho)");
+      }
+    }
     s.skin = p->verlet_skin > 0.0 ? p->verlet_skin : 0.2 * p->rho;
     s.rlist = (p->system == SPC2_VORTEX ? std::max(p->rho, p->xi_r) : p->rho) + s.skin;
     double fx = p->Lx / s.rlist, fy = p->Ly / s.rlist;

@@ -270,6 +270,9 @@ long CallsUntilWrite(long counter, int every) {
 BinaryObserver::BinaryObserver(const std::string &state_path, const std::string &summary_path, int state_every,
                                int summary_every, bool float_summary)
     : state_every_(state_every), summary_every_(summary_every), float_summary_(float_summary) {
+  // the output cadences are divisors (BinaryObserverPtr.cpp:153, :232)
+  if (!state_path.empty() && state_every <= 0) throw Spc2Error("BinaryObserver: state_every must be > 0");
+  if (!summary_path.empty() && summary_every <= 0) throw Spc2Error("BinaryObserver: summary_every must be > 0");
   if (!state_path.empty()) {
     state_file_.open(state_path, std::ios::binary | std::ios::out | std::ios::trunc);
     if (!state_file_.is_open()) throw Spc2Error("cannot open " + state_path);

@@ -102,11 +102,12 @@ int main(int argc, char **argv) {
     p.sigma = 0.2 + 1.8 * u01(rng);
     p.alpha = 1.2 + 0.37 * u01(rng);
     p.D = std::exp(std::log(0.001) + u01(rng) * (std::log(0.1) - std::log(0.001)));
-    if (p.system == SPC2_VORTEX) {  // the reference's set (SimulationEnginePtr.cpp:131-139) at rho = 0.1
-      p.v0 = 0.05;
+    if (p.system == SPC2_VORTEX) {  // the reference's set (SimulationEnginePtr.cpp:131-139) scaled to rho = 0.05
+      p.rho = 0.05;
+      p.v0 = 0.025;
       p.mu_plus = p.mu_minus = 100.0;
-      p.xi_a = 0.02;
-      p.xi_r = 0.01;
+      p.xi_a = 0.01;
+      p.xi_r = 0.005;
       p.kappa = 20.0;
     }
     p.seed = a.seed + (uint64_t)submitted;

@@ -40,3 +40,12 @@ def test_create_rejects_bad_parameters_before_touching_the_gpu():
     for kw in ({"n": 0}, {"rho": -1.0}, {"L": 0.0}, {"D": -0.1}):
         with pytest.raises(spc2.Spc2Error):
             spc2.Sim(spc2.params(**kw))
+
+
+def test_verlet_rejects_cohesion_range_beyond_the_reference_stencil():
+    """With linked cells of side h = L / int(L / rho) the reference sees
+    cohesion partners only within its 3 x 3 stencil; a Verlet list must not
+    reach beyond h (rejected before any device call)."""
+    kw = dict(n=1000, L=1.0, rho=0.05, velocity=0.025, mu_plus=100.0, mu_minus=100.0, xi_a=0.01, kappa=20.0)
+    with pytest.raises(spc2.Spc2Error, match="xi_r"):
+        spc2.Sim(spc2.params("vortex", "fp64", "verlet", xi_r=0.0501, **kw))

@@ -1,15 +1,26 @@
-"""Batched replicas (spc2_batch_*) against single handles.
+"""Batched replicas (spc2_batch_*) against the oracle and against single handles.
+
+Every replica's Stochastic Euler step (StochasticEulerStepperPtr.hpp:32-76,
+with the natural frequency added as (omega + a_phi) dt for the chimera) must
+meet the single-step bars against the oracle's step_se with the same Philox
+normals: 1e-12 (FP64) and 1e-5 (FP32-compensated) of the increment scale
+(tests/_parity.py), replicas on the linked-cell and on the all-pairs path alike.
 
 A replica's arithmetic, summation order and noise keys do not depend on the
-batch it runs in, so every replica must match a single handle created with the
-same parameters bit for bit: after plain steps, with per-replica step limits,
-and after the batch is re-packed with other replicas (continuous batching)."""
+batch it runs in, so every replica must also match a single handle created with
+the same parameters bit for bit: after plain steps, with per-replica step
+limits, and after the batch is re-packed with other replicas (continuous
+batching)."""
 import math
+import os
 
 import numpy as np
 import pytest
 
+import oracle as O
 import spc2_langevin_integration_amd as spc2
+from _parity import TOL, assert_state
+from spc2_langevin_integration_amd.ensemble import sweep
 
 pytestmark = pytest.mark.gpu
 
@@ -32,6 +43,83 @@ def replica(system, n, prec, seed, sigma=1.0, alpha=1.45, D=0.01):
 MIX = [("chimera", 500, 1, 0.2, 1.2, 0.001), ("chimera", 633, 2, 2.0, 1.57, 0.1), ("vortex", 620, 3, 1, 1, 0.02),
        ("chimera", 1000, 4, 1.0, 1.45, 0.01), ("vortex", 5000, 5, 1, 1, 0.05), ("chimera", 7000, 6, 0.6, 1.3, 0.03),
        ("chimera", 20000, 7, 1.4, 1.5, 0.002), ("vortex", 16384, 8, 1, 1, 0.0)]
+
+
+def oracle_sys(p):
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if p.system == spc2.CHIMERA:
+        return O.chimera(p.n, p.Lx, p.rho, p.v0, p.sigma, p.alpha, p.D, threads=threads)
+    return O.vortex(p.n, p.Lx, rho=p.rho, velocity=p.v0, mu_plus=p.mu_plus, mu_minus=p.mu_minus, xi_a=p.xi_a,
+                    xi_r=p.xi_r, kappa=p.kappa, D=p.D, threads=threads)
+
+
+def oracle_steps_match(b, reps, which, dt, steps, omega_std, prec):
+    """Steps the batch `steps` times, each step from the oracle's states of the
+    replicas in `which`, and checks each replica's step against step_se."""
+    cur, om, sysp = {}, {}, {}
+    for r in which:
+        p = reps[r]
+        cur[r], om[r] = O.init_state(p.seed, p.n, p.Lx, p.Ly, omega_std)
+        sysp[r] = oracle_sys(p)
+        assert np.array_equal(b.get_state(r), cur[r]), f"replica {r}: initial state"
+    worst = 0.0
+    for k in range(steps):
+        for r in which:
+            b.set_state(r, cur[r])
+        b.step(dt, 1)
+        for r in which:
+            p = reps[r]
+            chim = p.system == spc2.CHIMERA
+            ref = O.step_se(sysp[r], cur[r], dt, O.philox_normals(p.seed, k, 0, p.n, 1), omega=om[r] if chim else None)
+            worst = max(worst, assert_state(b.get_state(r), ref, cur[r], p.Lx, TOL[prec],
+                                            f"replica {r} ({'chimera' if chim else 'vortex'} N={p.n}) step {k}"))
+            cur[r] = ref
+    return worst
+
+
+@pytest.mark.parametrize("prec", ["fp32k", "fp64"])
+def test_batch_replicas_match_oracle(gpu, prec):
+    """Every replica of the mix (chimera and vortex, three of them on the
+    all-pairs path) against the oracle, three steps."""
+    reps = [replica(sy, n, prec, seed, sg, al, D) for sy, n, seed, sg, al, D in MIX]
+    assert sum(p.rho > 0.25 * p.Lx for p in reps) == 3
+    with spc2.Batch(reps) as b:
+        b.init_random_state(0.1)
+        worst = oracle_steps_match(b, reps, range(len(reps)), 0.01, 3, 0.1, prec)
+    print(f"batch/{prec}: largest error / bar {worst:.3e}")
+
+
+def test_batch_full_sweep(gpu):
+    """The 512-replica sweep (N log-uniform in [500, 32768], density 4000,
+    16 x 4 x 8 grid of sigma, alpha and D): its all-pairs replicas are the
+    N < 640 ones (rho > L / 4); sampled replicas over the N range, all-pairs
+    ones among them, step as the oracle does and end bit-identical to single
+    handles."""
+    reps = sweep(512)
+    ns = np.array([p.n for p in reps])
+    allpairs = ns < 640
+    assert np.array_equal(allpairs, np.array([p.rho > 0.25 * p.Lx for p in reps]))
+    assert 15 <= allpairs.sum() <= 50 and 3.5e6 < ns.sum() < 4.4e6
+    order = np.argsort(ns)
+    sample = sorted(set(order[np.linspace(0, len(order) - 1, 6).astype(int)].tolist())
+                    | set(np.nonzero(allpairs)[0][:2].tolist()))
+    dt, omega_std = 0.01, 0.1
+    with spc2.Batch(reps) as b:
+        b.init_random_state(omega_std)
+        worst = oracle_steps_match(b, reps, sample, dt, 2, omega_std, "fp32k")
+        mid = {r: b.get_state(r) for r in sample}
+        b.step(dt, 3)
+        assert (b.steps_done() == 5).all()
+        finals = {r: b.get_state(r) for r in sample}
+    print(f"sweep: {int(allpairs.sum())} all-pairs replicas, {int(ns.sum())} particles; "
+          f"largest error / bar {worst:.3e}")
+    for r in sample:
+        p = reps[r]
+        with spc2.Sim(p) as s:
+            s.init_random_state(omega_std)   # the natural frequencies
+            s.set_state(mid[r])              # the replica's state after its two checked steps
+            s.step("se", dt, 2, 3)
+            assert np.array_equal(s.get_state(), finals[r]), f"replica {r} (N={p.n})"
 
 
 def single(p, steps, dt, omega_std):

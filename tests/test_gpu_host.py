@@ -80,6 +80,29 @@ def test_device_loop_matches_oracle_trajectory(gpu, tmp_path):
         assert line[1] == pytest.approx(R, rel=5e-6) and line[2] == pytest.approx(psi, rel=5e-6, abs=1e-6)
 
 
+def test_float_summary_matches_oracle_f32(gpu, tmp_path):
+    """float_summary=1 writes the reference's own float order parameter
+    (BinaryObserverPtr.cpp:237-247: complex<float> sum of float cos and sin in
+    index order, divided by N) as text, t, |R| and arg R at the stream's default
+    6 significant digits: the same text as the oracle's orc_order_param_f32 of
+    the oracle's trajectory."""
+    n, L, dt, steps = 4000, 1.0, 0.005, 20
+    _, _, _, _, summary = run(tmp_path, "f32", method="srk", precision="fp64", dt=dt, steps=steps, seed=42,
+                              omega_std=0.1, state_every=10, summary_every=5, float_summary=1, **CHIM)
+    sysp = O.chimera(n, L, 0.1, 0.01, 1.0, 1.45, 0.01, threads=8)
+    st, om = O.init_state(42, n, L, L, 0.1)
+    traj, t = [st], [0.0]
+    for k in range(steps):
+        traj.append(O.step_srk(sysp, traj[-1], dt, O.philox_normals(42, k, 0, n, 2), omega=om))
+        t.append(t[-1] + dt)
+    lines = summary.decode().splitlines()
+    assert len(lines) == 4
+    for line, j in zip(lines, (1, 6, 11, 16)):
+        R, psi = O.order_param_f32(traj[j])
+        expect = "\t".join(f"{float(v):g}" for v in (np.float32(t[j]), np.float32(R), np.float32(psi)))
+        assert line == expect, (j, line, expect)
+
+
 VORT = dict(system="vortex", n=3000, L=1.0, rho=0.1, v0=0.1, mu_plus=100.0, mu_minus=100.0, xi_a=0.04,
             xi_r=0.02, kappa=20.0, D=0.01)
 

@@ -1,13 +1,19 @@
 """The continuous-batching request loop (spc2_server_*): requests that arrive
 while others run join the batch as slots free up, and every result equals the
-same request run alone through a single handle, bit for bit."""
+same request run alone through a single handle, bit for bit, and the oracle's
+Stochastic Euler trajectory from the same Philox initial state and noise within
+the single-step bars accumulated over the request's steps (1e-12 FP64, 1e-5
+FP32-compensated; tests/_parity.py)."""
 import math
+import os
 import time
 
 import numpy as np
 import pytest
 
+import oracle as O
 import spc2_langevin_integration_amd as spc2
+from _parity import TOL, periodic_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -19,12 +25,59 @@ def gpu():
         s.init_random_state()
 
 
-def request(system, n, seed, sigma=1.0, alpha=1.45, D=0.02):
+def request(system, n, seed, sigma=1.0, alpha=1.45, D=0.02, prec="fp32k"):
+    """A request at density 4000: chimera with rho = 0.1, vortex arrays with
+    the reference's parameter set scaled to rho = 0.05."""
     L = math.sqrt(n / 4000.0)
     if system == "chimera":
-        return spc2.params("chimera", "fp32k", n=n, L=L, rho=0.1, v0=0.01, sigma=sigma, alpha=alpha, D=D, seed=seed)
-    return spc2.params("vortex", "fp32k", n=n, L=L, rho=0.1, velocity=0.05, mu_plus=100.0, mu_minus=100.0,
-                       xi_a=0.02, xi_r=0.01, kappa=20.0, D=D, seed=seed)
+        return spc2.params("chimera", prec, n=n, L=L, rho=0.1, v0=0.01, sigma=sigma, alpha=alpha, D=D, seed=seed)
+    return spc2.params("vortex", prec, n=n, L=L, rho=0.05, velocity=0.025, mu_plus=100.0, mu_minus=100.0,
+                       xi_a=0.01, xi_r=0.005, kappa=20.0, D=D, seed=seed)
+
+
+def oracle_run(p, steps, dt, omega_std):
+    """The oracle's trajectory of a request and the sum over its steps of each
+    component's increment scale."""
+    threads = max(1, min(16, os.cpu_count() or 1))
+    chim = p.system == spc2.CHIMERA
+    if chim:
+        sysp = O.chimera(p.n, p.Lx, p.rho, p.v0, p.sigma, p.alpha, p.D, threads=threads)
+    else:
+        sysp = O.vortex(p.n, p.Lx, rho=p.rho, velocity=p.v0, mu_plus=p.mu_plus, mu_minus=p.mu_minus, xi_a=p.xi_a,
+                        xi_r=p.xi_r, kappa=p.kappa, D=p.D, threads=threads)
+    st, om = O.init_state(p.seed, p.n, p.Lx, p.Ly, omega_std)
+    inc = np.zeros(3)
+    for k in range(steps):
+        nxt = O.step_se(sysp, st, dt, O.philox_normals(p.seed, k, 0, p.n, 1), omega=om if chim else None)
+        d = np.abs(periodic_diff(nxt, st, p.Lx))
+        inc += [d[c::3].max() for c in range(3)]
+        st = nxt
+    return st, inc
+
+
+@pytest.mark.parametrize("prec,steps", [("fp64", (12, 9, 15, 7, 11)), ("fp32k", (1, 2, 1, 2, 1))])
+def test_served_requests_match_oracle(gpu, prec, steps):
+    """Five requests through a two-slot server (so that they queue and mix),
+    each against the oracle.  The FP32 requests take one or two steps: over more
+    steps a rounding difference can carry a pair across a cutoff, whose effect
+    is not a rounding error."""
+    dt, omega_std = 0.01, 0.1
+    specs = [("chimera", 3000, 21, 1.2), ("vortex", 2000, 22, 1.0), ("chimera", 1000, 23, 0.4),
+             ("vortex", 1200, 24, 1.0), ("chimera", 5000, 25, 1.9)]
+    reqs = [request(sy, n, seed, sigma=sg, prec=prec) for sy, n, seed, sg in specs]
+    with spc2.Server(max_replicas=2, super_steps=4, snapshot_every=0, keep_final=1, dt=dt,
+                     omega_std=omega_std) as srv:
+        ids = [srv.submit(p, k) for p, k in zip(reqs, steps)]
+        results = [srv.wait(rid)[0] for rid in ids]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["max_in_flight"] == 2
+    for got, p, k in zip(results, reqs, steps):
+        ref, inc = oracle_run(p, k, dt, omega_std)
+        d = np.abs(periodic_diff(got, ref, p.Lx))
+        for c, name in ((0, "x"), (1, "y"), (2, "phi")):
+            bar = TOL[prec] * inc[c] + 2 * k * np.spacing(np.abs(ref[c::3]).max())
+            assert d[c::3].max() <= bar, (f"{prec} request N={p.n} ({k} steps): {name} error "
+                                          f"{d[c::3].max():.3e} > {bar:.3e}")
 
 
 WAVES = [

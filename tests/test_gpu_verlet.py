@@ -14,6 +14,7 @@ import pytest
 
 import oracle as O
 import spc2_langevin_integration_amd as spc2
+from _parity import assert_rhs_terms
 
 pytestmark = pytest.mark.gpu
 
@@ -35,12 +36,6 @@ def periodic_diff(a, b, L):
     return d
 
 
-def assert_rhs(gpu, ref, v0, phi_scale, tol, what):
-    dv = np.abs(gpu - ref)
-    assert max(dv[0::3].max(), dv[1::3].max()) <= tol * v0, f"{what}: velocity"
-    assert dv[2::3].max() <= tol * phi_scale, f"{what}: phase derivative {dv[2::3].max():.3e}"
-
-
 @pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
 @pytest.mark.parametrize("kc", [0.5, 1.0])  # stage reach below / above skin / 2 (list from the stage)
 def test_verlet_chimera_rhs_k(gpu, prec, kc):
@@ -50,7 +45,8 @@ def test_verlet_chimera_rhs_k(gpu, prec, kc):
     kp = O.uniform(170, 3 * n, -1.0, 1.0)
     kn = O.uniform(171, 3 * n, 5.0, 6.0)
     dt = 0.01  # reach |kc dt k| ~ 0.007 (kc = 0.5) or 0.014 (kc = 1) against skin / 2 = 0.01
-    ref, rc = O.rhs_k(O.chimera(n, L, rho, 0.3, 1.2, 1.4, 0.05, threads=8), st, kp, kc, dt, kn)
+    ref, rc, ts = O.rhs_k(O.chimera(n, L, rho, 0.3, 1.2, 1.4, 0.05, threads=8), st, kp, kc, dt, kn,
+                          term_scale=True)
     p = spc2.params("chimera", prec, "verlet", n=n, L=L, rho=rho, v0=0.3, sigma=1.2, alpha=1.4, D=0.05)
     with spc2.Sim(p) as s:
         s.set_state(st)
@@ -58,7 +54,7 @@ def test_verlet_chimera_rhs_k(gpu, prec, kc):
         out = s.eval_rhs(kp, kc, dt, kn)
         cnt = s.counts()
         builds, entries = s.verlet_stats()
-    assert_rhs(out, ref, 0.3, 1.2, TOL[prec], f"chimera verlet/{prec}")
+    assert_rhs_terms(out, ref, ts, 0.3, TOL[prec], f"chimera verlet/{prec}", ref[2::3] == kn[2::3])
     assert np.array_equal(cnt[1:], rc[1:]), (cnt, rc)
     assert builds == (2 if kc == 1.0 else 1) and entries == cnt[0]
 
@@ -71,8 +67,9 @@ def test_verlet_vortex_rhs_both_forms(gpu, prec):
     sysp = O.vortex(n, L, rho=rho, D=D, threads=8, **VORT)
     kp = O.uniform(230, 3 * n, -1.0, 1.0)
     kn = O.uniform(231, 3 * n, 5.0, 6.0)
-    refk, rck = O.rhs_k(sysp, st, kp, 0.5, 0.01, kn)
+    refk, rck, tsk = O.rhs_k(sysp, st, kp, 0.5, 0.01, kn, term_scale=True)
     rd, ra, rci = O.rhs_ito(sysp, st)
+    _, _, tsi = O.rhs_k(sysp, st, np.zeros(3 * n), 0.0, 0.01, np.zeros(3 * n), term_scale=True)
     p = spc2.params("vortex", prec, "verlet", n=n, L=L, rho=rho, D=D, **VORT)
     with spc2.Sim(p) as s:
         s.set_state(st)
@@ -83,8 +80,8 @@ def test_verlet_vortex_rhs_both_forms(gpu, prec):
         gd, ga = s.eval_rhs_ito()
         ci = s.counts()
     tol = TOL[prec]
-    assert_rhs(outk, refk, 0.025, 120.0, tol, f"vortex verlet k/{prec}")
-    assert_rhs(gd, rd, 0.025, 120.0, tol, f"vortex verlet ito/{prec}")
+    assert_rhs_terms(outk, refk, tsk, 0.025, tol, f"vortex verlet k/{prec}", refk[2::3] == kn[2::3])
+    assert_rhs_terms(gd, rd, tsi, 0.025, tol, f"vortex verlet ito/{prec}")
     for c in (0, 1):
         scale = np.abs(ra[c::2]).max()
         assert np.abs(ga[c::2] - ra[c::2]).max() <= tol * scale, f"vortex verlet ito/{prec} additional[{c}]"
@@ -392,7 +389,8 @@ def test_verlet_lds_clustered_state_keeps_gather(gpu):
     st[1::3] = 1.0 + u[1::2]
     kp = O.uniform(311, 3 * n, -1.0, 1.0)
     kn = O.uniform(312, 3 * n, 5.0, 6.0)
-    ref, rc = O.rhs_k(O.chimera(n, L, rho, 0.3, 1.2, 1.4, 0.05, threads=8), st, kp, 0.5, 0.01, kn)
+    ref, rc, ts = O.rhs_k(O.chimera(n, L, rho, 0.3, 1.2, 1.4, 0.05, threads=8), st, kp, 0.5, 0.01, kn,
+                          term_scale=True)
     p = spc2.params("chimera", "fp32k", "verlet", n=n, L=L, rho=rho, v0=0.3, sigma=1.2, alpha=1.4, D=0.05)
     with spc2.Sim(p) as s:
         s.set_state(st)
@@ -400,7 +398,7 @@ def test_verlet_lds_clustered_state_keeps_gather(gpu):
         s.set_counting(True)
         out = s.eval_rhs(kp, 0.5, 0.01, kn)
         cnt = s.counts()
-    assert_rhs(out, ref, 0.3, 1.2, TOL["fp32k"], "clustered chimera verlet fp32k")
+    assert_rhs_terms(out, ref, ts, 0.3, TOL["fp32k"], "clustered chimera verlet fp32k", ref[2::3] == kn[2::3])
     assert np.array_equal(cnt[1:], rc[1:]), (cnt, rc)
 
 
