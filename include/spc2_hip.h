@@ -158,6 +158,15 @@ spc2_status spc2_host_free(void *p);
  * (StochasticRungeKuttaStepperPtr.hpp:49-56). */
 spc2_status spc2_noise(spc2_sim_t h, int32_t method, int64_t step, double *out);
 
+/* spc2_noise without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The normals are staged in a
+ * fixed 16 MiB array per GPU, held until the process ends, so a call takes at
+ * most 2^21 normals (N <= 2097152 for SE, 1048576 for SRK) and fails with
+ * SPC2_ERR_INVALID beyond that.  Calls on one GPU share the array: calls that
+ * overlap in time on one GPU (from several threads) race. */
+spc2_status spc2_noise_noalloc(spc2_sim_t h, int32_t method, int64_t step, double *out);
+
 /* Kuramoto order parameter R e^{i psi} = (1/N) sum e^{i phi_j}, FP64.
  * Replaces the reduction in BinaryObserverPtr::SaveSummaryStatistics
  * (BinaryObserverPtr.cpp:237-247), which sums in float. */

@@ -72,6 +72,7 @@ def load():
         "spc2_host_alloc": (st, [C.c_uint64, C.POINTER(C.c_void_p)]),
         "spc2_host_free": (st, [C.c_void_p]),
         "spc2_noise": (st, [h, C.c_int32, C.c_int64, dp]),
+        "spc2_noise_noalloc": (st, [h, C.c_int32, C.c_int64, dp]),
         "spc2_order_parameter": (st, [h, dp, dp]),
         "spc2_get_cell_list": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "spc2_num_cells": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -228,6 +229,13 @@ class Sim:
         m = METHODS[method] if isinstance(method, str) else int(method)
         out = np.empty(self.n * (2 if m == SRK else 1), np.float64)
         _check(load().spc2_noise(self.h, m, step, _dp(out)))
+        return out
+
+    def noise_noalloc(self, method, step):
+        """noise() through the entry point that allocates no device memory."""
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        out = np.empty(self.n * (2 if m == SRK else 1), np.float64)
+        _check(load().spc2_noise_noalloc(self.h, m, step, _dp(out)))
         return out
 
     def order_parameter(self):

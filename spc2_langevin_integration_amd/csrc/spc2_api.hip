@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -471,6 +472,37 @@ spc2_status check_handle(spc2_sim_t h) {
   return SPC2_OK;
 }
 
+// Device staging of spc2_noise_noalloc: one 16 MiB array per GPU, held until
+// the process ends (2^21 normals: the SRK draws of N = 1,048,576 particles).
+// Host code never passes `scratch` itself to a kernel or copy (on the host the
+// name is a stand-in, not a device address): it uses the address that
+// hipGetSymbolAddress returns for the current device, kept per device in
+// g_scratch.
+constexpr size_t kScratchNormals = size_t(1) << 21;
+static __device__ __align__(256) double scratch[kScratchNormals];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kScratchDevices] = {};
+
+// Looks up (once per device, with that device current) the address of
+// `scratch` on `device`.
+spc2_status scratch_lookup(int device) {
+  if (device < 0 || device >= kScratchDevices) return fail(SPC2_ERR_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return SPC2_OK;
+  int cur = -1;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != device) HIP_TRY(hipSetDevice(device));
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != device) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !p)
+    return fail(SPC2_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = static_cast<double *>(p);
+  return SPC2_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -573,6 +605,13 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
     std::string m = std::string("HIP init: ") + hipGetErrorString(e);
     delete h;
     return fail(SPC2_ERR_HIP, m);
+  }
+  // set-up of the device's staging array for spc2_noise_noalloc (first handle
+  // on a device), so that the entry point itself makes no lookup
+  if (spc2_status st = scratch_lookup(s.device)) {
+    (void)hipStreamDestroy(s.stream);
+    delete h;
+    return st;
   }
   spc2_status st = alloc_all(s);
   if (st == SPC2_OK) {
@@ -809,6 +848,25 @@ spc2_status spc2_noise(spc2_sim_t h, int32_t method, int64_t step, double *out) 
   (void)hipFree(buf);
   if (st) return st;
   if (e != hipSuccess) return fail(SPC2_ERR_HIP, std::string("noise copy: ") + hipGetErrorString(e));
+  return SPC2_OK;
+}
+
+spc2_status spc2_noise_noalloc(spc2_sim_t h, int32_t method, int64_t step, double *out) {
+  if (check_handle(h) || !out) return fail(SPC2_ERR_INVALID, "null argument");
+  if (step < 0) return fail(SPC2_ERR_INVALID, "step must be >= 0");
+  Sim &s = h->s;
+  const int npp = method == SPC2_SRK ? 2 : 1;
+  const size_t need = (size_t)npp * (size_t)s.n;
+  if (need > kScratchNormals)
+    return fail(SPC2_ERR_INVALID, "spc2_noise_noalloc stages at most 2097152 normals (N <= 2097152 for SE, "
+                                  "1048576 for SRK); this call needs " + std::to_string(need));
+  double *buf = s.device >= 0 && s.device < kScratchDevices ? g_scratch[s.device] : nullptr;
+  if (!buf) return fail(SPC2_ERR_INVALID, "no staging array for the handle's device");
+  // the kernel writes every one of the `need` normals: nothing is cleared
+  launch_noise(s, npp, (uint64_t)step, buf);
+  if (spc2_status st = check_launch("noise")) return st;
+  HIP_TRY(hipMemcpyAsync(out, buf, sizeof(double) * need, hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
   return SPC2_OK;
 }
 
