@@ -6,10 +6,14 @@
    / RK4 steps and the mt19937 + std::normal_distribution noise they consume.
 2. When the reference build is present (oracle/_ref), the same comparison on
    fresh seeds.
-3. Closed forms, invariants and brute force for what the reference build does
-   not cover (vortex arrays live only in the reference's MPI translation units):
-   W(r) values, two-body sums, the cohesion atan2 identity, Philox KATs, the
-   normal transform, the order parameter.
+   The MPI shared-memory classes (VortexArraysSystemPtr, ChimeraSystemPtr and
+   the *StepperPtr steppers, oracle/ref_harness_mpi.cpp) the same way: the
+   vortex-arrays RHS in both forms on the linked-cell and all-pairs paths, and
+   the SE / SRK / RK4 Ptr steps of both systems, with their outputs on 2 and 4
+   ranks equal to 1 rank's.
+3. Closed forms, invariants and brute force on top: W(r) values, two-body sums,
+   the cohesion atan2 identity, Philox KATs, the normal transform, the order
+   parameter.
 """
 import hashlib
 import math
@@ -96,7 +100,59 @@ def run_golden_case(name, case, args):
         out = {"state": st, "state1": first}
         out["noise"] = np.concatenate(noise) if noise else np.zeros(0)
         return out
+    if case in ("vrhs_k", "crhs_k"):
+        seed, mode, kc, dt = int(a[0]), int(a[1]), float(a[2]), float(a[3])
+        s = vort_args(a[4:]) if case[0] == "v" else chim_args(a[4:])
+        st = O.make_state(seed, mode, N, L, s.rho)
+        kp = O.uniform(seed + 101, 3 * N, -1.0, 1.0)
+        kn = O.uniform(seed + 202, 3 * N, 5.0, 6.0)
+        out, _ = O.rhs_k(s, st, kp, kc, dt, kn)
+        return {"k_next": out}
+    if case in ("vrhs_ito", "crhs_ito"):
+        seed, mode, init = int(a[0]), int(a[1]), int(a[2])
+        s = vort_args(a[4:]) if case[0] == "v" else chim_args(a[4:])
+        st = O.make_state(seed, mode, N, L, s.rho)
+        d0 = O.uniform(seed + 303, 3 * N, -1.0, 1.0) if init else None
+        a0 = O.uniform(seed + 404, 2 * N, -1.0, 1.0) if init else None
+        d, ad, _ = O.rhs_ito(s, st, d0, a0)
+        return {"derivative": d, "additional": ad}
+    if case in ("vse", "vsrk", "vrk4", "cse", "csrk", "crk4"):
+        # the *StepperPtr steppers (StochasticEulerStepperPtr.hpp:32-76,
+        # StochasticRungeKuttaStepperPtr.hpp:25-97, RungeKutta4StepperPtr.cpp:25-65)
+        seed, mtseed, steps, dt = int(a[0]), int(a[1]), int(a[2]), float(a[3])
+        s = vort_args(a[4:]) if case[0] == "v" else chim_args(a[4:])
+        mt = O.MT19937Normals(mtseed)
+        omega = None
+        if case[1:] == "srk":  # the SRK stepper draws N(0, 0) frequencies at construction
+            drawn = mt.normals(N) * 0.0 + 0.0
+            if case == "csrk":  # only the chimera step adds them (:65 against :94)
+                omega = drawn
+        st = O.make_state(seed, 0, N, L, s.rho)
+        kbuf = np.zeros(12 * N)
+        noise, first = [], None
+        for k in range(steps):
+            if case[1:] == "se":
+                xi = mt.normals(N)
+                noise.append(xi)
+                st = O.step_se(s, st, dt, xi, variant=0)
+            elif case[1:] == "srk":
+                xi = mt.normals(2 * N)
+                noise.append(xi)
+                st = O.step_srk(s, st, dt, xi, omega=omega)
+            else:
+                st = O.step_rk4(s, st, dt, kbuf)
+            if k == 0:
+                first = st.copy()
+        out = {"state": st, "state1": first}
+        out["noise"] = np.concatenate(noise) if noise else np.zeros(0)
+        return out
     raise KeyError(case)
+
+
+def vort_args(args):
+    v, mp, mm, xa, xr, D, kappa, rho = (float(x) for x in args)
+    return O.vortex(N, L, rho=rho, velocity=v, mu_plus=mp, mu_minus=mm, xi_a=xa, xi_r=xr, kappa=kappa, D=D,
+                    threads=8)
 
 
 def golden_case_names():
@@ -117,7 +173,21 @@ def test_oracle_matches_reference_golden(golden, name):
         assert sha(arr) == ref["sha256"], (name, out)
 
 
+def test_reference_outputs_do_not_depend_on_the_rank_count(golden):
+    """The reference's deterministic MPI cases gave the same bytes on 1, 2 and
+    4 ranks (ThreadSharedMemory's contiguous slices, ThreadSharedMemory.cpp:12-22),
+    so the oracle's single-process restatement stands for every rank count."""
+    multi = [(n, c) for n, c in golden["cases"].items() if "ranks" in c]
+    assert len(multi) >= 5
+    for name, c in multi:
+        for ranks, outs in c["ranks"].items():
+            for out, h in outs.items():
+                assert h == c["outputs"][out]["sha256"], (name, ranks, out)
+
+
 HARNESS = O.REF_HARNESS
+HARNESS_MPI = os.path.join(os.path.dirname(O.REF_HARNESS), "ref_harness_mpi")
+MPIEXEC = os.environ.get("MPIEXEC", "/opt/conda/bin/mpiexec")
 
 
 @pytest.mark.skipif(not os.path.exists(HARNESS), reason="reference build (oracle/_ref) not present")
@@ -135,6 +205,28 @@ def test_oracle_matches_live_reference(case, args, outputs):
         for out in outputs:
             ref = np.fromfile(os.path.join(tmp, out + ".bin"), got[out].dtype)
             assert np.array_equal(got[out], ref), (case, out)
+
+
+@pytest.mark.skipif(not (os.path.exists(HARNESS_MPI) and os.path.exists(MPIEXEC)),
+                    reason="reference MPI build (oracle/_ref) or mpiexec not present")
+@pytest.mark.parametrize("ranks,case,args,outputs", [
+    (1, "vrhs_k", ["91", "1", "0.5", "0.02", "0.5", "90.0", "110.0", "0.5", "0.3", "0.02", "15.0", "2.5"],
+     ("k_next",)),
+    (2, "vrhs_ito", ["92", "0", "1", "0.01", "0.8", "100.0", "100.0", "0.4472", "0.2236", "0.05", "20.0", "2.236"],
+     ("derivative", "additional")),
+    (1, "vsrk", ["93", "6", "2", "0.02", "1.0", "100.0", "100.0", "0.4472", "0.2236", "0.05", "20.0", "2.236"],
+     ("noise", "state")),
+    (4, "vrk4", ["94", "0", "2", "0.01", "1.0", "100.0", "100.0", "0.4472", "0.2236", "0.0", "20.0", "2.236"],
+     ("state",)),
+    (1, "csrk", ["95", "7", "2", "0.01", "0.3", "1.0", "2.0", "1.45", "0.05"], ("noise", "state")),
+])
+def test_oracle_matches_live_mpi_reference(ranks, case, args, outputs):
+    with tempfile.TemporaryDirectory() as tmp:
+        subprocess.run([MPIEXEC, "-n", str(ranks), HARNESS_MPI, case, tmp] + args, check=True)
+        got = run_golden_case("live", case, args)
+        for out in outputs:
+            ref = np.fromfile(os.path.join(tmp, out + ".bin"), got[out].dtype)
+            assert np.array_equal(got[out], ref), (case, ranks, out)
 
 
 # ---- closed forms and invariants ----------------------------------------------
