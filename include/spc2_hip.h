@@ -158,6 +158,28 @@ spc2_status spc2_host_free(void *p);
  * (StochasticRungeKuttaStepperPtr.hpp:49-56). */
 spc2_status spc2_noise(spc2_sim_t h, int32_t method, int64_t step, double *out);
 
+/* Each particle's partner counts at the current state, by particle id: within
+ * rho (chimera; vortex arrays: n0 + n1, within rho) and within xi_r (vortex
+ * cohesion; 0 for the chimera), the neighborhood_cardinality_ of
+ * VortexArraysSystemPtr.cpp:459-489 / the n of ChimeraSystemPtr.cpp:429-433.
+ * The FP32-compensated form decides pairs near a cutoff in FP64, so the counts
+ * are those of the reference method in every precision. */
+spc2_status spc2_neighbour_counts(spc2_sim_t h, int32_t *n_rho, int32_t *n_xi_r);
+
+/* Exact checkpoint of a handle (no counterpart in the reference, whose restart
+ * reads a float frame: InitializeSystemStateFromPreviousSolution,
+ * SimulationEnginePtr.cpp:52-73).  The blob holds the handle's parameters
+ * (seed included), the FP64 state, the natural frequencies, the phase
+ * components of the RK4 stepper's k_1..k_4 and, for Verlet handles, the
+ * current list's build positions and displacement bound.  Loaded into a handle
+ * created with the same parameters, the steps that follow (with the same step
+ * indices, which key the Philox noise) are bit-identical to those of the
+ * handle that saved it.  Loading fails with SPC2_ERR_INVALID for a blob of
+ * other parameters. */
+spc2_status spc2_checkpoint_size(spc2_sim_t h, uint64_t *bytes);
+spc2_status spc2_checkpoint_save(spc2_sim_t h, void *buf, uint64_t bytes);
+spc2_status spc2_checkpoint_load(spc2_sim_t h, const void *buf, uint64_t bytes);
+
 /* spc2_noise without any device allocation, for callers that cannot allocate
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  The normals are staged in a

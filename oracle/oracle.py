@@ -60,6 +60,7 @@ def lib():
         L.orc_rhs_k.argtypes = [sp, dp, dp, C.c_double, C.c_double, dp, C.c_void_p]
         L.orc_rhs_k_scaled.argtypes = [sp, dp, dp, C.c_double, C.c_double, dp, C.c_void_p, C.c_void_p]
         L.orc_rhs_ito.argtypes = [sp, dp, dp, dp, C.c_void_p]
+        L.orc_neighbour_counts.argtypes = [sp, dp, ip, ip]
         L.orc_step_se.argtypes = [sp, dp, C.c_double, C.c_void_p, dp, C.c_int32]
         L.orc_step_srk.argtypes = [sp, dp, C.c_double, C.c_void_p, dp]
         L.orc_step_rk4.argtypes = [sp, dp, C.c_double, dp]
@@ -72,7 +73,7 @@ def lib():
         L.orc_mt_seed.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_mt_uniform.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_double, dp]
         L.orc_mt_normals.argtypes = [C.c_void_p, C.c_int64, dp]
-        for f in ("orc_cell_list", "orc_rhs_k", "orc_rhs_k_scaled", "orc_rhs_ito", "orc_step_se", "orc_step_srk", "orc_step_rk4"):
+        for f in ("orc_cell_list", "orc_rhs_k", "orc_rhs_k_scaled", "orc_rhs_ito", "orc_neighbour_counts", "orc_step_se", "orc_step_srk", "orc_step_rk4"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -150,6 +151,15 @@ def rhs_k(sysp, state, k_prev, k_coef, dt, k_next, term_scale=False):
                                 counts.ctypes.data_as(C.c_void_p), _ptr(ts))
     assert rc == 0
     return (k_next, counts, ts) if term_scale else (k_next, counts)
+
+
+def neighbour_counts(sysp, state):
+    """Per-particle partner counts at the state: (within rho, within xi_r)."""
+    a = np.empty(sysp.n, np.int32)
+    b = np.empty(sysp.n, np.int32)
+    rc = lib().orc_neighbour_counts(C.byref(sysp), np.ascontiguousarray(state, np.float64), a, b)
+    assert rc == 0
+    return a, b
 
 
 def rhs_ito(sysp, state, derivative=None, additional=None):

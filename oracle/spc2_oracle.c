@@ -546,6 +546,55 @@ int orc_rhs_k_scaled(const orc_sys *s, const double *state, const double *k_prev
   return 0;
 }
 
+/* Partner counts of pass 1 at the state (k-form with k_coef = 0): within rho
+ * (chimera n; vortex n0 + n1) and within xi_r (vortex n2, else 0), by particle. */
+int orc_neighbour_counts(const orc_sys *s, const double *state, int32_t *n_rho, int32_t *n_xi_r) {
+  int64_t n = s->n;
+  double *k = (double *)calloc((size_t)(KS * n), sizeof(double));
+  if (!k) return -1;
+  int64_t cnt[3] = {0, 0, 0};
+  rhs_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.s = s;
+  c.g = make_geom(s->Lx, s->Ly);
+  c.use_ll = use_linked_list(s);
+  c.out = k;
+  c.cnt = cnt;
+  cell_bufs cb;
+  memset(&cb, 0, sizeof(cb));
+  if (c.use_ll) {
+    c.pos = state;
+    if (build_cells(s, &c, &cb)) { free(k); return -1; }
+  } else {
+    c.base = state;
+    c.k_prev = k;
+    c.k_coef = 0.0;
+    c.dt = 0.0;
+  }
+  if (s->system == ORC_CHIMERA) {
+    c.sumA = (double *)malloc(sizeof(double) * (size_t)n);
+    c.card = (double *)malloc(sizeof(double) * (size_t)n);
+    parallel_for(s->threads, n, chimera_pass1, &c);
+    for (int64_t i = 0; i < n; ++i) {
+      n_rho[i] = (int32_t)c.card[i];
+      n_xi_r[i] = 0;
+    }
+    free(c.sumA);
+    free(c.card);
+  } else {
+    c.card3 = (double *)malloc(sizeof(double) * (size_t)(3 * n));
+    parallel_for(s->threads, n, vortex_pass1, &c);
+    for (int64_t i = 0; i < n; ++i) {
+      n_rho[i] = (int32_t)(c.card3[3 * i] + c.card3[3 * i + 1]);
+      n_xi_r[i] = (int32_t)c.card3[3 * i + 2];
+    }
+    free(c.card3);
+  }
+  free_cells(&cb);
+  free(k);
+  return 0;
+}
+
 /* EvaluateRhs Ito form: ChimeraSystem.cpp:102-114 + :429-645 / :171-255;
  * VortexArraysSystemPtr.cpp:122-134 + :509-735 / :219-355.
  * derivative and additional are in/out, as in the reference. */

@@ -56,6 +56,8 @@ int orc_rhs_k(const orc_sys *s, const double *state, const double *k_prev, doubl
  * particle without partners. */
 int orc_rhs_k_scaled(const orc_sys *s, const double *state, const double *k_prev, double k_coef, double dt,
                      double *k_next, int64_t *counts, double *tscale);
+/* per-particle partner counts of pass 1 at the state: within rho, within xi_r */
+int orc_neighbour_counts(const orc_sys *s, const double *state, int32_t *n_rho, int32_t *n_xi_r);
 int orc_rhs_ito(const orc_sys *s, const double *state, double *derivative, double *additional, int64_t *counts);
 
 /* ---- steppers (state updated in place, engine wrap applied after) ------ */

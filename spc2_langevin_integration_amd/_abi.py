@@ -73,6 +73,10 @@ def load():
         "spc2_host_free": (st, [C.c_void_p]),
         "spc2_noise": (st, [h, C.c_int32, C.c_int64, dp]),
         "spc2_noise_noalloc": (st, [h, C.c_int32, C.c_int64, dp]),
+        "spc2_neighbour_counts": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "spc2_checkpoint_size": (st, [h, C.POINTER(C.c_uint64)]),
+        "spc2_checkpoint_save": (st, [h, C.c_void_p, C.c_uint64]),
+        "spc2_checkpoint_load": (st, [h, C.c_void_p, C.c_uint64]),
         "spc2_order_parameter": (st, [h, dp, dp]),
         "spc2_get_cell_list": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "spc2_num_cells": (st, [h, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -230,6 +234,26 @@ class Sim:
         out = np.empty(self.n * (2 if m == SRK else 1), np.float64)
         _check(load().spc2_noise(self.h, m, step, _dp(out)))
         return out
+
+    def neighbour_counts(self):
+        """Per-particle partner counts at the current state: (within rho, within xi_r)."""
+        a = np.empty(self.n, np.int32)
+        b = np.empty(self.n, np.int32)
+        _check(load().spc2_neighbour_counts(self.h, _ip(a), _ip(b)))
+        return a, b
+
+    def checkpoint(self):
+        """Exact checkpoint of the handle (bytes)."""
+        size = C.c_uint64()
+        _check(load().spc2_checkpoint_size(self.h, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        _check(load().spc2_checkpoint_save(self.h, buf, size.value))
+        return buf.raw
+
+    def restore(self, blob):
+        """Loads a checkpoint written by a handle with the same parameters."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        _check(load().spc2_checkpoint_load(self.h, buf, len(blob)))
 
     def noise_noalloc(self, method, step):
         """noise() through the entry point that allocates no device memory."""

@@ -851,6 +851,167 @@ spc2_status spc2_noise(spc2_sim_t h, int32_t method, int64_t step, double *out) 
   return SPC2_OK;
 }
 
+// ---- exact checkpoints ---------------------------------------------------------
+// Layout: CkptHeader, then by particle id the state (AoS x, y, phi), omega, the
+// phase components of the RK4 stepper's k_1..k_4 (the only k components a step
+// reads before writing them: a particle without partners keeps its phase
+// derivative from the previous step) and, for Verlet handles, the build
+// positions x, y of the current list.
+namespace {
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t has_list;  // the Verlet list's build positions and displacement bound are valid
+  spc2_params params;
+  double drift;
+  int64_t n;
+};
+constexpr char kCkptMagic[8] = {'S', 'P', 'C', '2', 'C', 'K', 'P', 'T'};
+constexpr uint32_t kCkptVersion = 1;
+
+uint64_t ckpt_bytes(const Sim &s) {
+  return sizeof(CkptHeader) + sizeof(double) * (uint64_t)s.n * (uint64_t)(3 + 1 + 4 + (s.verlet ? 2 : 0));
+}
+
+bool same_params(const spc2_params &a, const spc2_params &b) {
+  return a.system == b.system && a.precision == b.precision && a.neighbour == b.neighbour && a.n == b.n &&
+         a.Lx == b.Lx && a.Ly == b.Ly && a.rho == b.rho && a.v0 == b.v0 && a.sigma == b.sigma && a.alpha == b.alpha &&
+         a.D == b.D && a.mu_plus == b.mu_plus && a.mu_minus == b.mu_minus && a.xi_a == b.xi_a && a.xi_r == b.xi_r &&
+         a.kappa == b.kappa && a.seed == b.seed && a.verlet_skin == b.verlet_skin;
+}
+}  // namespace
+
+spc2_status spc2_checkpoint_size(spc2_sim_t h, uint64_t *bytes) {
+  if (check_handle(h) || !bytes) return fail(SPC2_ERR_INVALID, "null argument");
+  *bytes = ckpt_bytes(h->s);
+  return SPC2_OK;
+}
+
+spc2_status spc2_checkpoint_save(spc2_sim_t h, void *buf, uint64_t bytes) {
+  if (check_handle(h) || !buf) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  if (bytes < ckpt_bytes(s)) return fail(SPC2_ERR_INVALID, "checkpoint buffer smaller than spc2_checkpoint_size");
+  CkptHeader hd;
+  std::memset(&hd, 0, sizeof(hd));
+  std::memcpy(hd.magic, kCkptMagic, sizeof(kCkptMagic));
+  hd.version = kCkptVersion;
+  hd.has_list = s.verlet && s.vbuilt ? 1u : 0u;
+  hd.params = s.P;
+  hd.drift = s.drift;
+  hd.n = s.n;
+  char *out = static_cast<char *>(buf);
+  std::memcpy(out, &hd, sizeof(hd));
+  out += sizeof(hd);
+  const size_t n = (size_t)s.n;
+  std::vector<double> tmp(3 * n);
+  auto put = [&](size_t count) {
+    std::memcpy(out, tmp.data(), sizeof(double) * count);
+    out += sizeof(double) * count;
+  };
+  double *st[3] = {s.sx, s.sy, s.sp};
+  if (spc2_status e = download_aos(s, tmp.data(), 3, st, 3)) return e;
+  put(3 * n);
+  if (spc2_status e = download_aos(s, tmp.data(), 1, &s.omega, 1)) return e;
+  put(n);
+  for (int j = 0; j < 4; ++j) {
+    if (spc2_status e = download_aos(s, tmp.data(), 1, &s.k[j][2], 1)) return e;
+    put(n);
+  }
+  if (s.verlet) {
+    if (hd.has_list) {
+      double *b[2] = {s.xb, s.yb};
+      if (spc2_status e = download_aos(s, tmp.data(), 2, b, 2)) return e;
+    } else {
+      std::fill(tmp.begin(), tmp.begin() + 2 * n, 0.0);
+    }
+    put(2 * n);
+  }
+  return SPC2_OK;
+}
+
+spc2_status spc2_checkpoint_load(spc2_sim_t h, const void *buf, uint64_t bytes) {
+  if (check_handle(h) || !buf) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  CkptHeader hd;
+  if (bytes < sizeof(hd)) return fail(SPC2_ERR_INVALID, "checkpoint too short");
+  std::memcpy(&hd, buf, sizeof(hd));
+  if (std::memcmp(hd.magic, kCkptMagic, sizeof(kCkptMagic)) != 0 || hd.version != kCkptVersion)
+    return fail(SPC2_ERR_INVALID, "not an spc2 checkpoint (or another version)");
+  if (hd.n != s.n || bytes < ckpt_bytes(s)) return fail(SPC2_ERR_INVALID, "checkpoint of another particle count");
+  if (!same_params(hd.params, s.P))
+    return fail(SPC2_ERR_INVALID, "checkpoint of a handle with other parameters (system, precision, neighbour "
+                                  "mode, box, model parameters, seed or skin)");
+  if (hd.has_list && !s.verlet) return fail(SPC2_ERR_INVALID, "checkpoint holds a Verlet list for a cell handle");
+  const size_t n = (size_t)s.n;
+  const char *in = static_cast<const char *>(buf) + sizeof(hd);
+  auto take = [&](std::vector<double> &v, size_t count) {
+    v.resize(count);
+    std::memcpy(v.data(), in, sizeof(double) * count);
+    in += sizeof(double) * count;
+  };
+  std::vector<double> state, omega, kphi[4], xyb;
+  take(state, 3 * n);
+  take(omega, n);
+  for (auto &k : kphi) take(k, n);
+  if (s.verlet) take(xyb, 2 * n);
+  double *st[3] = {s.sx, s.sy, s.sp};
+  if (hd.has_list) {
+    // the list of the saving handle, rebuilt from its own build positions: the
+    // same cells, homes, entries and summation order
+    std::vector<double> pos(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+      pos[3 * i] = xyb[2 * i];
+      pos[3 * i + 1] = xyb[2 * i + 1];
+      pos[3 * i + 2] = state[3 * i + 2];
+    }
+    if (spc2_status e = upload_aos(s, pos.data(), 3, st, 3)) return e;
+    if (spc2_status e = verlet_build(s, nullptr, 0.0, true)) return e;
+  } else {
+    if (spc2_status e = upload_aos(s, state.data(), 3, st, 3)) return e;
+    if (spc2_status e = rehome_now(s)) return e;
+  }
+  // everything by particle id into the homes the build (or re-homing) chose;
+  // re-homing swaps the arrays with their spares, so the pointers are read again
+  double *st2[3] = {s.sx, s.sy, s.sp};
+  if (spc2_status e = upload_aos(s, state.data(), 3, st2, 3)) return e;
+  if (spc2_status e = upload_aos(s, omega.data(), 1, &s.omega, 1)) return e;
+  for (int j = 0; j < 4; ++j)
+    if (spc2_status e = upload_aos(s, kphi[j].data(), 1, &s.k[j][2], 1)) return e;
+  if (hd.has_list) s.drift = hd.drift;
+  s.tile_fresh = false;
+  s.built = false;
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  return SPC2_OK;
+}
+
+spc2_status spc2_neighbour_counts(spc2_sim_t h, int32_t *n_rho, int32_t *n_xi_r) {
+  if (check_handle(h) || !n_rho || !n_xi_r) return fail(SPC2_ERR_INVALID, "null argument");
+  Sim &s = h->s;
+  // pass 1 of the Ito form at the current state records each slot's counts
+  // (its derivative output is not written)
+  if (s.verlet)
+    if (spc2_status st = verlet_ensure(s, 0.0, false)) return st;
+  build(s, nullptr, 0.0);
+  StepArgs sa = step_args(s, 0.0, 0);
+  sa.zero_in = 1;
+  rhs_pass1(s, EPI_ITO1, nullptr, nullptr, nullptr, sa, nullptr, nullptr, nullptr);
+  // slots are homes on the Verlet and all-pairs paths, cell-sorted slots on the
+  // linked-cell path
+  const int *id_of_slot = (s.verlet || !s.linked) ? s.hid : s.slot_id;
+  launch_slot_to_id(s, id_of_slot, s.n_slot, s.io[0]);
+  launch_slot_to_id(s, id_of_slot, s.n2_slot, s.io[1]);
+  if (spc2_status st = check_launch("neighbour counts")) return st;
+  std::vector<double> a((size_t)s.n), b((size_t)s.n);
+  HIP_TRY(hipMemcpyAsync(a.data(), s.io[0], sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipMemcpyAsync(b.data(), s.io[1], sizeof(double) * (size_t)s.n, hipMemcpyDeviceToHost, s.stream));
+  HIP_TRY(hipStreamSynchronize(s.stream));
+  for (int i = 0; i < s.n; ++i) {
+    n_rho[i] = (int32_t)a[(size_t)i];
+    n_xi_r[i] = (int32_t)b[(size_t)i];
+  }
+  return SPC2_OK;
+}
+
 spc2_status spc2_noise_noalloc(spc2_sim_t h, int32_t method, int64_t step, double *out) {
   if (check_handle(h) || !out) return fail(SPC2_ERR_INVALID, "null argument");
   if (step < 0) return fail(SPC2_ERR_INVALID, "step must be >= 0");

@@ -245,6 +245,7 @@ void launch_drift(Sim &s, double *out);
 void launch_to_home(Sim &s, const double *src_id, double *dst_home);
 void launch_from_home(Sim &s, const double *src_home, double *dst_id);
 void launch_from_home_int(Sim &s, const int *src_home, int *dst_id);
+void launch_slot_to_id(Sim &s, const int *id_of_slot, const double *src_slot, double *dst_id);
 void launch_identity_homes(Sim &s);
 void launch_snapshot(Sim &s, float *out);
 // Re-homes the particle arrays to the last build's cell order; k_xy_live =

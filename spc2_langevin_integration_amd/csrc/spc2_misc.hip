@@ -21,12 +21,14 @@ inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / 
 // Kuramoto order parameter, two-level FP64 reduction (deterministic: fixed
 // partition and fixed tree); replaces the float sum of
 // BinaryObserverPtr::SaveSummaryStatistics (BinaryObserverPtr.cpp:237-247).
-__global__ __launch_bounds__(kBlock) void k_order_partial(int n, const double *sp, double *red) {
+// Partial sums over particle ids (the phase read through the id-to-home map),
+// so that the result does not depend on where re-homing has put the particles.
+__global__ __launch_bounds__(kBlock) void k_order_partial(int n, const double *sp, const int *hpos, double *red) {
   __shared__ double sc[kBlock], ss[kBlock];
   double c = 0.0, s = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double sn, cs;
-    sincos(sp[i], &sn, &cs);
+    sincos(sp[hpos[i]], &sn, &cs);
     c += cs;
     s += sn;
   }
@@ -99,6 +101,13 @@ __global__ __launch_bounds__(kBlock) void k_from_home(int n, const int *hpos, co
   if (id < n) dst_id[id] = src_home[hpos[id]];
 }
 
+// Per-slot values to particle-id order through a slot -> id map.
+__global__ __launch_bounds__(kBlock) void k_slot_to_id(int n, const int *id_of_slot, const double *src_slot,
+                                                       double *dst_id) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst_id[id_of_slot[k]] = src_slot[k];
+}
+
 __global__ __launch_bounds__(kBlock) void k_from_home_int(int n, const int *hpos, const int *src_home, int *dst_id) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id < n) dst_id[id] = src_home[hpos[id]];
@@ -169,11 +178,6 @@ __global__ __launch_bounds__(kBlock) void k_fill(double *p, double v, size_t n) 
 
 }  // namespace
 
-void launch_order_param(Sim &s, double *out2) {
-  int nb = std::min(kRedBlocks, std::max(1, blocks_for(s.n)));
-  hipLaunchKernelGGL(k_order_partial, dim3(nb), dim3(kBlock), 0, s.stream, s.n, s.sp, s.red);
-  hipLaunchKernelGGL(k_order_final, dim3(1), dim3(kBlock), 0, s.stream, nb, s.red, out2);
-}
 
 void launch_init_state(Sim &s, double omega_std) {
   hipLaunchKernelGGL(k_init_state, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.P.Lx, s.P.Ly, s.P.seed,
@@ -188,6 +192,14 @@ void ensure_hpos(Sim &s) {
 }
 }  // namespace
 
+void launch_order_param(Sim &s, double *out2) {
+  ensure_hpos(s);
+  int nb = std::min(kRedBlocks, std::max(1, blocks_for(s.n)));
+  hipLaunchKernelGGL(k_order_partial, dim3(nb), dim3(kBlock), 0, s.stream, s.n, (const double *)s.sp,
+                     (const int *)s.hpos, s.red);
+  hipLaunchKernelGGL(k_order_final, dim3(1), dim3(kBlock), 0, s.stream, nb, (const double *)s.red, out2);
+}
+
 void launch_to_home(Sim &s, const double *src_id, double *dst_home) {
   ensure_hpos(s);
   hipLaunchKernelGGL(k_to_home, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_id, dst_home);
@@ -196,6 +208,10 @@ void launch_to_home(Sim &s, const double *src_id, double *dst_home) {
 void launch_from_home(Sim &s, const double *src_home, double *dst_id) {
   ensure_hpos(s);
   hipLaunchKernelGGL(k_from_home, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, s.hpos, src_home, dst_id);
+}
+
+void launch_slot_to_id(Sim &s, const int *id_of_slot, const double *src_slot, double *dst_id) {
+  hipLaunchKernelGGL(k_slot_to_id, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, id_of_slot, src_slot, dst_id);
 }
 
 void launch_from_home_int(Sim &s, const int *src_home, int *dst_id) {

@@ -69,9 +69,11 @@ void DeviceSystem::SetNaturalFrequencies(const std::vector<Real> &omega) {
 namespace {
 
 spc2_params ChimeraParams(Real v_0, Real sigma, Real rho, Real alpha, Real D_phi, const PeriodicBoundaryConditions &pbc,
-                          long n, Precision precision, uint64_t seed) {
+                          long n, Precision precision, uint64_t seed, NeighbourMode neighbour, Real verlet_skin) {
   spc2_params p;
   spc2_default_params(&p);
+  p.neighbour = (int32_t)neighbour;
+  p.verlet_skin = verlet_skin;
   p.system = SPC2_CHIMERA;
   p.precision = (int32_t)precision;
   p.n = n;
@@ -87,9 +89,12 @@ spc2_params ChimeraParams(Real v_0, Real sigma, Real rho, Real alpha, Real D_phi
 }
 
 spc2_params VortexParams(Real velocity, Real mu_plus, Real mu_minus, Real xi_a, Real xi_r, Real D_phi, Real kappa,
-                         Real rho, const PeriodicBoundaryConditions &pbc, long n, Precision precision, uint64_t seed) {
+                         Real rho, const PeriodicBoundaryConditions &pbc, long n, Precision precision, uint64_t seed,
+                         NeighbourMode neighbour, Real verlet_skin) {
   spc2_params p;
   spc2_default_params(&p);
+  p.neighbour = (int32_t)neighbour;
+  p.verlet_skin = verlet_skin;
   p.system = SPC2_VORTEX;
   p.precision = (int32_t)precision;
   p.n = n;
@@ -141,8 +146,10 @@ void EvalIto(DeviceSystem &sys, const Real *state, std::vector<Real> &derivative
 }  // namespace
 
 ChimeraSystem::ChimeraSystem(Real v_0, Real sigma, Real rho, Real alpha, Real D_phi, Real rho_0,
-                             PeriodicBoundaryConditions &pbc_config, long n, Precision precision, uint64_t seed)
-    : DeviceSystem(ChimeraParams(v_0, sigma, rho, alpha, D_phi, pbc_config, n, precision, seed)) {
+                             PeriodicBoundaryConditions &pbc_config, long n, Precision precision, uint64_t seed,
+                             NeighbourMode neighbour, Real verlet_skin)
+    : DeviceSystem(
+          ChimeraParams(v_0, sigma, rho, alpha, D_phi, pbc_config, n, precision, seed, neighbour, verlet_skin)) {
   (void)rho_0;  // density label of the reference's output file names only
 }
 
@@ -160,9 +167,9 @@ void ChimeraSystem::EvaluateRhs(const Real *const system_state, std::vector<Real
 
 VortexArraysSystem::VortexArraysSystem(Real velocity, Real mu_plus, Real mu_minus, Real xi_a, Real xi_r, Real D_phi,
                                        Real kappa, Real rho, PeriodicBoundaryConditions &pbc_config, long n,
-                                       Precision precision, uint64_t seed)
-    : DeviceSystem(
-          VortexParams(velocity, mu_plus, mu_minus, xi_a, xi_r, D_phi, kappa, rho, pbc_config, n, precision, seed)) {}
+                                       Precision precision, uint64_t seed, NeighbourMode neighbour, Real verlet_skin)
+    : DeviceSystem(VortexParams(velocity, mu_plus, mu_minus, xi_a, xi_r, D_phi, kappa, rho, pbc_config, n, precision,
+                                seed, neighbour, verlet_skin)) {}
 
 void VortexArraysSystem::EvaluateRhs(const Real *const system_state, const std::vector<Real> &k_prev,
                                      std::vector<Real> &k_next, Real k_coef, Real dt) {
@@ -491,8 +498,14 @@ void EngineConfig::Set(const std::string &key, const std::string &value) {
   else if (key == "state_every") state_every = (int)l();
   else if (key == "summary_every") summary_every = (int)l();
   else if (key == "float_summary") float_summary = l() != 0;
+  else if (key == "neighbour") neighbour = value;
+  else if (key == "verlet_skin") verlet_skin = d();
   else if (key == "init_file") init_file = value;
   else if (key == "init_frame") init_frame = l();
+  else if (key == "init_state_file") init_state_file = value;
+  else if (key == "resume") resume = value;
+  else if (key == "checkpoint") checkpoint = value;
+  else if (key == "checkpoint_every") checkpoint_every = l();
   else throw Spc2Error("unknown configuration key '" + key + "'");
 }
 
@@ -503,6 +516,14 @@ Precision ParsePrecision(const std::string &s) {
   if (s == "fp32k") return Precision::kFP32Kahan;
   if (s == "fp64f") return Precision::kFP64Factored;
   throw Spc2Error("precision must be fp64, fp32k or fp64f");
+}
+
+NeighbourMode ParseNeighbour(const std::string &s) {
+  if (s == "auto") return NeighbourMode::kAuto;
+  if (s == "cells") return NeighbourMode::kCells;
+  if (s == "all_pairs") return NeighbourMode::kAllPairs;
+  if (s == "verlet") return NeighbourMode::kVerlet;
+  throw Spc2Error("neighbour must be auto, cells, all_pairs or verlet");
 }
 
 int32_t ParseMethod(const std::string &s) {
@@ -520,14 +541,18 @@ SimulationEngine::SimulationEngine(const EngineConfig &cfg) : cfg_(cfg), pbc_(cf
   if (cfg_.init != "philox" && cfg_.init != "mt19937") throw Spc2Error("init must be philox or mt19937");
   if (cfg_.noise != "philox" && cfg_.noise != "mt19937") throw Spc2Error("noise must be philox or mt19937");
   if (cfg_.loop != "device" && cfg_.loop != "host") throw Spc2Error("loop must be device or host");
+  if (cfg_.checkpoint_every < 0) throw Spc2Error("checkpoint_every must be >= 0");
+  if (cfg_.loop == "host" && (!cfg_.resume.empty() || !cfg_.checkpoint.empty()))
+    throw Spc2Error("checkpoints need loop=device");
   ParseMethod(cfg_.method);
   const Precision prec = ParsePrecision(cfg_.precision);
+  const NeighbourMode nb = ParseNeighbour(cfg_.neighbour);
   if (cfg_.system == "chimera") {
     system_ = new ChimeraSystem(cfg_.v0, cfg_.sigma, cfg_.rho, cfg_.alpha, cfg_.D, cfg_.rho0, pbc_, cfg_.n, prec,
-                                cfg_.seed);
+                                cfg_.seed, nb, cfg_.verlet_skin);
   } else if (cfg_.system == "vortex") {
     system_ = new VortexArraysSystem(cfg_.v0, cfg_.mu_plus, cfg_.mu_minus, cfg_.xi_a, cfg_.xi_r, cfg_.D, cfg_.kappa,
-                                     cfg_.rho, pbc_, cfg_.n, prec, cfg_.seed);
+                                     cfg_.rho, pbc_, cfg_.n, prec, cfg_.seed, nb, cfg_.verlet_skin);
   } else {
     throw Spc2Error("system must be chimera or vortex");
   }
@@ -579,17 +604,91 @@ void SimulationEngine::InitializeSystemStateFromPreviousSolution(const std::stri
   system_->Upload(st.data());
 }
 
+// SimulationEngine.cpp:52-73: 3N floats from the start of the file (the time
+// stamp of a frame is not there), widened to the state.
+void SimulationEngine::InitializeSystemStateFromFile(const std::string &file_name) {
+  std::ifstream in(file_name, std::ios::binary | std::ios::in);
+  if (!in.is_open()) throw Spc2Error("cannot open " + file_name);
+  const size_t m = (size_t)kS * (size_t)cfg_.n;
+  std::vector<float> f(m);
+  in.read(reinterpret_cast<char *>(f.data()), (std::streamsize)(m * sizeof(float)));
+  if (!in) throw Spc2Error(file_name + " holds fewer than 3N floats");
+  std::vector<Real> st(f.begin(), f.end());
+  system_->Upload(st.data());
+}
+
+namespace {
+constexpr char kEngineCkptMagic[8] = {'S', 'P', 'C', '2', 'E', 'N', 'G', '1'};
+}
+
+// Engine checkpoint: magic, step index, t, the library blob, the mt19937
+// stream as text (its standard serialisation).
+void SimulationEngine::SaveCheckpoint(const std::string &file_name, Real t) const {
+  uint64_t bytes = 0;
+  Spc2Check(spc2_checkpoint_size(system_->handle(), &bytes), "spc2_checkpoint_size");
+  std::vector<char> blob((size_t)bytes);
+  Spc2Check(spc2_checkpoint_save(system_->handle(), blob.data(), bytes), "spc2_checkpoint_save");
+  std::ostringstream mt;
+  mt << mt_;
+  const std::string mts = mt.str();
+  const std::string tmp = file_name + ".tmp";
+  {
+    std::ofstream out(tmp, std::ios::binary | std::ios::out | std::ios::trunc);
+    if (!out.is_open()) throw Spc2Error("cannot open " + tmp);
+    const int64_t step = step_;
+    const uint64_t mlen = mts.size();
+    out.write(kEngineCkptMagic, sizeof(kEngineCkptMagic));
+    out.write(reinterpret_cast<const char *>(&step), sizeof(step));
+    out.write(reinterpret_cast<const char *>(&t), sizeof(t));
+    out.write(reinterpret_cast<const char *>(&bytes), sizeof(bytes));
+    out.write(blob.data(), (std::streamsize)bytes);
+    out.write(reinterpret_cast<const char *>(&mlen), sizeof(mlen));
+    out.write(mts.data(), (std::streamsize)mlen);
+    if (!out) throw Spc2Error("writing " + tmp + " failed");
+  }
+  if (std::rename(tmp.c_str(), file_name.c_str()) != 0) throw Spc2Error("cannot rename " + tmp);
+}
+
+Real SimulationEngine::LoadCheckpoint(const std::string &file_name) {
+  std::ifstream in(file_name, std::ios::binary | std::ios::in);
+  if (!in.is_open()) throw Spc2Error("cannot open " + file_name);
+  char magic[8];
+  int64_t step = 0;
+  Real t = 0.0;
+  uint64_t bytes = 0, mlen = 0;
+  in.read(magic, sizeof(magic));
+  if (!in || std::memcmp(magic, kEngineCkptMagic, sizeof(magic)) != 0)
+    throw Spc2Error(file_name + " is not an engine checkpoint");
+  in.read(reinterpret_cast<char *>(&step), sizeof(step));
+  in.read(reinterpret_cast<char *>(&t), sizeof(t));
+  in.read(reinterpret_cast<char *>(&bytes), sizeof(bytes));
+  if (!in || bytes > (uint64_t)1 << 40) throw Spc2Error(file_name + ": truncated checkpoint");
+  std::vector<char> blob((size_t)bytes);
+  in.read(blob.data(), (std::streamsize)bytes);
+  in.read(reinterpret_cast<char *>(&mlen), sizeof(mlen));
+  if (!in || mlen > (uint64_t)1 << 20) throw Spc2Error(file_name + ": truncated checkpoint");
+  std::string mts((size_t)mlen, '\0');
+  in.read(&mts[0], (std::streamsize)mlen);
+  if (!in) throw Spc2Error(file_name + ": truncated checkpoint");
+  Spc2Check(spc2_checkpoint_load(system_->handle(), blob.data(), bytes), "spc2_checkpoint_load");
+  std::istringstream mt(mts);
+  mt >> mt_;
+  if (!mt) throw Spc2Error(file_name + ": bad mt19937 state");
+  step_ = step;
+  return t;
+}
+
 std::vector<Real> SimulationEngine::State() const {
   std::vector<Real> st((size_t)kS * (size_t)cfg_.n);
   system_->Download(st.data());
   return st;
 }
 
-// Number of loop iterations of `while (t <= t_1) t += dt` from t_0.
-long SimulationEngine::StepCount() const {
+// Number of loop iterations of `while (t <= t_1) t += dt` from t_start.
+long SimulationEngine::StepCount(Real t_start) const {
   if (cfg_.t1 < 0.0) return cfg_.steps;
   long k = 0;
-  for (Real t = cfg_.t0; t <= cfg_.t1; t += cfg_.dt) ++k;
+  for (Real t = t_start; t <= cfg_.t1; t += cfg_.dt) ++k;
   return k;
 }
 
@@ -599,18 +698,23 @@ long SimulationEngine::StepCount() const {
 // the device in one call.
 void SimulationEngine::Run() {
   const int32_t method = ParseMethod(cfg_.method);
-  if (!cfg_.init_file.empty())
+  Real t = cfg_.t0;
+  if (!cfg_.resume.empty()) {
+    t = LoadCheckpoint(cfg_.resume);
+  } else if (!cfg_.init_file.empty()) {
     InitializeSystemStateFromPreviousSolution(cfg_.init_file, cfg_.init_frame);
-  else
+  } else if (!cfg_.init_state_file.empty()) {
+    InitializeSystemStateFromFile(cfg_.init_state_file);
+  } else {
     InitializeRandomSystemState();
+  }
 
-  const long total = StepCount();
+  const long total = StepCount(t);
   if (cfg_.loop == "host") {
     RunHostLoop(total);
     return;
   }
   BinaryObserver obs(cfg_.state_path, cfg_.summary_path, cfg_.state_every, cfg_.summary_every, cfg_.float_summary);
-  Real t = cfg_.t0;
   obs.SaveSystemState(*system_, t);
   const bool mt_noise = cfg_.noise == "mt19937" && method != SPC2_RK4;
   const long per_step = method == SPC2_SRK ? 2 * cfg_.n : cfg_.n;
@@ -619,7 +723,9 @@ void SimulationEngine::Run() {
   auto t_start = std::chrono::steady_clock::now();
   long done = 0;
   while (done < total) {
-    const long k = std::min(total - done, obs.CallsUntilOutput());
+    long k = std::min(total - done, obs.CallsUntilOutput());
+    if (cfg_.checkpoint_every > 0 && !cfg_.checkpoint.empty())
+      k = std::min(k, cfg_.checkpoint_every - done % cfg_.checkpoint_every);
     if (mt_noise) {
       // the reference's per-step draws: a fresh N(0,1) per DoStep, particles
       // in index order, (U_1, U_2) per particle for SRK
@@ -637,9 +743,12 @@ void SimulationEngine::Run() {
     if (k > 1) obs.SkipCalls(k - 1);
     obs.SaveSystemState(*system_, t);
     obs.SaveSummaryStatistics(*system_, t);
+    if (cfg_.checkpoint_every > 0 && !cfg_.checkpoint.empty() && done % cfg_.checkpoint_every == 0)
+      SaveCheckpoint(cfg_.checkpoint, t);
   }
   obs.Flush();
   seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  if (!cfg_.checkpoint.empty()) SaveCheckpoint(cfg_.checkpoint, t);
 }
 
 // The reference's loop verbatim (SimulationEnginePtr.cpp:104-115) on a host

@@ -83,6 +83,15 @@ class DeviceSystem {
 };
 
 enum class Precision { kFP64 = SPC2_FP64, kFP32Kahan = SPC2_FP32_KAHAN, kFP64Factored = SPC2_FP64_FACTORED };
+// neighbour search: the reference's dispatch (linked cells when rho <= L/4, else
+// all pairs), linked cells, all pairs, or a Verlet list of skin verlet_skin
+// (0: 0.2 rho)
+enum class NeighbourMode {
+  kAuto = SPC2_NEIGH_AUTO,
+  kCells = SPC2_NEIGH_CELLS,
+  kAllPairs = SPC2_NEIGH_ALL_PAIRS,
+  kVerlet = SPC2_NEIGH_VERLET
+};
 
 class ChimeraSystem : public DeviceSystem {
  public:
@@ -90,7 +99,7 @@ class ChimeraSystem : public DeviceSystem {
   // seed appended; L comes from the boundary object
   ChimeraSystem(Real v_0, Real sigma, Real rho, Real alpha, Real D_phi, Real rho_0,
                 PeriodicBoundaryConditions &pbc_config, long n, Precision precision = Precision::kFP64,
-                uint64_t seed = 42);
+                uint64_t seed = 42, NeighbourMode neighbour = NeighbourMode::kAuto, Real verlet_skin = 0.0);
   void EvaluateRhs(const Real *const system_state, const std::vector<Real> &k_prev, std::vector<Real> &k_next,
                    Real k_coef, Real dt);
   void EvaluateRhs(const Real *const system_state, std::vector<Real> &derivative,
@@ -106,7 +115,8 @@ class VortexArraysSystem : public DeviceSystem {
   // reference order (VortexArraysSystemPtr.hpp:16-25) with N, precision and seed appended
   VortexArraysSystem(Real velocity, Real mu_plus, Real mu_minus, Real xi_a, Real xi_r, Real D_phi, Real kappa,
                      Real rho, PeriodicBoundaryConditions &pbc_config, long n, Precision precision = Precision::kFP64,
-                     uint64_t seed = 42);
+                     uint64_t seed = 42, NeighbourMode neighbour = NeighbourMode::kAuto,
+                     Real verlet_skin = 0.0);
   void EvaluateRhs(const Real *const system_state, const std::vector<Real> &k_prev, std::vector<Real> &k_next,
                    Real k_coef, Real dt);
   void EvaluateRhs(const Real *const system_state, std::vector<Real> &derivative,
@@ -236,8 +246,17 @@ struct EngineConfig {
   std::string state_path, summary_path;  // empty: no output
   int state_every = 10, summary_every = 10;
   bool float_summary = false;
+  std::string neighbour = "auto";  // auto | cells | all_pairs | verlet
+  Real verlet_skin = 0.0;          // Verlet list skin (0: 0.2 rho)
   std::string init_file;   // non-empty: start from a frame of a previous trajectory
   long init_frame = 0;
+  std::string init_state_file;  // non-empty: start from a file of 3N floats (SimulationEngine.cpp:52-73)
+  // exact restarts: `resume` loads an engine checkpoint (state, k buffers,
+  // frequencies, Verlet list, t, step index and the mt19937 stream) and runs
+  // `steps` more steps (or until t1); `checkpoint` writes one at the end of the
+  // run and, with checkpoint_every > 0, every checkpoint_every steps
+  std::string resume, checkpoint;
+  long checkpoint_every = 0;
   // key=value parser used by the command-line runner
   void Set(const std::string &key, const std::string &value);
 };
@@ -249,6 +268,12 @@ class SimulationEngine {
   void InitializeRandomSystemState();
   // frame `frame` of a float trajectory file (SimulationEnginePtr.cpp:52-73 reads frame 2900)
   void InitializeSystemStateFromPreviousSolution(const std::string &file_name, long frame);
+  // 3N floats from the start of a file, no time stamp (SimulationEngine.cpp:52-73)
+  void InitializeSystemStateFromFile(const std::string &file_name);
+  // exact checkpoint of the run so far (the library's checkpoint of the
+  // handle, the time, the step index and the mt19937 stream), and its reverse
+  void SaveCheckpoint(const std::string &file_name, Real t) const;
+  Real LoadCheckpoint(const std::string &file_name);
   void RunChimeraSimulation();
   void RunVortexArraySimulation();
   // host copy of the current state (AoS, by particle id)
@@ -260,7 +285,7 @@ class SimulationEngine {
  private:
   void Run();
   void RunHostLoop(long total);
-  long StepCount() const;
+  long StepCount(Real t_start) const;
   EngineConfig cfg_;
   PeriodicBoundaryConditions pbc_;
   DeviceSystem *system_ = nullptr;

@@ -46,7 +46,8 @@ int main(int argc, char **argv) {
     std::cout << "usage: spc2_run [config=FILE] [key=value ...]\n"
                  "keys: system method precision init noise loop n L v0 sigma rho alpha D rho0 omega_std\n"
                  "      mu_plus mu_minus xi_a xi_r kappa dt t0 t1 steps seed state summary\n"
-                 "      state_every summary_every float_summary init_file init_frame\n";
+                 "      state_every summary_every float_summary init_file init_frame init_state_file\n"
+                 "      neighbour verlet_skin resume checkpoint checkpoint_every\n";
     return 0;
   }
   try {

@@ -442,3 +442,29 @@ def test_srk_ensemble_matches_reference_ci(gpu, golden):
     se = np.sqrt(Rg.var(0, ddof=1)[1:] / members + Rref.var(0, ddof=1)[1:] / members)
     z = np.abs(mg - mr) / np.maximum(se, 1e-12)
     assert np.all(z <= 3.0), (z, mg, mr)
+
+
+# ---- per-particle partner counts --------------------------------------------------
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+@pytest.mark.parametrize("system,neighbour,n", [("vortex", "cells", 40000), ("vortex", "verlet", 40000),
+                                                ("vortex", "auto", 300), ("chimera", "cells", 30000),
+                                                ("chimera", "verlet", 30000)])
+def test_neighbour_counts_match_oracle(gpu, prec, system, neighbour, n):
+    """spc2_neighbour_counts gives each particle's partners within rho and
+    within xi_r, exactly the oracle's in every precision."""
+    L = math.sqrt(n / 4000.0)
+    rho = 0.05 if system == "vortex" else 0.1
+    st = O.make_state(50 + n, 1, n, L, rho)
+    if system == "vortex":
+        p = vort_params(n, L, precision=prec, neighbour=neighbour)
+        sysp = orc_vortex(n, L)
+    else:
+        p = chim_params(n, L, rho, 0.01, 1.0, 1.45, 0.01, prec, neighbour=neighbour)
+        sysp = O.chimera(n, L, rho, threads=8)
+    with spc2.Sim(p) as s:
+        s.set_state(st)
+        a, b = s.neighbour_counts()
+    ra, rb = O.neighbour_counts(sysp, st)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+    assert ra.sum() > 0 and (rb.sum() > 0) == (system == "vortex")

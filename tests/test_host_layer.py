@@ -58,6 +58,12 @@ def test_runner_help_and_config_errors(built, tmp_path):
     assert r.returncode == 1 and "method must be se, srk or rk4" in r.stderr
     r = subprocess.run([RUNNER, "loop=both"], capture_output=True, text=True)
     assert r.returncode == 1 and "loop must be device or host" in r.stderr
+    r = subprocess.run([RUNNER, "neighbour=kdtree"], capture_output=True, text=True)
+    assert r.returncode == 1 and "neighbour must be auto, cells, all_pairs or verlet" in r.stderr
+    r = subprocess.run([RUNNER, "loop=host", f"checkpoint={tmp_path / 'c'}"], capture_output=True, text=True)
+    assert r.returncode == 1 and "checkpoints need loop=device" in r.stderr
+    r = subprocess.run([RUNNER, "checkpoint_every=-1"], capture_output=True, text=True)
+    assert r.returncode == 1 and "checkpoint_every must be >= 0" in r.stderr
 
 
 def test_host_library_exports_the_server_interface(built):
