@@ -221,6 +221,43 @@ int Steps(ThreadSharedMemory &th, char **a, Args p, int normals_per_particle) {
   return 0;
 }
 
+// Ensemble of the reference's vortex SRK stepper driven by its own mt19937 +
+// std::normal_distribution noise (generator seeded 2000 + member): |R|(t) of
+// the Kuramoto order parameter in FP64 every `every` steps, from positions as
+// MakeState(seed + member, 0) with every phase 1 (R = 1 at t = 0).  One rank.
+// <seed> <members> <steps> <every> <dt> <vortex args>
+int EnsembleVortexSrk(ThreadSharedMemory &th, char **a, VortexArgs p) {
+  uint64_t seed = std::strtoull(a[0], nullptr, 10);
+  int members = std::atoi(a[1]), steps = std::atoi(a[2]), every = std::atoi(a[3]);
+  double dt = std::atof(a[4]);
+  PeriodicBoundaryConditions pbc(&th, kL, kL);
+  std::vector<Real> out;
+  for (int r = 0; r < members; ++r) {
+    mersenne_twister_generator.seed(uint32_t(2000 + r));
+    VortexSys s(th, pbc, p);
+    StochasticRungeKuttaStepperPtr stepper(&th);
+    std::vector<Real> st = MakeState(seed + uint64_t(r), 0);
+    for (int i = 0; i < kN; ++i) st[kS * i + 2] = 1.0;
+    double t = 0.0;
+    for (int k = 0; k <= steps; ++k) {
+      if (k % every == 0) {
+        double cr = 0.0, ci = 0.0;
+        for (int i = 0; i < kN; ++i) {
+          cr += std::cos(st[kS * i + 2]);
+          ci += std::sin(st[kS * i + 2]);
+        }
+        out.push_back(std::sqrt(cr * cr + ci * ci) / kN);
+      }
+      if (k == steps) break;
+      t += dt;
+      stepper.DoStep(s.sys, st.data(), t, dt);
+      pbc.ApplyPeriodicBoundaryConditions(st.data(), kS * kN);
+    }
+  }
+  Write("R", out);
+  return 0;
+}
+
 // Seconds per step of `steps` RK4 steps of the vortex system after `warm`
 // untimed ones (rank 0 prints one JSON line).  <seed> <warm> <steps> <dt> <vortex args>
 int TimeRk4(ThreadSharedMemory &th, char **a, VortexArgs p) {
@@ -273,7 +310,13 @@ int main(int argc, char **argv) {
       const int na = argc - 3;
       const bool vort = c[0] == 'v', chim = c[0] == 'c';
       const int sys_args = vort ? 8 : 5;
-      if ((vort || chim || c == "time_rk4") && na < 4 + (c == "time_rk4" ? 8 : sys_args)) {
+      if (c == "ens_vsrk") {
+        if (na < 13) {
+          if (g_rank == 0) std::fprintf(stderr, "case %s: too few arguments\n", c.c_str());
+        } else {
+          rc = EnsembleVortexSrk(th, a, ParseVortex(a + 5));
+        }
+      } else if ((vort || chim || c == "time_rk4") && na < 4 + (c == "time_rk4" ? 8 : sys_args)) {
         if (g_rank == 0) std::fprintf(stderr, "case %s: too few arguments\n", c.c_str());
       } else if (c == "vrhs_k") {
         rc = RhsK<VortexSys>(th, a, ParseVortex(a + 4));

@@ -8,7 +8,7 @@ array, its SHA-256 over the raw little-endian bytes, a few sampled entries and
 summary statistics.  Inputs are regenerated from the seed by splitmix64, so the
 fixture holds no input data.  The reference fixes N = 8192, L = 64.
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [--only-vortex-ensemble]
 """
 from __future__ import annotations
 
@@ -100,6 +100,13 @@ HARNESS_MPI = os.path.join(ROOT, "oracle", "_ref", "ref_harness_mpi")
 # (seeds 1000 + r): |R|(t) every 10 steps, for the 95 % CI comparison.
 ENSEMBLE = ("ens_srk", ["21", "24", "200", "10", "0.01"] + CHIM(1.0, 1.0, 4.0, 1.45, 0.05))
 
+# Ensemble of the reference's vortex SRK stepper (VortexArraysSystemPtr +
+# StochasticRungeKuttaStepperPtr, its own mt19937 noise, generator seeds 2000 + r,
+# one rank): |R|(t) every 10 steps from aligned phases (R = 1 at t = 0) as the
+# state dephases, dt = 0.001, D = 0.5.
+ENSEMBLE_VORTEX = ("ens_vsrk", ["51", "24", "200", "10", "0.001"] + VORT(1.0, 100.0, 100.0, 0.4472, 0.2236, 0.5,
+                                                                          20.0, 2.236))
+
 SAMPLE_IDX = [0, 1, 2, 3, 100, 1000, 4095, 4096, 8191]
 
 
@@ -124,9 +131,28 @@ def run_case(case, args, outputs, tmp, ranks=0):
     return {k: digest(np.fromfile(os.path.join(tmp, k + ".bin"), np.dtype(dt))) for k, dt in outputs.items()}
 
 
+def vortex_ensemble(tmp):
+    case, args = ENSEMBLE_VORTEX
+    subprocess.run([MPIEXEC, "-n", "1", HARNESS_MPI, case, tmp] + args, check=True)
+    members = int(args[1])
+    R = np.fromfile(os.path.join(tmp, "R.bin")).reshape(members, -1)
+    return {"case": case, "args": args, "members": members, "every": int(args[3]), "R": R.tolist()}
+
+
 def main():
     if not os.path.exists(HARNESS) or not os.path.exists(HARNESS_MPI):
         sys.exit("build the reference harnesses first: make -C oracle ref")
+    if len(sys.argv) > 1 and sys.argv[1] == "--only-vortex-ensemble":
+        # regenerates that entry alone, keeping the rest of the fixture
+        path = os.path.join(HERE, "golden.json")
+        with open(path) as f:
+            out = json.load(f)
+        with tempfile.TemporaryDirectory() as tmp:
+            out["ensemble_vortex"] = vortex_ensemble(tmp)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        print("wrote", path)
+        return
     out = {
         "about": ("Golden vectors from the reference's own classes: the non-MPI ones (ref_harness.cpp) and "
                   "the MPI shared-memory ones under mpiexec (ref_harness_mpi.cpp, cases marked harness=mpi); "
@@ -161,6 +187,7 @@ def main():
         R = R.reshape(members, -1)
         out["ensemble"] = {"case": case, "args": args, "members": members, "every": int(args[3]),
                            "R": R.tolist()}
+        out["ensemble_vortex"] = vortex_ensemble(tmp)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", os.path.join(HERE, "golden.json"))

@@ -468,3 +468,39 @@ def test_neighbour_counts_match_oracle(gpu, prec, system, neighbour, n):
     ra, rb = O.neighbour_counts(sysp, st)
     assert np.array_equal(a, ra) and np.array_equal(b, rb)
     assert ra.sum() > 0 and (rb.sum() > 0) == (system == "vortex")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+def test_vortex_srk_ensemble_matches_reference_ci(gpu, golden, prec):
+    """StochasticRungeKuttaStepperPtr on the vortex system
+    (StochasticRungeKuttaStepperPtr.hpp:70-97): the ensemble mean |R|(t) of 24
+    runs with Philox noise against the reference's own 24 runs with its mt19937
+    noise (tests/golden, ensemble_vortex), from aligned phases as the state
+    dephases.  Family-wise 95% over the 20 sampled times (Bonferroni:
+    |z| <= 3.02 at each)."""
+    ens = golden["ensemble_vortex"]
+    a = ens["args"]
+    seed, members, steps, every, dt = int(a[0]), int(a[1]), int(a[2]), int(a[3]), float(a[4])
+    v, mp, mm, xa, xr, D, kappa, rho = (float(x) for x in a[5:])
+    Rref = np.array(ens["R"])
+    Rg = []
+    for r in range(members):
+        st = O.make_state(seed + r, 0, N8, L64, rho)
+        st[2::3] = 1.0
+        p = spc2.params("vortex", prec, n=N8, L=L64, rho=rho, velocity=v, mu_plus=mp, mu_minus=mm, xi_a=xa,
+                        xi_r=xr, kappa=kappa, D=D, seed=7000 + r)
+        with spc2.Sim(p) as s:
+            s.set_state(st)
+            row = [s.order_parameter()[0]]
+            for k in range(steps // every):
+                s.step("srk", dt, step0=k * every, nsteps=every)
+                row.append(s.order_parameter()[0])
+        Rg.append(row)
+    Rg = np.array(Rg)
+    assert np.allclose(Rg[:, 0], 1.0, rtol=0, atol=1e-12) and np.allclose(Rref[:, 0], 1.0, rtol=0, atol=1e-12)
+    mg, mr = Rg.mean(0)[1:], Rref.mean(0)[1:]
+    se = np.sqrt(Rg.var(0, ddof=1)[1:] / members + Rref.var(0, ddof=1)[1:] / members)
+    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
+    print(f"vortex SRK ensemble {prec}: max |z| {z.max():.2f}; mean R at t = 0.1: gpu {mg[9]:.4f}, "
+          f"reference {mr[9]:.4f}")
+    assert np.all(z <= 3.02), (z, mg, mr)
