@@ -6,10 +6,23 @@ Workload: vortex arrays, deterministic RK4 (D = 0), dt = 0.01, N = 1,048,576 in 
 periodic box at density 4000 (L = sqrt(N / 4000) = 16.19086), rho = 0.05,
 xi_a = 0.01, xi_r = 0.005, v = 0.025, mu+ = mu- = 100, kappa = 20; uniform
 random positions and phases (Philox, seed 42).  Neighbours from a Verlet list
-with skin 0.125 rho, rebuilt before any particle can have moved skin / 2
-(--neighbour cells: the reference's linked cells rebuilt at every RHS).  Inputs
-are generated on the device and stay resident; a step is one full RK4 step
-(4 RHS evaluations + combine + wrap, plus the list rebuilds that fall in it).
+with skin 0.2 rho (the workload's stated skin; --skin), rebuilt before any
+particle can have moved skin / 2 (--neighbour cells: the reference's linked
+cells rebuilt at every RHS).  Inputs are generated on the device and stay
+resident; a step is one full RK4 step (4 RHS evaluations + combine + wrap, plus
+the list rebuilds that fall in it).  The FP32 form is also timed at the tuned
+skin 0.125 rho (value_tuned_skin).
+
+Besides the timings the line carries the FP32-vs-FP64 errors at the workload's
+500-step horizon, framed by the chaos of the system: an ensemble of FP64 runs
+from initial phases perturbed by 1e-7 gives the spread any rounding-size
+difference grows to; the order parameter is also compared time-averaged, with
+95% confidence intervals from batch means; and phase errors are split between
+particles whose neighbour counts differed between the FP32 and FP64 runs at
+some step (a cutoff crossing, not a rounding error) and the others.
+
+Order: error analysis, pair-kernel profiling and the FP64 timings run first,
+the headline FP32 timing last (the GPU busy right up to its timed region).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--neighbour verlet|cells]
 """
@@ -28,10 +41,10 @@ sys.path.insert(0, ROOT)
 PEAK = {"fp32k": 157.3, "fp64": 78.6, "fp64f": 78.6}  # TFLOP/s, MI355X vector FP32 / FP64
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the FP32 Verlet pair kernel (k_pass1_verlet_lds
-# <VortAcc<P32K>>) on this workload (skin 0.125 rho), from the FETCH_SIZE (x2.000)
+# <VortAcc<P32K>>) on this workload (skin 0.2 rho), from the FETCH_SIZE (x2.000)
 # and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
 # copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 228.0e6
+TRAFFIC_VERLET_FP32K = 240.2e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -114,16 +127,20 @@ def fixed_bytes_per_particle(w, precision, neighbour):
     return own + 4 + (24 * 3 + 8) / 4.0 + (72 + 48) / 4.0
 
 
-def profile_pair_kernel(spc2, w, precision, steps, neighbour):
-    """HIP-event timing of the pair kernels on the handle's stream, with pair counts."""
+def profile_pair_kernel(spc2, w, precision, warmup, steps, neighbour):
+    """HIP-event timing of the pair kernels on the handle's stream over the
+    timed region's steps (warmup .. warmup + steps of a fresh handle), and the
+    pair counts of the same steps on another fresh handle."""
     s = make_sim(spc2, w, precision, neighbour)
-    s.step(w["method"], w["dt"], 0, 2)
+    s.step(w["method"], w["dt"], 0, warmup)
     s.set_profiling(True)            # timing pass: events bound to the pair-kernel dispatches
-    s.step(w["method"], w["dt"], 2, steps)
+    s.step(w["method"], w["dt"], warmup, steps)
     ms, launches = s.profile()
-    s.set_profiling(False)
-    s.set_counting(True)             # counting pass over the same number of launches
-    s.step(w["method"], w["dt"], 2 + steps, steps)
+    s.close()
+    s = make_sim(spc2, w, precision, neighbour)
+    s.step(w["method"], w["dt"], 0, warmup)
+    s.set_counting(True)             # counting pass over the same launches
+    s.step(w["method"], w["dt"], warmup, steps)
     cnt = s.counts()
     lds = neighbour == "verlet" and s.verlet_layout()[0] == 1
     s.close()
@@ -155,43 +172,119 @@ def phase_diff(a, b):
     return np.abs(np.mod(a - b + np.pi, 2 * np.pi) - np.pi)
 
 
-def error_horizon(spc2, w, nb, horizons):
-    """FP32-compensated vs FP64 from the same initial state after each of
-    `horizons` steps, and the FP64 run against FP64 with the other neighbour
-    structure (same arithmetic, another summation order): the divergence floor
-    of this chaotic system at that horizon."""
+def order_param(phases):
     import numpy as np
-    horizons = sorted(set(horizons))
-    ph = {}
+    return float(np.abs(np.exp(1j * phases).mean()))
+
+
+def batch_means_ci(series, batches=5):
+    """Mean of a correlated time series with a 95% confidence half-width from
+    batch means (Student t, batches - 1 degrees of freedom)."""
+    import numpy as np
+    x = np.asarray(series, float)
+    k = len(x) // batches
+    m = x[:k * batches].reshape(batches, k).mean(1)
+    t975 = {2: 12.706, 3: 4.303, 4: 3.182, 5: 2.776, 6: 2.571, 8: 2.365, 10: 2.262}.get(batches, 2.0)
+    return float(x.mean()), float(t975 * m.std(ddof=1) / math.sqrt(batches))
+
+
+def error_analysis(spc2, w, nb, horizon=500, members=8, flip_steps=25, every=10):
+    """FP32-compensated against FP64 (the reference method) from the same
+    initial state over `horizon` steps:
+      * the order parameter R every `every` steps, its difference at the
+        horizon against the spread of `members` FP64 runs whose initial phases
+        are perturbed by U(-1e-7, 1e-7) (the growth of rounding-size
+        differences in this chaotic system), and the time-averaged R of both
+        runs with 95% confidence intervals;
+      * phase errors after `flip_steps` and 100 steps, split between
+        particles whose partner counts (within rho, within xi_r) differed
+        between the two runs at one of the first `flip_steps` steps and the
+        others;
+      * FP64 against FP64 with the other neighbour structure (another
+        summation order) as the floor at the same horizons."""
+    import numpy as np
+    res = {}
+    series, phases = {}, {}
+    checkpoints = (flip_steps, 100, horizon)
     other = "cells" if nb == "verlet" else "verlet"
-    for prec, neigh in (("fp64", nb), ("fp32k", nb), ("fp64", other)):
+    counts = {}
+    for name, prec, neigh in (("fp64", "fp64", nb), ("fp32", "fp32k", nb), ("fp64_reordered", "fp64", other)):
         s = make_sim(spc2, w, prec, neigh)
-        k = 0
-        for h in horizons:
-            s.step(w["method"], w["dt"], k, h - k)
-            k = h
-            ph[(prec, neigh, h)] = s.get_state()[2::3]
+        rs = [order_param(s.get_state()[2::3])]
+        track = name in ("fp64", "fp32")
+        cnt = []
+        for k in range(horizon):
+            s.step(w["method"], w["dt"], k, 1)
+            if track and k < flip_steps:
+                cnt.append(s.neighbour_counts())
+            if (k + 1) % every == 0 or (k + 1) in checkpoints:
+                ph = s.get_state()[2::3]
+                if (k + 1) % every == 0:
+                    rs.append(order_param(ph))
+                if (k + 1) in checkpoints:
+                    phases[(name, k + 1)] = ph
+        counts[name] = cnt
+        series[name] = rs
         s.close()
-    R = lambda p: float(np.abs(np.exp(1j * p).mean()))
-    out = {}
-    for h in horizons:
-        ref = ph[("fp64", nb, h)]
-        for key, name in ((("fp32k", nb, h), "fp32"), (("fp64", other, h), "fp64_reordered")):
-            d = phase_diff(ph[key], ref)
-            out[(name, h)] = {"phase_err_max": float(d.max()), "phase_err_rms": float(np.sqrt((d ** 2).mean())),
-                              "order_param_abs_err": abs(R(ph[key]) - R(ref))}
-    return out
+    flip = np.zeros(w["n"], bool)
+    for (a32, b32), (a64, b64) in zip(counts["fp32"], counts["fp64"]):
+        flip |= (a32 != a64) | (b32 != b64)
+    for h in (flip_steps, 100):
+        ref = phases[("fp64", h)]
+        d = phase_diff(phases[("fp32", h)], ref)
+        dr = phase_diff(phases[("fp64_reordered", h)], ref)
+        entry = {"order_param_abs_err": abs(order_param(phases[("fp32", h)]) - order_param(ref)),
+                 "phase_err_max": float(d.max()), "phase_err_rms": float(np.sqrt((d ** 2).mean())),
+                 "fp64_reordered_phase_err_rms": float(np.sqrt((dr ** 2).mean())),
+                 "fp64_reordered_order_param_abs_err": abs(order_param(phases[("fp64_reordered", h)]) -
+                                                           order_param(ref))}
+        for tag, m in (("with_count_flip", flip), ("without_count_flip", ~flip)):
+            entry[tag] = {"particles": int(m.sum()),
+                          "phase_err_max": float(d[m].max()) if m.any() else 0.0,
+                          "phase_err_rms": float(np.sqrt((d[m] ** 2).mean())) if m.any() else 0.0}
+        res[f"after_{h}_steps"] = entry
+    # FP64 members from perturbed initial phases
+    st0 = None
+    R_members = []
+    for m in range(members):
+        s = make_sim(spc2, w, "fp64", nb)
+        if st0 is None:
+            st0 = s.get_state()
+        st = st0.copy()
+        st[2::3] += np.random.default_rng(1000 + m).uniform(-1e-7, 1e-7, w["n"])
+        s.set_state(st)
+        s.step(w["method"], w["dt"], 0, horizon)
+        R_members.append(order_param(s.get_state()[2::3]))
+        s.close()
+    R64, R32 = series["fp64"][-1], series["fp32"][-1]
+    spread = np.array(R_members) - R64
+    m64, ci64 = batch_means_ci(series["fp64"][1:])
+    m32, ci32 = batch_means_ci(series["fp32"][1:])
+    res[f"after_{horizon}_steps"] = {
+        "order_param_fp64": R64, "order_param_fp32": R32, "order_param_abs_err": abs(R32 - R64),
+        "fp64_perturbed_members": members, "fp64_perturbation": "initial phases + U(-1e-7, 1e-7)",
+        "fp64_perturbed_order_param_spread_rms": float(np.sqrt((spread ** 2).mean())),
+        "fp64_perturbed_order_param_spread_max": float(np.abs(spread).max()),
+        "fp32_err_within_fp64_spread": bool(abs(R32 - R64) <= max(np.abs(spread).max(), 1e-300)),
+        "fp64_reordered_order_param_abs_err": abs(series["fp64_reordered"][-1] - R64),
+        "time_avg_order_param_fp64": m64, "time_avg_order_param_fp64_ci95": ci64,
+        "time_avg_order_param_fp32": m32, "time_avg_order_param_fp32_ci95": ci32,
+        "time_avg_order_param_abs_err": abs(m32 - m64),
+        "time_avg_within_ci95": bool(abs(m32 - m64) <= math.hypot(ci64, ci32)),
+        "order_param_every": every,
+    }
+    return res
 
 
 COPY_BYTES = 1 << 29  # calibration copy: reads and writes this many bytes
 
 
-def pmc_run(spc2, w, nb):
+def pmc_run(spc2, w, nb, precision="fp32k"):
     """Workload steps and a calibration copy for the FETCH_SIZE / WRITE_SIZE
     counters: the copy reads and writes COPY_BYTES, so the counters' ratio to
     it converts their units to bytes."""
     import torch
-    s = make_sim(spc2, w, "fp32k", nb)
+    s = make_sim(spc2, w, precision, nb)
     s.step(w["method"], w["dt"], 0, 3)
     s.synchronize()
     s.close()
@@ -200,7 +293,8 @@ def pmc_run(spc2, w, nb):
     for _ in range(2):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"pmc_run": True, "copy_bytes": COPY_BYTES, "n": w["n"], "neighbour": nb}), flush=True)
+    print(json.dumps({"pmc_run": True, "copy_bytes": COPY_BYTES, "n": w["n"], "neighbour": nb,
+                      "precision": precision}), flush=True)
 
 
 def cpu_baseline(w, target_s=12.0, max_steps=40):
@@ -237,9 +331,20 @@ def cpu_baseline(w, target_s=12.0, max_steps=40):
         sec += dt_s
     return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} {w['method'].upper()} steps of the same N={n} {w['system']} workload, FP64 "
-                      f"reference method "
-                      f"(oracle/spc2_oracle.c, -O2), {threads} threads, contiguous particle slices",
+                      f"reference method (oracle/spc2_oracle.c, -O3, bit-identical to the reference's own build), "
+                      f"{threads} threads over contiguous particle slices; the port runs at 0.8-1.2x the speed of "
+                      f"the reference's MPI build (-O3, mpiexec -n 1..8) on the same cores "
+                      f"(profiles/cpu_reference_calibration.md)",
             "seconds": sec}
+
+
+# Survey section 8(d) byte convention for the vortex RK4 + Verlet workload:
+# 1,108 B per particle-step (4 x [24 + 12 + 12 + 45.2 x 4 list + 12 + 12] + 96
+# combine) over the step's 4 pair-kernel launches.
+SURVEY_BYTES_PER_PARTICLE_STEP = 1108.0
+# VALU issue of the same kernel from the SQ counter pass (SQ_INSTS_VALU x 4
+# cycles over the 1024 SIMDs during the launch); profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
+VALU_ISSUE_FRAC_VERLET_FP32K = 0.83
 
 
 def main():
@@ -247,7 +352,6 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precisions", default="fp32k,fp64,fp64f",
@@ -259,10 +363,14 @@ def main():
     ap.add_argument("--pmc-run", action="store_true",
                     help="for rocprofv3 --pmc: a few steps of the FP32 workload and a plain device copy of "
                          "COPY_BYTES bytes (the counter calibration), no timing")
-    ap.add_argument("--skin", type=float, default=0.125,
-                    help="Verlet skin as a fraction of rho (0.125: the fastest of 0.1-0.3 on this workload)")
-    ap.add_argument("--error-horizon", type=int, default=100,
-                    help="steps after which FP32 and FP64 runs are compared before chaotic decorrelation")
+    ap.add_argument("--pmc-precision", default="fp32k", choices=["fp32k", "fp64", "fp64f"],
+                    help="the form --pmc-run steps")
+    ap.add_argument("--skin", type=float, default=0.2,
+                    help="Verlet skin as a fraction of rho (0.2: the workload's stated skin)")
+    ap.add_argument("--tuned-skin", type=float, default=0.125,
+                    help="skin of the extra FP32 timing (0.125: the fastest of 0.1-0.3 on this workload; 0: none)")
+    ap.add_argument("--error-horizon", type=int, default=500,
+                    help="steps of the FP32-vs-FP64 error analysis (the workload's 500; 0: none)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (a large system would shard by "
@@ -273,46 +381,73 @@ def main():
     w = workload(args.n, args.workload)
     w["skin"] = args.skin
     if args.pmc_run:
-        pmc_run(spc2, w, args.neighbour)
+        pmc_run(spc2, w, args.neighbour, args.pmc_precision)
         return
-    res = {}
     precs = args.precisions.split(",")
+    head = precs[0]
     nb = args.neighbour
-    for prec in precs:
+    verlet = nb == "verlet"
+    headline = verlet and args.n == 1048576 and w["system"] == "vortex"
+
+    # untimed analysis first
+    err = None
+    if "fp64" in precs and args.error_horizon > 0:
+        err = error_analysis(spc2, w, nb, args.error_horizon)
+        for k, v in err.items():
+            print(f"# errors {k}: {json.dumps(v)}", file=sys.stderr, flush=True)
+    prof = profile_pair_kernel(spc2, w, "fp32k", args.warmup, args.steps, nb)
+    prof64 = profile_pair_kernel(spc2, w, "fp64", args.warmup, min(args.steps, 50), nb) if "fp64" in precs else None
+    print(f"# pair kernel fp32k: {prof['avg_ms']:.4f} ms/launch, {prof['tflops']:.2f} TFLOP/s, "
+          f"{prof['gbs']:.0f} GB/s" + (f"; fp64: {prof64['avg_ms']:.4f} ms/launch" if prof64 else ""),
+          file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(w)
+
+    # timings: the other forms and the tuned skin, then the headline form last
+    res = {}
+    tuned = None
+    for prec in precs[1:]:
         res[prec] = time_precision(spc2, w, prec, args.steps, args.warmup, nb)
         print(f"# {prec} ({nb}): {res[prec]['particle_steps_per_s']:.4e} particle-steps/s, "
               f"{res[prec]['ms_per_step']:.3f} ms/step, list builds {res[prec]['list_builds']}",
               file=sys.stderr, flush=True)
-    prof = profile_pair_kernel(spc2, w, "fp32k", args.profile_steps, nb)
-    prof64 = profile_pair_kernel(spc2, w, "fp64", max(4, args.profile_steps // 4), nb) if "fp64" in res else None
-    print(f"# pair kernel fp32k: {prof['avg_ms']:.4f} ms/launch, {prof['tflops']:.2f} TFLOP/s, "
-          f"{prof['gbs']:.0f} GB/s" + (f"; fp64: {prof64['avg_ms']:.4f} ms/launch" if prof64 else ""),
-          file=sys.stderr, flush=True)
-    horizons = [h for h in (args.error_horizon, args.warmup + args.steps) if h > 0]
-    hz = error_horizon(spc2, w, nb, horizons) if "fp64" in res and args.error_horizon > 0 else None
-    if hz:
-        for h in sorted(set(horizons)):
-            print(f"# after {h} steps: fp32 vs fp64 {hz[('fp32', h)]}; fp64 reordered {hz[('fp64_reordered', h)]}",
-                  file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(w)
+    if verlet and args.tuned_skin > 0 and args.tuned_skin != args.skin:
+        wt = dict(w, skin=args.tuned_skin)
+        tuned = time_precision(spc2, wt, head, args.steps, args.warmup, nb)
+        print(f"# {head} at skin {args.tuned_skin} rho: {tuned['particle_steps_per_s']:.4e} particle-steps/s, "
+              f"{tuned['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    res[head] = time_precision(spc2, w, head, args.steps, args.warmup, nb)
+    v = res[head]
+    print(f"# {head} ({nb}, skin {args.skin} rho): {v['particle_steps_per_s']:.4e} particle-steps/s, "
+          f"{v['ms_per_step']:.3f} ms/step, list builds {v['list_builds']}", file=sys.stderr, flush=True)
 
-    v = res["fp32k"]
-    hbm = nb == "verlet"
-    headline = hbm and args.n == 1048576 and w["system"] == "vortex"
-    err = {}
-    if "fp64" in res:
-        import numpy as np
-        dphi = phase_diff(v["phases"], res["fp64"]["phases"])
-        err = {"errors_fp32_vs_fp64": {
-            f"after_{args.warmup + args.steps}_steps": {
-                "phase_err_max": float(dphi.max()), "phase_err_rms": float(np.sqrt((dphi ** 2).mean())),
-                "order_param_abs_err": abs(v["R_final"] - res["fp64"]["R_final"])},
-            **({f"{'fp64_reordered_' if name == 'fp64_reordered' else ''}after_{h}_steps": val
-                for (name, h), val in sorted(hz.items(), key=lambda kv: (kv[0][1], kv[0][0]))
-                if not (name == "fp32" and h == args.warmup + args.steps)} if hz else {})}}
     acc = "VortAcc" if w["system"] == "vortex" else "ChimAcc, Chim2Acc"
-    kname = (f"k_pass1_verlet_lds<{acc}<P32K>>" if hbm else f"k_pass1_lds<{acc}<P32K>>") + \
+    kname = (f"k_pass1_verlet_lds<{acc}<P32K>>" if verlet else f"k_pass1_lds<{acc}<P32K>>") + \
         f" ({w['system']} pair sums, FP32 Kahan)"
+    avg_s = prof["avg_ms"] / 1e3
+    if headline:
+        # survey 8(d): 1,108 B per particle-step over 4 launches
+        alg_bytes = SURVEY_BYTES_PER_PARTICLE_STEP / 4.0 * w["n"]
+        roof = {"bound": "hbm", "kernel": kname, "achieved": alg_bytes / avg_s / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": alg_bytes / avg_s / 1e9 / HBM_PEAK_GBS,
+                "traffic": TRAFFIC_VERLET_FP32K, "traffic_GBs": TRAFFIC_VERLET_FP32K / avg_s / 1e9,
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "algorithmic_bytes_convention": "1,108 B per particle-step (4 launches), 45.2 list entries x 4 B",
+                "limiter": "VALU issue", "valu_issue_frac": VALU_ISSUE_FRAC_VERLET_FP32K,
+                "measured_list_bytes_model_per_launch": prof["bytes_per_launch"],
+                "measured_list_bytes_model_GBs": prof["gbs"]}
+    elif verlet:
+        roof = {"bound": "hbm", "kernel": kname, "achieved": prof["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": prof["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                "algorithmic_bytes_per_launch": prof["bytes_per_launch"]}
+    else:
+        roof = {"bound": "valu", "kernel": kname, "achieved": prof["tflops"], "peak": PEAK["fp32k"],
+                "unit": "TFLOP/s", "frac": prof["tflops"] / PEAK["fp32k"], "traffic": None}
+    roof.update({"avg_launch_ms": prof["avg_ms"], "launches_timed": prof["launches"],
+                 "flops_per_launch": prof["flops_per_launch"], "valu_tflops": prof["tflops"],
+                 "valu_flop_frac": prof["tflops"] / PEAK["fp32k"],
+                 "candidates_per_particle": prof["candidates_per_particle"],
+                 "neighbours_per_particle": prof["neighbours_per_particle"],
+                 "fp64_pair_kernel_avg_ms": prof64["avg_ms"] if prof64 else None})
     line = {
         "metric": ("particle-steps/s at N=1M (FP32-comp vs FP64, same GPU)" if headline else
                    f"particle-steps/s, {args.workload} at N={w['n']} (FP32-comp vs FP64, same GPU)"),
@@ -325,38 +460,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair sums (Kahan) + f64 state",
+        "dtype": "f32 pair sums (Kahan) + f64 state" if head == "fp32k" else head,
         "data": "synthetic (Philox uniform positions and phases, seed 42)",
         "config": {"workload": (f"vortex-arrays RK4, N={w['n']}, density 4000 (L={w['L']:.5f}), rho=0.05, "
                                 "xi_a=0.01, xi_r=0.005, v=0.025, mu+-=100, kappa=20, D=0, dt=0.01, "
                                 if w["system"] == "vortex" else
                                 f"chimera SRK, N={w['n']}, density 4000 (L={w['L']:.5f}), rho=0.1, v0=0.01, "
                                 "sigma=1, alpha=1.45, D=0.01, omega~N(0,0.1), dt=0.005, ")
-                               + (f"Verlet list, skin {args.skin:g} rho, rebuilt at skin/2 displacement" if hbm
+                               + (f"Verlet list, skin {args.skin:g} rho, rebuilt at skin/2 displacement" if verlet
                                   else "linked cells per RHS"),
-                   "n": w["n"], "method": w["method"], "neighbour": nb,
+                   "n": w["n"], "method": w["method"], "neighbour": nb, "skin_over_rho": args.skin if verlet else None,
                    "list_builds_in_timed_steps": v["list_builds"]},
+        "value_tuned_skin": tuned["particle_steps_per_s"] if tuned else None,
+        "ms_per_step_tuned_skin": tuned["ms_per_step"] if tuned else None,
+        "tuned_skin_over_rho": args.tuned_skin if tuned else None,
         "fp64_value": res["fp64"]["particle_steps_per_s"] if "fp64" in res else None,
         "fp64_factored_value": res["fp64f"]["particle_steps_per_s"] if "fp64f" in res else None,
         "fp32_over_fp64": (v["particle_steps_per_s"] / res["fp64"]["particle_steps_per_s"]) if "fp64" in res else None,
-        **err,
-        "roofline": {
-            "bound": "hbm" if hbm else "valu", "kernel": kname,
-            "achieved": prof["gbs"] if hbm else prof["tflops"],
-            "peak": HBM_PEAK_GBS if hbm else PEAK["fp32k"],
-            "unit": "GB/s" if hbm else "TFLOP/s",
-            "frac": prof["gbs"] / HBM_PEAK_GBS if hbm else prof["tflops"] / PEAK["fp32k"],
-            "traffic": TRAFFIC_VERLET_FP32K if headline else None,
-            "traffic_GBs": (TRAFFIC_VERLET_FP32K / (prof["avg_ms"] / 1e3) / 1e9) if headline else None,
-            "avg_launch_ms": prof["avg_ms"], "bytes_per_launch": prof["bytes_per_launch"],
-            "flops_per_launch": prof["flops_per_launch"], "valu_tflops": prof["tflops"],
-            "valu_frac": prof["tflops"] / PEAK["fp32k"],
-            "candidates_per_particle": prof["candidates_per_particle"],
-            "neighbours_per_particle": prof["neighbours_per_particle"],
-            "fp64_pair_kernel_avg_ms": prof64["avg_ms"] if prof64 else None,
-        },
+        "errors_fp32_vs_fp64": err,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if cpu:
+        line["gpu_over_cpu_baseline"] = v["particle_steps_per_s"] / cpu["value"]
     print(json.dumps(line), flush=True)
 
 
