@@ -192,6 +192,7 @@ spc2_status spc2_server::round(std::vector<int64_t> &admit) {
       q.t_done = tf;
       q.final_state = std::move(finals[r]);
       st.completed += 1;
+      st.in_flight -= 1;  // leaves the batch at the next re-pack
       st.particle_steps += (double)q.p.n * (double)q.steps;
       latencies.push_back(q.t_done - q.t_submit);
     }
