@@ -71,7 +71,7 @@ void free_all(Sim &s) {
   f(s.noise); f(s.counts); f(s.red); f(s.bsum); f(s.snap);
   f(s.vcnt); f(s.wmax); f(s.wrow); f(s.vlist); f(s.xb); f(s.yb); f(s.vmask);
   f(s.bh_nrun); f(s.bh_run); f(s.bh_off); f(s.bh_max); f(s.vslot); f(s.wmax_s); f(s.wrow_s);
-  f(s.wmax8); f(s.wrow8);
+  f(s.wmax8); f(s.wrow8); f(s.vrho);
   for (auto &kk : s.kev)
     for (auto &p : kk) f(p);
   if (s.snap_ev) (void)hipEventDestroy(s.snap_ev);
@@ -199,7 +199,7 @@ spc2_status alloc_all(Sim &s) {
       s.vmask_words = (int)std::min(16.0, std::max(1.0, std::ceil(1.5 * mean / 32.0)));
       A(s.vmask, (size_t)s.vmask_words * n);
     }
-    if (s.prec == P32K) {
+    {  // the LDS layout's halo runs and lane slots (every precision form)
       s.nblk = (s.n + 255) / 256;
       A(s.bh_nrun, (size_t)s.nblk); A(s.bh_run, (size_t)s.nblk * 2 * kVerletRunMax);
       A(s.bh_off, (size_t)s.nblk * (kVerletRunMax + 1)); A(s.bh_max, 1);
@@ -263,8 +263,8 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   launch_rehome(s, k_xy_live);  // slot k becomes home k: tiles, cells and start[] are now by home
   HIP_TRY(hipMemcpyAsync(s.xb, s.t.x, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
   HIP_TRY(hipMemcpyAsync(s.yb, s.t.y, sizeof(double) * (size_t)s.n, hipMemcpyDeviceToDevice, s.stream));
-  // the LDS layout (FP32 form): every pass workgroup's halo, and the largest
-  // (the count pass also reads its candidates from the halos that fit its LDS)
+  // the LDS layout: every pass workgroup's halo, and the largest (the FP32
+  // count pass also reads its candidates from the halos that fit its LDS)
   const bool try_lds = s.vlds_allowed && s.bh_max && verlet_lds_unbased(s.P.system);
   if (try_lds) {
     HIP_TRY(hipMemsetAsync(s.bh_max, 0, sizeof(int), s.stream));
@@ -295,8 +295,9 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
     HIP_TRY(hipMemcpyAsync(&halo, s.bh_max, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   }
   HIP_TRY(hipStreamSynchronize(s.stream));
-  // pass 2 stages 20 B per halo particle, plus one sentinel tile after the halo
-  const int cap_limit = kLdsBytes / 20;
+  // pass 2 stages 20 B per halo particle in the FP32 form (32 B, 40 B in the
+  // FP64 forms), plus one sentinel tile after the halo
+  const int cap_limit = kLdsBytes / verlet_lds_tile_bytes(s.prec);
   s.vlds = try_lds && halo >= 0 && halo < cap_limit;  // halo is INT_MAX when a workgroup cannot be staged
   s.vlds_cap = s.vlds ? std::min(cap_limit, std::max(64, (halo + 1 + 63) / 64 * 64)) : 0;
   s.vsorted = s.vlds && want_sort;
@@ -318,6 +319,22 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
     const size_t cap = need + need / 8 + 64 * 256;
     HIP_TRY(dalloc(&s.vlist, cap));
     s.vlist_cap = cap;
+  }
+  s.vrho_fresh = false;
+  if (s.vlds && s.prec == P32K && s.P.system == SPC2_CHIMERA && s.vrho_allowed) {
+    const size_t bytes = need / 4;  // a byte per lane and group
+    if (bytes > s.vrho_cap || !s.vrho) {
+      if (s.vrho) (void)hipFree(s.vrho);
+      s.vrho = nullptr;
+      s.vrho_cap = 0;
+      const size_t cap = bytes + bytes / 8 + 64 * 64;
+      HIP_TRY(dalloc(&s.vrho, cap));
+      s.vrho_cap = cap;
+    }
+  } else if (s.vrho) {
+    (void)hipFree(s.vrho);
+    s.vrho = nullptr;
+    s.vrho_cap = 0;
   }
   if (s.vlds)
     launch_verlet_fill_lds(s);
@@ -599,6 +616,7 @@ spc2_status spc2_create(const spc2_params *p, spc2_sim_t *out) {
   if (const char *env = std::getenv("SPC2_VERLET_LDS")) s.vlds_allowed = std::atoi(env) != 0;
   if (const char *env = std::getenv("SPC2_VERLET_SORT")) s.vsort_mode = std::atoi(env) != 0 ? 1 : 0;
   if (const char *env = std::getenv("SPC2_RK4_STORE_KXY")) s.rk4_store_kxy = std::atoi(env) != 0;
+  if (const char *env = std::getenv("SPC2_VERLET_RHO_BITS")) s.vrho_allowed = std::atoi(env) != 0;
   hipError_t e = hipGetDevice(&s.device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {

@@ -152,6 +152,13 @@ struct Sim {
   // layout in use
   int *vslot = nullptr, *wmax_s = nullptr, *wrow_s = nullptr;
   bool vsorted = false;  // the current LDS-layout list uses the lane slots
+  // chimera FP32 LDS layout: pass 1 of an Ito pair stores each entry's
+  // within-rho decision (a byte per lane and group, the list's layout) for the
+  // pass 2 that follows (vrho_fresh: the last pass-1 launch wrote them);
+  // vrho_allowed: SPC2_VERLET_RHO_BITS
+  uint8_t *vrho = nullptr;
+  size_t vrho_cap = 0;
+  bool vrho_fresh = false, vrho_allowed = true;
   int vsort_mode = -1;   // -1: by list length, 0: never, 1: always (SPC2_VERLET_SORT)
   // fused RK4 steps store k_1..k_3's x / y components instead of rebuilding
   // them in the combine (SPC2_RK4_STORE_KXY)
@@ -234,6 +241,7 @@ constexpr int kVerletRunMax = 16;  // halo runs per pass workgroup (kRunMax of s
 void launch_block_halo(Sim &s);
 // the LDS layout's pass kernels of this system start their dynamic LDS at address 0
 bool verlet_lds_unbased(int system);
+int verlet_lds_tile_bytes(int prec);
 void launch_verlet_fill_lds(Sim &s);
 void launch_verlet_order(Sim &s);
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt);

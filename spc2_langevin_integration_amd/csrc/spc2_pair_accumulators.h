@@ -8,8 +8,9 @@
 //   pair(src, m, params)     candidate at tile index m,
 //   finish(params, phase, out).
 // The FP32 forms also take Verlet-list entries: pair_c<MODE>(src, m, fx, fy,
-// tile, aj, params) with the partner's tile loaded and its stencil column and
-// row as floats, and flush() at the end of each list group (SumMode).
+// tile, aj, params) with the partner's tile loaded (its y in the Verlet
+// passes' frame, ColOffsets::frame) and its stencil column and row as floats,
+// and flush() at the end of each list group (SumMode).
 // Compiled with -ffp-contract=off.
 #pragma once
 
@@ -114,6 +115,7 @@ struct ColOffsets {
   float oM, hxf, oy;
   int b1, b2;
   float kx, ky;  // -h - xl, -h - yl: the offsets of stencil column / row d are (d + 1) h + k
+  float yl0, yf;  // the centre's cell-local y, and its y in the Verlet passes' frame
   __device__ __forceinline__ void set(float xl, float h) {
     oM = -xl;
     hxf = h;
@@ -121,7 +123,13 @@ struct ColOffsets {
   __device__ __forceinline__ void set_codes(float xl, float yl, float hx, float hy) {
     kx = -hx - xl;
     ky = -hy - yl;
+    yl0 = yl;
+    yf = yl;
   }
+  // The Verlet passes' y frame: the centre's y is yl + rows h_y (rows in {0, 1},
+  // exact) and a list entry's tile arrives with its y in the same frame, so
+  // that the separation's y is v.y - yf.
+  __device__ __forceinline__ void frame(float rows_hy) { yf = yl0 + rows_hy; }
   // x / y offsets from a list entry's stencil column dcx + 1 and row dcy + 1
   // (each 0, 1 or 2, as floats)
   __device__ __forceinline__ float cdx(float fx, float hx) const { return __builtin_fmaf(fx, hx, kx); }
@@ -273,18 +281,19 @@ struct ChimAcc<P32K> {
     const float4 v = s.f4[m];
     pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
   }
-  // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
+  // a list entry with the partner's tile already loaded (aj: pass-2 input,
+  // unused here); returns whether the partner is within rho
   template <int MODE>
-  __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float,
+  __device__ __forceinline__ bool pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float,
                                          const KParams &P) {
-    pair_d<MODE>(s, m, v, v.x + co.cdx(fx, P.hxf), v.y + co.cdy(fy, P.hyf), P);
+    return pair_d<MODE>(s, m, v, v.x + co.cdx(fx, P.hxf), v.y - co.yf, P);
   }
   __device__ __forceinline__ void flush() {
     S.add(ps);
     C.add(pc);
   }
   template <int MODE = SUM_KAHAN>
-  __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
+  __device__ __forceinline__ bool pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
                                          const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f_hi;  // outside the band, the same as r2 <= rho2f
@@ -297,6 +306,7 @@ struct ChimAcc<P32K> {
       group_add<MODE>(pc, in ? v.z : 0.0f);
     }
     n += in;
+    return in;
   }
   __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
     double sa, ca;
@@ -466,18 +476,21 @@ struct VortAcc<P32K> {
     const float4 v = s.f4[m];
     pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
   }
-  // a list entry with the partner's tile already loaded (aj: pass-2 input, unused here)
-  template <int MODE>
-  __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float,
+  // a list entry with the partner's tile already loaded (aj: pass-2 input,
+  // unused here).  DEFER: the cohesion term is left to the caller, who gets
+  // whether the partner lies within xi_r's band and calls cohesion() later, with
+  // the same separation, in list order.
+  template <int MODE, bool DEFER = false>
+  __device__ __forceinline__ bool pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float,
                                          const KParams &P) {
-    pair_d<MODE>(s, m, v, v.x + co.cdx(fx, P.hxf), v.y + co.cdy(fy, P.hyf), P);
+    return pair_d<MODE, DEFER>(s, m, v, v.x + co.cdx(fx, P.hxf), v.y - co.yf, P);
   }
   __device__ __forceinline__ void flush() {
     SW.add(psw);
     CW.add(pcw);
   }
-  template <int MODE = SUM_KAHAN>
-  __device__ __forceinline__ void pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
+  template <int MODE = SUM_KAHAN, bool DEFER = false>
+  __device__ __forceinline__ bool pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
                                          const KParams &P) {
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     // Outside the rho band the FP32 test decides (r2 <= rho2 + band there means
@@ -499,28 +512,35 @@ struct VortAcc<P32K> {
       group_add_prod<MODE>(pcw, w, v.z);
     }
     n01 += inr;
-    if (r2 <= P.xr2f_hi) {  // within xi_r or its band: decided as for rho
-      bool inc = r2 <= P.xr2f;
-      if (r2 >= P.xr2f - P.band_xr2f)
-        inc = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy)) <= P.xi_r;
-      if (inc) {
-        if (r2 >= P.close2f) {
-          float ir = __builtin_amdgcn_rsqf(r2);
-          Tx.add_prod(dx, ir);
-          Ty.add_prod(dy, ir);
+    if constexpr (DEFER) {
+      return r2 <= P.xr2f_hi;
+    } else {
+      if (r2 <= P.xr2f_hi) cohesion(s, m, dx, dy, r2, P);
+      return false;
+    }
+  }
+  // The cohesion term of a partner within xi_r or its band (r2 <= xr2f_hi):
+  // decided as for rho.
+  __device__ __forceinline__ void cohesion(const Src32 &s, int m, float dx, float dy, float r2, const KParams &P) {
+    bool inc = r2 <= P.xr2f;
+    if (r2 >= P.xr2f - P.band_xr2f) inc = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy)) <= P.xi_r;
+    if (inc) {
+      if (r2 >= P.close2f) {
+        float ir = __builtin_amdgcn_rsqf(r2);
+        Tx.add_prod(dx, ir);
+        Ty.add_prod(dy, ir);
+      } else {
+        double ddx, ddy;
+        exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
+        double dr = sqrt(ddx * ddx + ddy * ddy);
+        if (dr > 0.0) {
+          Tx.add((float)(ddx / dr));
+          Ty.add((float)(ddy / dr));
         } else {
-          double ddx, ddy;
-          exact_d(tx, ty, k, s.gslot(m), P, ddx, ddy);
-          double dr = sqrt(ddx * ddx + ddy * ddy);
-          if (dr > 0.0) {
-            Tx.add((float)(ddx / dr));
-            Ty.add((float)(ddy / dr));
-          } else {
-            Tx.add(1.0f);  // atan2(+0, +0) = 0
-          }
+          Tx.add(1.0f);  // atan2(+0, +0) = 0
         }
-        ++n2;
       }
+      ++n2;
     }
   }
   __device__ __forceinline__ void finish(const KParams &P, double phase, Pass1Out &o) const {
@@ -684,11 +704,20 @@ struct Chim2Acc<P32K> {
   template <int MODE>
   __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float ajv,
                                          const KParams &P) {
-    float dx = v.x + co.cdx(fx, P.hxf), dy = v.y + co.cdy(fy, P.hyf);
+    float dx = v.x + co.cdx(fx, P.hxf), dy = v.y - co.yf;
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f_hi;  // outside the band, the same as r2 <= rho2f
     if ((r2 >= P.rho2f_lo) & in) in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
     sum<MODE>(v, in, ajv);  // aj multiplies cos, sin terms that are zero outside rho
+  }
+  // a list entry with pass 1's rho decision: v = (cos, sin, aj, 0) of the partner
+  template <int MODE>
+  __device__ __forceinline__ void pair_m(const float4 &v, bool in) {
+    const float cz = in ? v.x : 0.0f, sw = in ? v.y : 0.0f;
+    group_add<MODE>(pc, cz);
+    group_add<MODE>(ps, sw);
+    group_add_prod<MODE>(ppc, v.z, cz);
+    group_add_prod<MODE>(pqs, v.z, sw);
   }
   __device__ __forceinline__ void flush() {
     C.add(pc);
@@ -900,7 +929,7 @@ struct Vort2Acc<P32K> {
   template <int MODE>
   __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float ajv,
                                          const KParams &P) {
-    pair_t(s, m, v, v.x + co.cdx(fx, P.hxf), v.y + co.cdy(fy, P.hyf), ajv, true, P);
+    pair_t(s, m, v, v.x + co.cdx(fx, P.hxf), v.y - co.yf, ajv, true, P);
   }
   __device__ __forceinline__ void flush() {}
   __device__ __forceinline__ void pair_t(const Src32 &s, int m, const float4 &v, float dx, float dy, float ajv,

@@ -18,10 +18,11 @@
 // Per RHS a stage kernel writes the tiles by home (FP64 wrapped stage position
 // for the cutoff decisions, and the precision's own tile), then one pass kernel
 // per RHS pass runs the same accumulators and epilogues as the cell-stencil
-// kernels, a thread per particle gathering its neighbours' tiles (they are
-// spatially close in home order, so the gathers are L1/L2 hits; staging a
-// segment's halo in LDS instead measured 1.4x slower on gfx950).  Compiled with
-// -ffp-contract=off.
+// kernels, a thread per particle.  The passes stage each workgroup's halo of
+// tiles in LDS and walk lists of 16-bit halo indices (all three precision forms,
+// below); a build whose halo does not fit keeps the gather layout, whose passes
+// read the neighbours' tiles from memory (spatially close in home order, so L1 /
+// L2 hits).  Compiled with -ffp-contract=off.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -30,6 +31,7 @@
 
 #include "spc2_device.h"
 #include "spc2_internal.h"
+#include "spc2_lds_stencil.h"
 #include "spc2_pair_accumulators.h"
 
 namespace spc2 {
@@ -301,7 +303,9 @@ __global__ __launch_bounds__(64) void k_block_halo(KParams P, const int *start, 
   const int y0 = c0 / nx, y1 = c1 / nx;
   int *run = bh_run + (size_t)b * 2 * kRunMax;  // run[2r], run[2r + 1]: first and last cell
   int nrun = 0;
-  bool ok = y1 - y0 <= 1;
+  // at least four build rows, so that the halo rows y0 - 1 .. y1 + 1 are
+  // distinct and each has one row offset from y0 (the staged tiles' y frame)
+  bool ok = y1 - y0 <= 1 && ny >= 4;
   // column intervals of halo row r: the workgroup's intervals in rows r - 1 .. r + 1
   // widened by one column (and wrapped), emitted as runs in increasing column order
   for (int r = y0 - 1; ok && r <= y1 + 1; ++r) {
@@ -344,8 +348,8 @@ __global__ __launch_bounds__(64) void k_block_halo(KParams P, const int *start, 
     atomicMax(bh_max, INT_MAX);
     return;
   }
-  // sort the runs by first cell, then merge overlapping or adjacent ones (wrapped
-  // rows and columns may repeat cells)
+  // sort the runs by first cell, then merge overlapping or adjacent ones of one
+  // row (wrapped columns may repeat cells); a run stays within one row
   for (int k = 1; k < nrun; ++k) {
     const int f = run[2 * k], l = run[2 * k + 1];
     int j = k - 1;
@@ -359,7 +363,7 @@ __global__ __launch_bounds__(64) void k_block_halo(KParams P, const int *start, 
   }
   int u = 0;
   for (int k = 0; k < nrun; ++k) {
-    if (u > 0 && run[2 * k] <= run[2 * u - 1] + 1) {
+    if (u > 0 && run[2 * k] <= run[2 * u - 1] + 1 && run[2 * k] / nx == run[2 * u - 1] / nx) {
       run[2 * u - 1] = max(run[2 * u - 1], run[2 * k + 1]);
     } else {
       run[2 * u] = run[2 * k];
@@ -449,13 +453,18 @@ struct LdsTail {
   int off[kRunMax + 1];  // LDS index of each run's first tile; off[n] = halo size
   int n;                 // run count
   long long part[3][kBlock / 64];
+  float ryf[kRunMax];  // each run's y frame offset (row - r0) * h_y (exact)
 };
 static_assert(sizeof(LdsTail) % 8 == 0, "LdsTail keeps the 8-byte alignment of its offset");
 constexpr size_t lds_pass_bytes(int cap, int pass) { return (size_t)cap * (pass == 2 ? 20 : 16) + sizeof(LdsTail); }
+// pass 1 of the vortex system: the cohesion queue (walk_lds) follows the tail
+constexpr int kCohQueue = 4;
+constexpr size_t kCohQueueBytes = sizeof(uint32_t) * kBlock * kCohQueue;
 
 // Build pass 2 for the LDS layout (FP32 form), per lane slot: as k_verlet_fill,
 // with EntLds entries (the partner's index in the workgroup's staged halo in
 // place of its home), eight per group, and the halo's sentinel tile as padding.
+template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int *start, Tiles t, double rlist2,
                                                             const uint32_t *mask, int mwords, const int *vslot,
                                                             const int *wrow, const int *bh_nrun, const int *bh_run,
@@ -509,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_fill_lds(KParams P, const int
   }
   if (total > 32 * mwords) {  // the count pass kept no bits for this home: test again
     int last = -1, base = 0;
-    for_each_build_candidate<P32K>(P, start, t, i, rlist2, [&](int j, int code, bool in) {
+    for_each_build_candidate<PREC>(P, start, t, i, rlist2, [&](int j, int code, bool in) {
       if (!in) return;
       if (code != last) {
         const int jc = adjust_cell(cy + (code >> 2) - 1, P.ny) * P.nx + adjust_cell(cx + (code & 3) - 1, P.nx);
@@ -604,6 +613,19 @@ __device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, con
   acc.pair(src, j, P);
 }
 
+// The FP32 passes' y frame.  The homes of a pass workgroup (kBlock consecutive
+// homes) lie in build rows r0 and r0 + 1, r0 the row of its first home.  A
+// tile's y enters the walk as its cell-local y plus (row - r0) h_y, exact for
+// row - r0 in {-1, 0, 1, 2} (staged once per halo run in the LDS layout, formed
+// per entry from the stencil row in the gather layout, the same value), and the
+// home's y as yl + (row_i - r0) h_y, so that a separation's y is one
+// difference.  Returns row_i - r0; a home whose workgroup spans more rows
+// (gather layout only) takes its own row as r0.
+__device__ __forceinline__ int yframe_rows(const KParams &P, const int *cell, int i) {
+  const int ri = cell[i] / P.nx - cell[i & ~(kBlock - 1)] / P.nx;
+  return ri > 1 ? 0 : ri;
+}
+
 // FP32 list walk over a gather list, a group of four entries in flight: the
 // next group's entries and the current group's partner tiles (and pass-2
 // derivatives) are requested before the group is summed, in list order, so one
@@ -611,7 +633,8 @@ __device__ __forceinline__ void visit_entry(Acc &acc, const S &src, int ent, con
 // accumulator's partials and flushed into its compensated sums, as walk_lds
 // does, so that both layouts give the same sums.
 template <int PASS, class Acc>
-__device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P) {
+__device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P,
+                                       float ysh) {
   using F = EntGather;
   if (cnt <= 0) return;
   const char *f4b = (const char *)src.f4;
@@ -630,6 +653,9 @@ __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l
         v[u] = F::tile(f4b, off);
         a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)  // into the frame: + (row_i - r0 + dcy) h_y, ysh = (row_i - r0 - 1) h_y
+        v[u].y = v[u].y + __builtin_fmaf(F::fy(e[u]), P.hyf, ysh);
       l += 64;
       g = *l;  // the next group (a spare group follows the list)
 #pragma unroll
@@ -648,17 +674,41 @@ __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l
 // halves of four whose staged tiles (and pass-2 derivatives) are requested
 // before the half is summed; the lane's next group of entries is requested
 // before the current group is summed.  Sentinel padding is summed like any entry (it adds
-// zero), and each group's partials are flushed once.
-template <int PASS, class Acc>
-__device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P) {
+// zero), and each group's partials are flushed once.  RHO_BITS (chimera pass 1):
+// the group's eight within-rho decisions are stored as one byte, bit k for
+// entry k, at mb (the list's interleaved layout, a byte per lane and group).
+// COH_QUEUE (vortex pass 1): entries within xi_r's band, a few per particle, are
+// queued in the lane's LDS column cq (kCohQueue slots, stride kBlock) and their
+// cohesion terms summed after the walk, in list order (a full queue is emptied
+// first), so that the divergent cohesion branch runs once per wavefront rather
+// than at every entry where one lane needs it.
+
+template <class Acc>
+__device__ __forceinline__ void coh_drain(Acc &acc, const Src32 &src, const uint32_t *cq, int qn,
+                                          const KParams &P) {
+  using F = EntLds;
+  for (int j = 0; j < qn; ++j) {
+    const uint32_t e = cq[j * kBlock];
+    const float4 v = F::tile(F::off(e));
+    // the separation and r^2 exactly as pair_c forms them
+    const float dx = v.x + acc.co.cdx(F::fx(e), P.hxf), dy = v.y - acc.co.yf;
+    acc.cohesion(src, F::index(e), dx, dy, __builtin_fmaf(dx, dx, dy * dy), P);
+  }
+}
+
+template <int PASS, bool RHO_BITS = false, bool COH_QUEUE = false, class Acc>
+__device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P,
+                                         uint8_t *mb = nullptr, uint32_t *cq = nullptr) {
   using F = EntLds;
   if (cnt <= 0) return;
   const char *ajb = (const char *)src.aj;
+  int qn = 0;
   int4 g = *l;
   for (int q = 0; q < cnt; q += 8) {
     const uint32_t w[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
     l += 64;
     if (q + 8 < cnt) g = *l;  // the lane's next group, if it has one
+    uint32_t bits = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t e[4] = {w[2 * h] & 0xffffu, w[2 * h] >> 16, w[2 * h + 1] & 0xffffu, w[2 * h + 1] >> 16};
@@ -672,10 +722,78 @@ __device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (h == 0 && u == 0)
+        if constexpr (RHO_BITS) {
+          const bool in =
+              h == 0 && u == 0
+                  ? acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P)
+                  : acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+          bits |= (uint32_t)in << (4 * h + u);
+        } else if constexpr (COH_QUEUE) {
+          const bool near =
+              h == 0 && u == 0
+                  ? acc.template pair_c<SUM_GROUP_FIRST, true>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u],
+                                                               a[u], P)
+                  : acc.template pair_c<SUM_GROUP, true>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u],
+                                                         P);
+          if (near) {
+            if (qn == kCohQueue) {
+              coh_drain(acc, src, cq, qn, P);
+              qn = 0;
+            }
+            // the entry from an opaque copy of its word, so that the common path
+            // keeps its one-instruction decode of the high half
+            uint32_t wq = w[2 * h + (u >> 1)];
+            asm volatile("" : "+v"(wq));
+            cq[qn * kBlock] = (u & 1) ? wq >> 16 : wq & 0xffffu;
+            ++qn;
+          }
+        } else if (h == 0 && u == 0) {
           acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
-        else
+        } else {
           acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+        }
+      }
+    }
+    acc.flush();
+    if constexpr (RHO_BITS) {
+      *mb = (uint8_t)bits;
+      mb += 64;
+    }
+  }
+  if constexpr (COH_QUEUE) coh_drain(acc, src, cq, qn, P);
+}
+
+// Chimera pass-2 walk with pass 1's within-rho decisions (one byte per group at
+// mb): the staged tiles hold (cos, sin, aj, 0) of each halo particle, and no
+// separation is formed.  Same terms, order and groups as walk_lds<2>.
+template <class Acc>
+__device__ __forceinline__ void walk_lds_rho_bits(Acc &acc, const int4 *l, const uint8_t *mb, int cnt) {
+  using F = EntLds;
+  if (cnt <= 0) return;
+  int4 g = *l;
+  uint32_t bits = *mb;
+  for (int q = 0; q < cnt; q += 8) {
+    const uint32_t w[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
+    const uint32_t b = bits;
+    l += 64;
+    mb += 64;
+    if (q + 8 < cnt) {  // the lane's next group, if it has one
+      g = *l;
+      bits = *mb;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t e[4] = {w[2 * h] & 0xffffu, w[2 * h] >> 16, w[2 * h + 1] & 0xffffu, w[2 * h + 1] >> 16};
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = F::tile(F::off(e[u]));
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool in = (b >> (4 * h + u)) & 1u;
+        if (h == 0 && u == 0)
+          acc.template pair_m<SUM_GROUP_FIRST>(v[u], in);
+        else
+          acc.template pair_m<SUM_GROUP>(v[u], in);
       }
     }
     acc.flush();
@@ -724,10 +842,13 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet(KParams P, const int *_
     acc.init(src, i, i, t, P);
     cnt = vcnt[i];
     const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
-    if constexpr (PREC == P32K)
-      walk32<1>(acc, src, l, cnt, P);
-    else
+    if constexpr (PREC == P32K) {
+      const int ri = yframe_rows(P, t.cell, i);
+      acc.co.frame((float)ri * P.hyf);
+      walk32<1>(acc, src, l, cnt, P, (float)(ri - 1) * P.hyf);
+    } else {
       walk64(acc, src, l, cnt, P);
+    }
     acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
     const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
     if (EPI == EPI_K && e.next == 1) {
@@ -758,10 +879,13 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet(KParams P, const int *_
   acc.init(src, i, i, t, P, in);
   const int cnt = vcnt[i];
   const int4 *l = (const int4 *)vlist + (size_t)wrow[i >> 6] * 64 + (i & 63);
-  if constexpr (PREC == P32K)
-    walk32<2>(acc, src, l, cnt, P);
-  else
+  if constexpr (PREC == P32K) {
+    const int ri = yframe_rows(P, t.cell, i);
+    acc.co.frame((float)ri * P.hyf);
+    walk32<2>(acc, src, l, cnt, P, (float)(ri - 1) * P.hyf);
+  } else {
     walk64(acc, src, l, cnt, P);
+  }
   Pass2Out o;
   acc.finish(P, PREC == P32K ? gp[i] : 0.0, o);
   const EpiState r = epilogue2<EPI>(P, e, i, hid[i], o);
@@ -784,7 +908,14 @@ __device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, 
   if (threadIdx.x <= kRunMax) {
     const int k = threadIdx.x, nrun = bh_nrun[seg];
     if (k <= nrun) tl->off[k] = bh_off[(size_t)seg * (kRunMax + 1) + k];
-    if (k < nrun) tl->gst[k] = start[bh_run[(size_t)seg * 2 * kRunMax + 2 * k]];
+    if (k < nrun) {
+      const int c = bh_run[(size_t)seg * 2 * kRunMax + 2 * k];  // the run's first cell
+      tl->gst[k] = start[c];
+      int d = c / P.nx - t.cell[seg * kBlock] / P.nx;  // its row less r0, unwrapped to -1 .. 2
+      if (d > 2) d -= P.ny;
+      else if (d < -1) d += P.ny;
+      tl->ryf[k] = (float)d * P.hyf;
+    }
   }
   __syncthreads();
   const int nrun = tl->n, total = tl->off[nrun];
@@ -792,7 +923,9 @@ __device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, 
     int k = 0;
     while (k + 1 < nrun && tl->off[k + 1] <= el) ++k;
     const int g = tl->gst[k] + (el - tl->off[k]);
-    f4l[el] = t.f4[g];
+    float4 v = t.f4[g];
+    v.y = v.y + tl->ryf[k];  // into the workgroup's y frame
+    f4l[el] = v;
     if (PASS == 2) ajl[el] = e.aphi_slot_f[g];
   }
   if (threadIdx.x == 0) {  // the sentinel tile that padding entries name (total < cap)
@@ -803,15 +936,15 @@ __device__ __forceinline__ Src32 stage_halo(const KParams &P, const int *start, 
   return Src32{f4l, nullptr, PASS == 2 ? ajl : nullptr, tl->gst, tl->off, nrun};
 }
 
-template <class Acc, int EPI>
-__global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const int *__restrict__ vcnt,
+template <class Acc, int EPI, bool RHO_BITS = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_pass1_verlet_lds(KParams P, const int *__restrict__ vcnt,
                                                              const int *__restrict__ vslot,
                                                              const int *__restrict__ wrow,
                                                              const int *__restrict__ vlist, const int *start,
                                                              const int *bh_nrun, const int *bh_run,
                                                              const int *bh_off, Tiles t, const int *__restrict__ hid,
                                                              const double *gp, EpiCtx e,
-                                                             unsigned long long *counts) {
+                                                             unsigned long long *counts, uint8_t *rho_bits) {
   extern __shared__ float4 dyn_lds[];
   const int seg = xcd_segment(blockIdx.x, gridDim.x);
   LdsTail *tl;
@@ -824,8 +957,13 @@ __global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds(KParams P, const in
     const int i = vslot ? vslot[slot] : slot;
     Acc acc;
     acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P);
+    acc.co.frame((float)yframe_rows(P, t.cell, i) * P.hyf);
     cnt = vcnt[i];
-    walk_lds<1>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), cnt, P);
+    const size_t at = (size_t)wrow[slot >> 6] * 64 + (slot & 63);
+    constexpr bool coh = std::is_same<Acc, VortAcc<P32K>>::value;
+    uint32_t *cq = (uint32_t *)(tl + 1) + threadIdx.x;  // the lane's cohesion queue column (vortex)
+    walk_lds<1, RHO_BITS, coh>(acc, src, (const int4 *)vlist + at, cnt, P, RHO_BITS ? rho_bits + at : nullptr,
+                               cq);
     acc.finish(P, gp[i], o);
     const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
     if (EPI == EPI_K && e.next == 1) {
@@ -860,11 +998,182 @@ __global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds(KParams P, const in
   }
   Acc acc;
   acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P, in);
+  acc.co.frame((float)yframe_rows(P, t.cell, i) * P.hyf);
   walk_lds<2>(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
   Pass2Out o;
   acc.finish(P, gp[i], o);
   const EpiState r = epilogue2<EPI>(P, e, i, hid[i], o);
   if (EPI == EPI_SRK && e.next == 2) write_stage<P32K>(P, e.tn, e.gpn, i, r.x, r.y, r.p);  // the next step's stage
+}
+
+// Stages workgroup seg's halo for the chimera pass 2 with pass 1's rho bits:
+// (cos, sin, pass-1 phi derivative, 0) per halo particle, and a zero tile for
+// the padding entries.
+__device__ __forceinline__ void stage_halo_cs(const int *start, const int *bh_nrun, const int *bh_run,
+                                              const int *bh_off, int seg, const Tiles &t, const EpiCtx &e,
+                                              void *dyn, int cap, LdsTail **tail) {
+  float4 *f4l = (float4 *)dyn;
+  LdsTail *tl = (LdsTail *)(f4l + cap);
+  *tail = tl;
+  if (threadIdx.x == 0) tl->n = bh_nrun[seg];
+  if (threadIdx.x <= kRunMax) {
+    const int k = threadIdx.x, nrun = bh_nrun[seg];
+    if (k <= nrun) tl->off[k] = bh_off[(size_t)seg * (kRunMax + 1) + k];
+    if (k < nrun) tl->gst[k] = start[bh_run[(size_t)seg * 2 * kRunMax + 2 * k]];
+  }
+  __syncthreads();
+  const int nrun = tl->n, total = tl->off[nrun];
+  for (int el = threadIdx.x; el < total; el += kBlock) {
+    int k = 0;
+    while (k + 1 < nrun && tl->off[k + 1] <= el) ++k;
+    const int g = tl->gst[k] + (el - tl->off[k]);
+    const float4 v = t.f4[g];
+    f4l[el] = make_float4(v.z, v.w, e.aphi_slot_f[g], 0.0f);
+  }
+  if (threadIdx.x == 0) f4l[total] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __syncthreads();
+}
+
+template <int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds_rho_bits(
+    KParams P, const int *__restrict__ vcnt, const int *__restrict__ vslot, const int *__restrict__ wrow,
+    const int *__restrict__ vlist, const uint8_t *__restrict__ rho_bits, const int *start, const int *bh_nrun,
+    const int *bh_run, const int *bh_off, Tiles t, const int *__restrict__ hid, const double *gp, EpiCtx e) {
+  extern __shared__ float4 dyn_lds[];
+  const int seg = xcd_segment(blockIdx.x, gridDim.x);
+  LdsTail *tl;
+  stage_halo_cs(start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds, P.lds_cap, &tl);
+  const int slot = seg * kBlock + threadIdx.x;
+  if (slot >= P.n) return;
+  const int i = vslot ? vslot[slot] : slot;
+  Pass2In in{e.aphi_slot[i], e.n_slot[i], e.n2_slot[i], 0.0, 0.0};
+  if (EPI == EPI_ITO2) {
+    in.in0 = e.a0[i];
+    in.in1 = e.a1[i];
+  }
+  Chim2Acc<P32K> acc;
+  acc.init(Src32{t.f4, nullptr, nullptr}, i, i, t, P, in);
+  const size_t at = (size_t)wrow[slot >> 6] * 64 + (slot & 63);
+  walk_lds_rho_bits(acc, (const int4 *)vlist + at, rho_bits + at, vcnt[i]);
+  Pass2Out o;
+  acc.finish(P, gp[i], o);
+  const EpiState r = epilogue2<EPI>(P, e, i, hid[i], o);
+  if (EPI == EPI_SRK && e.next == 2) write_stage<P32K>(P, e.tn, e.gpn, i, r.x, r.y, r.p);  // the next step's stage
+}
+
+// ---- LDS-staged passes (FP64 forms) ---------------------------------------------
+// The FP64 forms walk the LDS layout too: the same 16-bit halo indices, with the
+// workgroup's halo staged as FP64 arrays (LdsTile: wrapped x, y and the phase,
+// or cos and sin for the factored form; for pass 2 the pass-1 derivatives).
+// Their pair arithmetic and order are those of the gather passes, so the sums
+// are the same bit for bit; the list's padding entries are never read (the walk
+// stops at the count).
+
+template <int PREC, int PASS>
+constexpr size_t lds_pass64_bytes(int cap) {
+  return (size_t)cap * LdsTile<PREC, PASS>::bytes + sizeof(LdsTail);
+}
+
+template <int PREC, int PASS>
+__device__ __forceinline__ Src64 stage_halo64(const int *start, const int *bh_nrun, const int *bh_run,
+                                              const int *bh_off, int seg, const Tiles &t, const EpiCtx &e,
+                                              void *dyn, int cap, LdsTail **tail) {
+  LdsTile<PREC, PASS> lt(dyn, cap);
+  LdsTail *tl = (LdsTail *)((char *)dyn + (size_t)cap * LdsTile<PREC, PASS>::bytes);
+  *tail = tl;
+  if (threadIdx.x == 0) tl->n = bh_nrun[seg];
+  if (threadIdx.x <= kRunMax) {
+    const int k = threadIdx.x, nrun = bh_nrun[seg];
+    if (k <= nrun) tl->off[k] = bh_off[(size_t)seg * (kRunMax + 1) + k];
+    if (k < nrun) tl->gst[k] = start[bh_run[(size_t)seg * 2 * kRunMax + 2 * k]];
+  }
+  __syncthreads();
+  const int nrun = tl->n, total = tl->off[nrun];
+  for (int el = threadIdx.x; el < total; el += kBlock) {
+    int k = 0;
+    while (k + 1 < nrun && tl->off[k + 1] <= el) ++k;
+    lt.load(el, tl->gst[k] + (el - tl->off[k]), t, e);
+  }
+  __syncthreads();
+  return lt.src();
+}
+
+// FP64 walk over an LDS list: the entries below the count, in list order.
+template <class Acc>
+__device__ __forceinline__ void walk64_lds(Acc &acc, const Src64 &src, const int4 *l, int cnt, const KParams &P) {
+  for (int q = 0; q < cnt; q += 8) {
+    const int4 g = l[(size_t)(q >> 3) * 64];
+    const uint32_t w[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q + u < cnt) acc.pair(src, EntLds::index((u & 1) ? w[u >> 1] >> 16 : w[u >> 1] & 0xffffu), P);
+  }
+}
+
+template <class Acc, int PREC, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass1_verlet_lds64(KParams P, const int *__restrict__ vcnt,
+                                                               const int *__restrict__ vslot,
+                                                               const int *__restrict__ wrow,
+                                                               const int *__restrict__ vlist, const int *start,
+                                                               const int *bh_nrun, const int *bh_run,
+                                                               const int *bh_off, Tiles t,
+                                                               const int *__restrict__ hid, EpiCtx e,
+                                                               unsigned long long *counts) {
+  static_assert(PREC != P32K, "the FP32 form has its own LDS passes");
+  extern __shared__ double dyn_lds64[];
+  const int seg = xcd_segment(blockIdx.x, gridDim.x);
+  LdsTail *tl;
+  const Src64 src = stage_halo64<PREC, 1>(start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds64, P.lds_cap, &tl);
+  const int slot = seg * kBlock + threadIdx.x;
+  const bool active = slot < P.n;
+  Pass1Out o{};
+  int cnt = 0;
+  if (active) {
+    const int i = vslot ? vslot[slot] : slot;
+    Acc acc;
+    acc.init(verlet_src<PREC>(t, e, 1), i, i, t, P);
+    cnt = vcnt[i];
+    walk64_lds(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), cnt, P);
+    acc.finish(P, 0.0, o);
+    const EpiState r = epilogue1<EPI>(P, e, i, hid[i], i, o);
+    if (EPI == EPI_K && e.next == 1) {
+      write_stage<PREC>(P, e.tn, e.gpn, i, e.sx[i] + e.kcdt_n * r.ax, e.sy[i] + e.kcdt_n * r.ay,
+                        e.sp[i] + e.kcdt_n * r.ap);
+    } else if ((EPI == EPI_RK4_FINAL || EPI == EPI_SE) && e.next == 2) {
+      write_stage<PREC>(P, e.tn, e.gpn, i, r.x, r.y, r.p);
+    }
+  }
+  if (counts) count_add_lds(tl->part, counts, cnt, active ? o.n : 0, active ? o.n2 : 0);
+}
+
+template <class Acc, int PREC, int EPI>
+__global__ __launch_bounds__(kBlock) void k_pass2_verlet_lds64(KParams P, const int *__restrict__ vcnt,
+                                                               const int *__restrict__ vslot,
+                                                               const int *__restrict__ wrow,
+                                                               const int *__restrict__ vlist, const int *start,
+                                                               const int *bh_nrun, const int *bh_run,
+                                                               const int *bh_off, Tiles t,
+                                                               const int *__restrict__ hid, EpiCtx e) {
+  static_assert(PREC != P32K, "the FP32 form has its own LDS passes");
+  extern __shared__ double dyn_lds64[];
+  const int seg = xcd_segment(blockIdx.x, gridDim.x);
+  LdsTail *tl;
+  const Src64 src = stage_halo64<PREC, 2>(start, bh_nrun, bh_run, bh_off, seg, t, e, dyn_lds64, P.lds_cap, &tl);
+  const int slot = seg * kBlock + threadIdx.x;
+  if (slot >= P.n) return;
+  const int i = vslot ? vslot[slot] : slot;
+  Pass2In in{e.aphi_slot[i], e.n_slot[i], e.n2_slot[i], 0.0, 0.0};
+  if (EPI == EPI_ITO2) {
+    in.in0 = e.a0[i];
+    in.in1 = e.a1[i];
+  }
+  Acc acc;
+  acc.init(verlet_src<PREC>(t, e, 2), i, i, t, P, in);
+  walk64_lds(acc, src, (const int4 *)vlist + (size_t)wrow[slot >> 6] * 64 + (slot & 63), vcnt[i], P);
+  Pass2Out o;
+  acc.finish(P, 0.0, o);
+  const EpiState r = epilogue2<EPI>(P, e, i, hid[i], o);
+  if (EPI == EPI_SRK && e.next == 2) write_stage<PREC>(P, e.tn, e.gpn, i, r.x, r.y, r.p);  // the next step's stage
 }
 
 // Largest squared displacement since the build (min image), two-level and
@@ -908,20 +1217,45 @@ void vpass1_typed(Sim &s, const EpiCtx &e) {
       KParams kp = s.kp;
       kp.lds_cap = s.vlds_cap;
       const size_t sh = lds_pass_bytes(s.vlds_cap, 1);
+      if constexpr (SYS == SPC2_CHIMERA && EPI == EPI_ITO1) {
+        if (s.vrho) {  // the pass 2 that follows reads the rho decisions
+          launch_pair(s, k_pass1_verlet_lds<ChimAcc<P32K>, EPI, true>, grid, blk, sh, kp, (const int *)s.vcnt,
+                      (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist,
+                      (const int *)s.start, (const int *)s.bh_nrun, (const int *)s.bh_run, (const int *)s.bh_off,
+                      s.t, (const int *)s.hid, (const double *)s.gp, e, cnt, s.vrho);
+          s.vrho_fresh = true;
+          return;
+        }
+      }
       if (SYS == SPC2_CHIMERA)
         launch_pair(s, k_pass1_verlet_lds<ChimAcc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
                     (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
                     (const int *)s.bh_nrun,
                     (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
-                    e, cnt);
+                    e, cnt, (uint8_t *)nullptr);
       else
-        launch_pair(s, k_pass1_verlet_lds<VortAcc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+        launch_pair(s, k_pass1_verlet_lds<VortAcc<P32K>, EPI>, grid, blk, sh + kCohQueueBytes, kp, (const int *)s.vcnt,
                     (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist, (const int *)s.start,
                     (const int *)s.bh_nrun,
                     (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp,
-                    e, cnt);
+                    e, cnt, (uint8_t *)nullptr);
       return;
     }
+  } else if (s.vlds) {
+    KParams kp = s.kp;
+    kp.lds_cap = s.vlds_cap;
+    const size_t sh = lds_pass64_bytes<PREC, 1>(s.vlds_cap);
+    if (SYS == SPC2_CHIMERA)
+      launch_pair(s, k_pass1_verlet_lds64<ChimAcc<PREC>, PREC, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                  (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist,
+                  (const int *)s.start, (const int *)s.bh_nrun, (const int *)s.bh_run, (const int *)s.bh_off, s.t,
+                  (const int *)s.hid, e, cnt);
+    else
+      launch_pair(s, k_pass1_verlet_lds64<VortAcc<PREC>, PREC, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                  (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist,
+                  (const int *)s.start, (const int *)s.bh_nrun, (const int *)s.bh_run, (const int *)s.bh_off, s.t,
+                  (const int *)s.hid, e, cnt);
+    return;
   }
   if (SYS == SPC2_CHIMERA)
     launch_pair(s, k_pass1_verlet<ChimAcc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
@@ -958,6 +1292,13 @@ void vpass2_typed(Sim &s, const EpiCtx &e) {
     if (s.vlds) {
       KParams kp = s.kp;
       kp.lds_cap = s.vlds_cap;
+      if (SYS == SPC2_CHIMERA && s.vrho_fresh) {  // pass 1 left its rho decisions
+        launch_pair(s, k_pass2_verlet_lds_rho_bits<EPI>, grid, blk, lds_pass_bytes(s.vlds_cap, 1), kp,
+                    (const int *)s.vcnt, (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow,
+                    (const int *)s.vlist, (const uint8_t *)s.vrho, (const int *)s.start, (const int *)s.bh_nrun,
+                    (const int *)s.bh_run, (const int *)s.bh_off, s.t, (const int *)s.hid, (const double *)s.gp, e);
+        return;
+      }
       const size_t sh = lds_pass_bytes(s.vlds_cap, 2);
       if (SYS == SPC2_CHIMERA)
         launch_pair(s, k_pass2_verlet_lds<Chim2Acc<P32K>, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
@@ -973,6 +1314,21 @@ void vpass2_typed(Sim &s, const EpiCtx &e) {
                     e);
       return;
     }
+  } else if (s.vlds) {
+    KParams kp = s.kp;
+    kp.lds_cap = s.vlds_cap;
+    const size_t sh = lds_pass64_bytes<PREC, 2>(s.vlds_cap);
+    if (SYS == SPC2_CHIMERA)
+      launch_pair(s, k_pass2_verlet_lds64<Chim2Acc<PREC>, PREC, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                  (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist,
+                  (const int *)s.start, (const int *)s.bh_nrun, (const int *)s.bh_run, (const int *)s.bh_off, s.t,
+                  (const int *)s.hid, e);
+    else
+      launch_pair(s, k_pass2_verlet_lds64<Vort2Acc<PREC>, PREC, EPI>, grid, blk, sh, kp, (const int *)s.vcnt,
+                  (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.vlist,
+                  (const int *)s.start, (const int *)s.bh_nrun, (const int *)s.bh_run, (const int *)s.bh_off, s.t,
+                  (const int *)s.hid, e);
+    return;
   }
   if (SYS == SPC2_CHIMERA)
     launch_pair(s, k_pass2_verlet<Chim2Acc<PREC>, PREC, EPI>, grid, blk, 0, s.kp, (const int *)s.vcnt,
@@ -1008,6 +1364,10 @@ bool lds_passes_unbased() {
       reinterpret_cast<const void *>(&k_pass1_verlet_lds<A1, EPI_RK4_FINAL>),
       reinterpret_cast<const void *>(&k_pass2_verlet_lds<A2, EPI_ITO2>),
       reinterpret_cast<const void *>(&k_pass2_verlet_lds<A2, EPI_SRK>),
+      // the chimera passes with rho decisions (harmless duplicates for vortex)
+      reinterpret_cast<const void *>(&k_pass1_verlet_lds<ChimAcc<P32K>, EPI_ITO1, true>),
+      reinterpret_cast<const void *>(&k_pass2_verlet_lds_rho_bits<EPI_ITO2>),
+      reinterpret_cast<const void *>(&k_pass2_verlet_lds_rho_bits<EPI_SRK>),
   };
   for (const void *k : kern) {
     hipFuncAttributes a;
@@ -1074,10 +1434,16 @@ void launch_block_halo(Sim &s) {
 
 void launch_verlet_fill_lds(Sim &s) {
   const double r2 = s.rlist * s.rlist * (1.0 + 1e-12);
-  hipLaunchKernelGGL(k_verlet_fill_lds, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
+  const auto k = s.prec == P64 ? k_verlet_fill_lds<P64> : s.prec == P64F ? k_verlet_fill_lds<P64F> : k_verlet_fill_lds<P32K>;
+  hipLaunchKernelGGL(k, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.kp, (const int *)s.start,
                      s.t, r2, (const uint32_t *)s.vmask, s.vmask_words,
                      (const int *)(s.vsorted ? s.vslot : nullptr), (const int *)s.wrow, (const int *)s.bh_nrun,
                      (const int *)s.bh_run, (const int *)s.bh_off, s.vlist);
+}
+
+int verlet_lds_tile_bytes(int prec) {
+  // a staged halo particle's bytes in the larger (pass-2) form
+  return prec == P64 ? LdsTile<P64, 2>::bytes : prec == P64F ? LdsTile<P64F, 2>::bytes : 20;
 }
 
 void launch_verlet_order(Sim &s) {
@@ -1087,6 +1453,7 @@ void launch_verlet_order(Sim &s) {
 
 void launch_verlet_stage(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt) {
   const dim3 g(blocks_for(s.n)), b(kBlock);
+  s.vrho_fresh = false;  // new tiles: the rho decisions no longer hold
   switch (s.prec) {
     case P64:
       hipLaunchKernelGGL(k_verlet_stage<P64>, g, b, 0, s.stream, s.kp, s.sx, s.sy, s.sp, kx, ky, kp, kcdt, s.t, s.gp);
@@ -1130,6 +1497,7 @@ void swap_tiles(Sim &s) {
 void launch_verlet_pass1(Sim &s, int epi, double *ox, double *oy, double *op, const StepArgs &sa, double *const k1[3],
                          double *const k2[3], double *const k3[3]) {
   EpiCtx e = make_ctx(s, ox, oy, op, sa, k1, k2, k3);
+  s.vrho_fresh = false;  // set again by a launch that writes the rho decisions
   const int next = take_next(s, e);
   switch (s.prec) {
     case P64: vpass1_sys<P64>(s, epi, e); break;

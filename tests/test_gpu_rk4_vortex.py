@@ -48,8 +48,9 @@ PATHS = {
     "verlet-lds": ("verlet", "1", "0"),
     "verlet-lds-sorted": ("verlet", "1", "1"),
 }
-COMBOS = [("fp64", "cells"), ("fp64", "verlet-gather"), ("fp64f", "cells"), ("fp64f", "verlet-gather"),
-          ("fp32k", "cells"), ("fp32k", "verlet-gather"), ("fp32k", "verlet-lds"), ("fp32k", "verlet-lds-sorted")]
+COMBOS = [("fp64", "cells"), ("fp64", "verlet-gather"), ("fp64", "verlet-lds"), ("fp64f", "cells"),
+          ("fp64f", "verlet-gather"), ("fp64f", "verlet-lds"), ("fp32k", "cells"), ("fp32k", "verlet-gather"),
+          ("fp32k", "verlet-lds"), ("fp32k", "verlet-lds-sorted")]
 
 
 def threads():

@@ -330,17 +330,21 @@ def run_layout(params, st, method, dt, steps, monkeypatch, lds, omega=None, sort
             return s.get_state(), s.counts(), s.verlet_stats(), layout
 
 
+@pytest.mark.parametrize("prec", ["fp32k", "fp64", "fp64f"])
 @pytest.mark.parametrize("sort", ["0", "1"])
 @pytest.mark.parametrize("system,method", [("chimera", "srk"), ("vortex", "rk4"), ("vortex", "se")])
-def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method, sort):
-    """The LDS-staged FP32 passes gather the same tiles from each workgroup's
-    staged halo, with the same arithmetic and order as the gather passes,
-    whichever lane takes a particle: trajectories across several rebuilds are
-    bit-identical."""
+def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method, sort, prec):
+    """The LDS-staged passes (FP32 tiles, or the FP64 forms' positions and
+    phases) read the same tiles from each workgroup's staged halo, with the same
+    arithmetic and order as the gather passes, whichever lane takes a particle:
+    trajectories across several rebuilds are bit-identical."""
     n, dt, steps = 30000, 0.01, 24
     L = math.sqrt(n / 4000.0)
     if system == "chimera":
-        rho = 0.1
+        # the factored form stages 40 B per halo particle: at rho = 0.1 (62
+        # particles per build cell) a workgroup spanning two rows overflows its
+        # LDS capacity and the build keeps the gather layout
+        rho = 0.08 if prec == "fp64f" else 0.1
         kw = dict(n=n, L=L, rho=rho, v0=0.2 * rho / 8 / dt, sigma=1.0, alpha=1.45, D=0.02, seed=5)
         om = O.uniform(81, n, -0.5, 0.5)
     else:
@@ -348,7 +352,7 @@ def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method,
         kw = dict(n=n, L=L, rho=rho, D=0.01 if method == "se" else 0.0, seed=6, verlet_skin=0.004, **VORT)
         om = None
     st = O.make_state(9, 0 if system == "chimera" else 1, n, L, rho)
-    p = spc2.params(system, "fp32k", "verlet", **kw)
+    p = spc2.params(system, prec, "verlet", **kw)
     a, ca, sa, la = run_layout(p, st, method, dt, steps, monkeypatch, True, om, sort)
     b, cb, sb, lb = run_layout(p, st, method, dt, steps, monkeypatch, False, om)
     assert la[0] == 1 and la[1] > 0, la
@@ -358,12 +362,13 @@ def test_verlet_lds_passes_equal_gather_passes(gpu, monkeypatch, system, method,
     assert np.array_equal(a, b), np.abs(a - b).max()
 
 
-def test_verlet_lds_vortex_ito_equals_gather(gpu, monkeypatch):
+@pytest.mark.parametrize("prec", ["fp32k", "fp64", "fp64f"])
+def test_verlet_lds_vortex_ito_equals_gather(gpu, monkeypatch, prec):
     """Vortex pass 2 (Ito form) from the staged halo equals the gather pass."""
     n, rho, D = 40000, 0.05, 0.02
     L = math.sqrt(n / 4000.0)
     st = O.make_state(23, 1, n, L, rho)
-    p = spc2.params("vortex", "fp32k", "verlet", n=n, L=L, rho=rho, D=D, **VORT)
+    p = spc2.params("vortex", prec, "verlet", n=n, L=L, rho=rho, D=D, **VORT)
     out = {}
     for lds in (True, False):
         with monkeypatch.context() as m:
@@ -420,3 +425,32 @@ def test_verlet_fresh_handles_build_sane_lists(gpu):
             s.step("rk4", 0.01, 0, 2)
         assert builds == 1
         assert 0.9 * expect < entries / n < 1.1 * expect, (k, entries / n, expect)
+
+
+def test_chimera_pass2_from_rho_bits_equals_recomputed(gpu, monkeypatch):
+    """Chimera pass 2 on the LDS layout reads pass 1's within-rho decisions
+    (a bit per list entry) instead of forming each separation again: SRK
+    trajectories across rebuilds and the Ito evaluation are bit-identical to
+    the pass 2 that recomputes them (SPC2_VERLET_RHO_BITS=0)."""
+    n, rho, dt, steps = 30000, 0.1, 0.01, 24
+    L = math.sqrt(n / 4000.0)
+    kw = dict(n=n, L=L, rho=rho, v0=0.2 * rho / 8 / dt, sigma=1.0, alpha=1.45, D=0.02, seed=5)
+    st = O.make_state(19, 0, n, L, rho)
+    om = O.uniform(82, n, -0.5, 0.5)
+    p = spc2.params("chimera", "fp32k", "verlet", **kw)
+    out = {}
+    for bits in ("1", "0"):
+        with monkeypatch.context() as m:
+            m.setenv("SPC2_VERLET_RHO_BITS", bits)
+            m.delenv("SPC2_VERLET_LDS", raising=False)
+            with spc2.Sim(p) as s:
+                s.set_state(st)
+                s.set_omega(om)
+                ito = s.eval_rhs_ito()
+                assert s.verlet_layout()[0] == 1
+                s.step("srk", dt, 0, steps)
+                out[bits] = (ito, s.get_state(), s.verlet_stats())
+    (ia, sa, va), (ib, sb, vb) = out["1"], out["0"]
+    assert va == vb and va[0] >= 2
+    assert np.array_equal(ia[0], ib[0]) and np.array_equal(ia[1], ib[1])
+    assert np.array_equal(sa, sb), np.abs(sa - sb).max()
