@@ -236,7 +236,8 @@ spc2_status spc2_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
  * created with the same parameters: noise is keyed by the replica's seed, its
  * own step and the particle id within it.  All replicas share one precision and
  * use the reference's neighbour dispatch (linked cells iff rho <= L/4).
- * Method: Stochastic Euler (StochasticEulerStepperPtr.hpp:32-76). */
+ * Methods: Stochastic Euler (StochasticEulerStepperPtr.hpp:32-76) and the
+ * stochastic Runge-Kutta step (StochasticRungeKuttaStepperPtr.hpp:32-97). */
 typedef struct spc2_batch *spc2_batch_t;
 
 /* Device bytes one replica takes in a batch (admission control). */
@@ -259,8 +260,8 @@ spc2_status spc2_batch_get_state(spc2_batch_t b, int32_t replica, double *state)
 spc2_status spc2_batch_set_omega(spc2_batch_t b, int32_t replica, const double *omega);
 
 /* Advances every replica by min(nsteps, limit[r]) steps (limit == NULL: nsteps)
- * with time step dt; replica r's step k uses noise of its own step
- * steps_done[r] + k. */
+ * with time step dt and method SPC2_SE or SPC2_SRK; replica r's step k uses
+ * noise of its own step steps_done[r] + k (SPC2_ERR_UNSUPPORTED: RK4). */
 spc2_status spc2_batch_step(spc2_batch_t b, int32_t method, double dt, int64_t nsteps, const int64_t *limit);
 spc2_status spc2_batch_steps_done(spc2_batch_t b, int64_t *steps);
 spc2_status spc2_batch_order_parameters(spc2_batch_t b, double *R, double *psi);

@@ -274,11 +274,12 @@ spc2_status verlet_build(Sim &s, double *const *kprev, double kcdt, bool k_xy_li
   launch_scan_array(s, s.wmax, s.nw, s.wrow);
   if (spc2_status st = check_launch("verlet count")) return st;
   // lane slots sorted by list length pay off when lists are long enough for the
-  // saved wavefront iterations to outweigh the epilogue's scattered accesses
-  // (measured: 150 entries per particle gain, 40 lose); the mean list length is
-  // estimated from the mean density
+  // saved wavefront iterations and list padding to outweigh the epilogue's
+  // scattered accesses (measured on the vortex RK4 pass at 45 entries per
+  // particle: 5% faster over 500 steps); the mean list length is estimated from
+  // the mean density
   const double mean_list = (double)s.n / (s.P.Lx * s.P.Ly) * 3.14159265358979 * s.rlist * s.rlist;
-  const bool want_sort = try_lds && (s.vsort_mode == 1 || (s.vsort_mode < 0 && mean_list > 72.0));
+  const bool want_sort = try_lds && (s.vsort_mode == 1 || (s.vsort_mode < 0 && mean_list > 32.0));
   int *wrow_l = want_sort ? s.wrow_s : s.wrow8;  // the LDS layout's offsets (groups of eight)
   if (try_lds) {
     if (want_sort) {

@@ -13,8 +13,10 @@
 // replica's own step, particle id within the replica) are those of a single
 // handle, so a replica's trajectory is bit-identical alone or in any batch.
 //
-// Method: the Stochastic Euler step (StochasticEulerStepperPtr.hpp:32-76) of
-// chimera and vortex-arrays replicas.  Compiled with -ffp-contract=off.
+// Methods: the Stochastic Euler step (StochasticEulerStepperPtr.hpp:32-76) and
+// the stochastic Runge-Kutta step (StochasticRungeKuttaStepperPtr.hpp:32-97:
+// pass 1 with the Ito intermediates, pass 2 with the update) of chimera and
+// vortex-arrays replicas.  Compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,7 +51,7 @@ struct Rep {
   uint64_t seed;
   int64_t step;         // the replica's own step counter at the start of the call
   int64_t steps;        // steps the replica takes in the current call
-  double dt, sqdt, sq2D;
+  double dt, sqdt, sq2D, D, h1;  // h1 = dt^1.5 / 2 (the SRK step's delta_Z)
 };
 
 // K1: stage (the state itself for SE) and binning into the replica's cells.
@@ -127,14 +129,17 @@ __device__ __forceinline__ EpiCtx rep_ctx(const EpiCtx &e, const Rep &R, int k) 
   c.sa.seed = R.seed;
   c.sa.step = (uint64_t)(R.step + k);
   c.sa.omega = R.system == SPC2_CHIMERA ? e.sa.omega : nullptr;  // no natural frequency in the vortex drift
+  c.sa.D = R.D;
+  c.sa.h1 = R.h1;
   return c;
 }
 
-// Pair sums + SE update over every row segment of every linked replica.
-template <int PREC>
-__global__ __launch_bounds__(kB) void k_b_pass_se(const Rep *reps, const int *seg_rep, const int *start, Tiles t,
-                                                  const int *slot_h, const int *slot_id, const double *gp, EpiCtx e,
-                                                  int k) {
+// Pass 1 over every row segment of every linked replica: pair sums and the SE
+// update (EPI_SE), or the SRK step's Ito intermediates by slot (EPI_ITO1).
+template <int PREC, int EPI>
+__global__ __launch_bounds__(kB) void k_b_pass1(const Rep *reps, const int *seg_rep, const int *start, Tiles t,
+                                                const int *slot_h, const int *slot_id, const double *gp, EpiCtx e,
+                                                int k) {
   extern __shared__ float4 dyn_lds[];
   const int sg = xcd_segment(blockIdx.x, gridDim.x);
   const Rep &R = reps[seg_rep[sg]];
@@ -142,18 +147,35 @@ __global__ __launch_bounds__(kB) void k_b_pass_se(const Rep *reps, const int *se
   const KParams P = R.kp;
   const EpiCtx c = rep_ctx(e, R, k);
   if (R.system == SPC2_CHIMERA)
-    pass1_segment<ChimAcc<PREC>, PREC, EPI_SE>(P, start + R.coff, t, slot_h, slot_id, gp, c, nullptr, sg - R.soff,
-                                               dyn_lds);
+    pass1_segment<ChimAcc<PREC>, PREC, EPI>(P, start + R.coff, t, slot_h, slot_id, gp, c, nullptr, sg - R.soff,
+                                            dyn_lds);
   else
-    pass1_segment<VortAcc<PREC>, PREC, EPI_SE>(P, start + R.coff, t, slot_h, slot_id, gp, c, nullptr, sg - R.soff,
-                                               dyn_lds);
+    pass1_segment<VortAcc<PREC>, PREC, EPI>(P, start + R.coff, t, slot_h, slot_id, gp, c, nullptr, sg - R.soff,
+                                            dyn_lds);
+}
+
+// Pass 2 of the SRK step over every row segment of every linked replica.
+template <int PREC>
+__global__ __launch_bounds__(kB) void k_b_pass2_srk(const Rep *reps, const int *seg_rep, const int *start, Tiles t,
+                                                    const int *slot_h, const int *slot_id, const double *gp,
+                                                    EpiCtx e, int k) {
+  extern __shared__ float4 dyn_lds[];
+  const int sg = xcd_segment(blockIdx.x, gridDim.x);
+  const Rep &R = reps[seg_rep[sg]];
+  if (k >= R.steps) return;
+  const KParams P = R.kp;
+  const EpiCtx c = rep_ctx(e, R, k);
+  if (R.system == SPC2_CHIMERA)
+    pass2_segment<Chim2Acc<PREC>, PREC, EPI_SRK>(P, start + R.coff, t, slot_h, slot_id, gp, c, sg - R.soff, dyn_lds);
+  else
+    pass2_segment<Vort2Acc<PREC>, PREC, EPI_SRK>(P, start + R.coff, t, slot_h, slot_id, gp, c, sg - R.soff, dyn_lds);
 }
 
 // All-pairs replicas (rho > L/4): thread per particle; their homes are in id
-// order (one cell, sorted by id).
-template <int PREC>
-__global__ __launch_bounds__(kB) void k_b_allpairs_se(const Rep *reps, const int *ap_rep, const double *gx,
-                                                      const double *gy, const double *gp, EpiCtx e, int k) {
+// order (one cell, sorted by id), so a particle's slot is its home.
+template <int PREC, int EPI>
+__global__ __launch_bounds__(kB) void k_b_allpairs1(const Rep *reps, const int *ap_rep, const double *gx,
+                                                    const double *gy, const double *gp, EpiCtx e, int k) {
   const Rep &R = reps[ap_rep[blockIdx.x]];
   if (k >= R.steps) return;
   const int i = (blockIdx.x - R.apoff) * kB + threadIdx.x;
@@ -165,7 +187,26 @@ __global__ __launch_bounds__(kB) void k_b_allpairs_se(const Rep *reps, const int
     allpairs_pass1<PREC, SPC2_CHIMERA>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
   else
     allpairs_pass1<PREC, SPC2_VORTEX>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
-  epilogue1<EPI_SE>(P, c, R.poff + i, i, R.poff + i, o);
+  epilogue1<EPI>(P, c, R.poff + i, i, R.poff + i, o);
+}
+
+__global__ __launch_bounds__(kB) void k_b_allpairs2_srk(const Rep *reps, const int *ap_rep, const double *gx,
+                                                        const double *gy, const double *gp, EpiCtx e, int k) {
+  const Rep &R = reps[ap_rep[blockIdx.x]];
+  if (k >= R.steps) return;
+  const int i = (blockIdx.x - R.apoff) * kB + threadIdx.x;
+  if (i >= R.n) return;
+  const KParams P = R.kp;
+  const EpiCtx c = rep_ctx(e, R, k);
+  const int o0 = R.poff;
+  Pass2Out o;
+  if (R.system == SPC2_CHIMERA)
+    allpairs_pass2<SPC2_CHIMERA>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
+                                 0.0, 0.0, o);
+  else
+    allpairs_pass2<SPC2_VORTEX>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
+                                0.0, 0.0, o);
+  epilogue2<EPI_SRK>(P, c, o0 + i, i, o);
 }
 
 // Re-homing of the SE arrays (x, y, phi, omega): slot k becomes home k.
@@ -295,6 +336,9 @@ struct spc2_batch {
   Tiles t{};
   double *io = nullptr, *red = nullptr;
   size_t io_cap = 0;
+  // the SRK step's Ito intermediates by slot (allocated at the first SRK step)
+  double *aphi_slot = nullptr, *n_slot = nullptr, *n2_slot = nullptr;
+  float *aphi_slot_f = nullptr;
   float *snap = nullptr;
   hipEvent_t snap_ev = nullptr;
   bool snap_pending = false;
@@ -326,6 +370,7 @@ void bfree(spc2_batch &b) {
   f(b.hid); f(b.hpos); f(b.spare_i); f(b.bsum);
   f(b.t.x); f(b.t.y); f(b.t.p); f(b.t.c); f(b.t.s); f(b.t.f4); f(b.t.cell);
   f(b.io); f(b.red); f(b.snap);
+  f(b.aphi_slot); f(b.n_slot); f(b.n2_slot); f(b.aphi_slot_f);
   if (b.snap_ev) (void)hipEventDestroy(b.snap_ev);
   if (b.stream) (void)hipStreamDestroy(b.stream);
 }
@@ -446,7 +491,7 @@ spc2_status ensure_io(spc2_batch &b, size_t count) {
 }
 
 template <int PREC>
-void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k) {
+void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk) {
   const dim3 g(bfor(b.nt)), blk(kB);
   hipLaunchKernelGGL(k_b_stage_bin, g, blk, 0, b.stream, b.nt, (const Rep *)b.d_reps, (const int *)b.rep_of,
                      (const double *)b.sx, (const double *)b.sy, (const double *)b.sp, b.gx, b.gy, b.gp, b.cell_of,
@@ -456,12 +501,35 @@ void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k) {
   hipLaunchKernelGGL(k_b_rank_tiles<PREC>, g, blk, 0, b.stream, b.nt, (const Rep *)b.d_reps, (const int *)b.rep_of,
                      (const int *)b.tmp, (const int *)b.tmpid, (const int *)b.cell_of, (const int *)b.start,
                      (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, b.slot_h, b.slot_id, b.t);
+  if (!srk) {
+    if (b.nseg > 0)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_pass1<PREC, EPI_SE>), dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
+                         (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
+                         (const int *)b.slot_id, (const double *)b.gp, e, k);
+    if (b.nap > 0)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_allpairs1<PREC, EPI_SE>), dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
+                         (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e,
+                         k);
+    return;
+  }
+  // SRK (StochasticRungeKuttaStepperPtr.hpp:32-97): both passes read the same
+  // stage tiles; pass 2 reads every replica's pass-1 values
+  EpiCtx e1 = e;
+  e1.sa.zero_in = 1;  // derivative zero-filled each step
   if (b.nseg > 0)
-    hipLaunchKernelGGL(k_b_pass_se<PREC>, dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_pass1<PREC, EPI_ITO1>), dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
+                       (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
+                       (const int *)b.slot_id, (const double *)b.gp, e1, k);
+  if (b.nap > 0)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_allpairs1<PREC, EPI_ITO1>), dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
+                       (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e1,
+                       k);
+  if (b.nseg > 0)
+    hipLaunchKernelGGL(k_b_pass2_srk<PREC>, dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
                        (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
                        (const int *)b.slot_id, (const double *)b.gp, e, k);
   if (b.nap > 0)
-    hipLaunchKernelGGL(k_b_allpairs_se<PREC>, dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
+    hipLaunchKernelGGL(k_b_allpairs2_srk, dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
                        (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e, k);
 }
 
@@ -476,7 +544,15 @@ void launch_rehome(spc2_batch &b) {
   std::swap(b.hid, b.spare_i);
 }
 
-spc2_status run_steps(spc2_batch &b, double dt, int64_t nsteps, const int64_t *limit) {
+spc2_status run_steps(spc2_batch &b, int method, double dt, int64_t nsteps, const int64_t *limit) {
+  const bool srk = method == SPC2_SRK;
+  if (srk && !b.aphi_slot) {
+    const size_t n = (size_t)b.nt;
+    B_TRY(balloc(&b.aphi_slot, n));
+    B_TRY(balloc(&b.n_slot, n));
+    B_TRY(balloc(&b.n2_slot, n));
+    B_TRY(balloc(&b.aphi_slot_f, n));
+  }
   int64_t kmax = 0;
   for (int r = 0; r < b.count; ++r) {
     Rep &R = b.reps[r];
@@ -485,6 +561,8 @@ spc2_status run_steps(spc2_batch &b, double dt, int64_t nsteps, const int64_t *l
     R.dt = dt;
     R.sqdt = std::sqrt(dt);
     R.sq2D = std::sqrt(2.0 * b.params[r].D);
+    R.D = b.params[r].D;
+    R.h1 = 0.5 * dt * std::sqrt(dt);
     kmax = std::max(kmax, R.steps);
   }
   B_TRY(hipMemcpyAsync(b.d_reps, b.reps.data(), sizeof(Rep) * b.reps.size(), hipMemcpyHostToDevice, b.stream));
@@ -494,15 +572,19 @@ spc2_status run_steps(spc2_batch &b, double dt, int64_t nsteps, const int64_t *l
   e.sp = b.sp;
   e.sa.wrap = 1;
   e.sa.sqrt3 = std::sqrt(3.0);
+  e.aphi_slot = b.aphi_slot;
+  e.aphi_slot_f = b.aphi_slot_f;
+  e.n_slot = b.n_slot;
+  e.n2_slot = b.n2_slot;
   for (int64_t k = 0; k < kmax; ++k) {
     e.sx = b.sx;
     e.sy = b.sy;
     e.sp = b.sp;
     e.sa.omega = b.omega;
     switch (b.prec) {
-      case P64: launch_step_kernels<P64>(b, e, (int)k); break;
-      case P64F: launch_step_kernels<P64F>(b, e, (int)k); break;
-      default: launch_step_kernels<P32K>(b, e, (int)k);
+      case P64: launch_step_kernels<P64>(b, e, (int)k, srk); break;
+      case P64F: launch_step_kernels<P64F>(b, e, (int)k, srk); break;
+      default: launch_step_kernels<P32K>(b, e, (int)k, srk);
     }
     if (++b.since_rehome >= kRehomeEvery) {
       launch_rehome(b);
@@ -538,6 +620,7 @@ uint64_t spc2_batch_footprint(const spc2_params *p) {
   const size_t cells = cell_grid(*p, linked, nx, ny) == SPC2_OK ? (size_t)nx * (size_t)ny : 1;
   const size_t tile = p->precision == SPC2_FP32_KAHAN ? 16 + 16 : (p->precision == SPC2_FP64 ? 24 : 40);
   // 11 doubles (state, omega, stage, spare), 11 ints, tiles; 2 ints per cell
+  // (SE stepping; the first SRK step adds 28 B per particle)
   return n * (11 * 8 + 11 * 4 + tile) + cells * 8 + sizeof(Rep) + 64;
 }
 
@@ -707,9 +790,10 @@ spc2_status spc2_batch_set_omega(spc2_batch_t b, int32_t r, const double *omega)
 
 spc2_status spc2_batch_step(spc2_batch_t b, int32_t method, double dt, int64_t nsteps, const int64_t *limit) {
   if (check_batch(b)) return SPC2_ERR_INVALID;
-  if (method != SPC2_SE) return set_error(SPC2_ERR_UNSUPPORTED, "batched replicas step with Stochastic Euler");
+  if (method != SPC2_SE && method != SPC2_SRK)
+    return set_error(SPC2_ERR_UNSUPPORTED, "batched replicas step with Stochastic Euler or stochastic Runge-Kutta");
   if (!(dt > 0.0) || nsteps < 0) return set_error(SPC2_ERR_INVALID, "dt must be > 0, steps >= 0");
-  return run_steps(*b, dt, nsteps, limit);
+  return run_steps(*b, method, dt, nsteps, limit);
 }
 
 spc2_status spc2_batch_steps_done(spc2_batch_t b, int64_t *steps) {
@@ -801,9 +885,9 @@ spc2_status spc2_batch_autotune(spc2_batch_t b, double dt, int32_t trial_steps, 
     st = set_error(SPC2_ERR_HIP, "event creation failed");
   auto time_trial = [&](double &ms) -> spc2_status {
     if (spc2_status q = upload_layout(*trial)) return q;
-    if (spc2_status q = run_steps(*trial, dt, 1, nullptr)) return q;  // warm-up
+    if (spc2_status q = run_steps(*trial, SPC2_SE, dt, 1, nullptr)) return q;  // warm-up
     B_TRY(hipEventRecord(e0, trial->stream));
-    if (spc2_status q = run_steps(*trial, dt, trial_steps, nullptr)) return q;
+    if (spc2_status q = run_steps(*trial, SPC2_SE, dt, trial_steps, nullptr)) return q;
     B_TRY(hipEventRecord(e1, trial->stream));
     B_TRY(hipEventSynchronize(e1));
     float f = 0.0f;

@@ -1353,5 +1353,71 @@ __device__ __forceinline__ void allpairs_pass1(const KParams &P, int i, const do
   o.phase = pi;
 }
 
+
+// All-pairs pass 2 (rho > L/4) of particle i, arrays by particle id (FP64 form
+// for every precision: the small-N path); in0, in1: the additional values before
+// the call (zero for the SRK step).
+template <int SYS>
+__device__ __forceinline__ void allpairs_pass2(const KParams &P, int i, const double *gx, const double *gy,
+                                               const double *gp, const double *aphi, const double *nslot,
+                                               const double *n2slot, double in0, double in1, Pass2Out &o) {
+  const double xi = gx[i], yi = gy[i], pi = gp[i];
+  const double ai = aphi[i], n = nslot[i];
+  double a0 = in0, a1 = in1;
+  if (SYS == SPC2_VORTEX) {
+    // VortexArraysSystemPtr.cpp:305-353
+    const double n2 = n2slot[i];
+    const double vxi = P.v0 * cos(pi), vyi = P.v0 * sin(pi);
+    for (int j = 0; j < P.n; ++j) {
+      if (j == i) continue;
+      double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+      double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+      double dr = sqrt(dx * dx + dy * dy);
+      const bool al = (dr <= P.rho) && (n > 0.0), co = (dr <= P.xi_r) && (n2 > 0.0);
+      if (!al && !co) continue;
+      const double pj = gp[j];
+      const double vxj = P.v0 * cos(pj), vyj = P.v0 * sin(pj);
+      if (al) {
+        double cph = cos(pj - pi);
+        double sph = sin(pj - pi);
+        double w = vortex_w(P, dr);
+        double dw = vortex_dw(P, dr);
+        a0 += -w * cph / n;
+        a1 += ((vxj - vxi) * dw * dx / dr + (vyj - vyi) * dw * dy / dr) * sph / n + (aphi[j] - ai) * w * cph / n -
+              2.0 * P.D * sph * w / n;
+      }
+      if (co) {
+        double th = atan2(dy, dx);
+        double cpp = cos(th - pi);
+        double spp = sin(th - pi);
+        a0 += P.kappa * cpp / n2;
+        a1 += -P.kappa * ((vxj - vxi) * (-dy) / dr / dr + (vyj - vyi) * dx / dr / dr - ai) * cpp / n2 +
+              P.kappa * P.D * spp / n2;
+      }
+    }
+    o.phase = pi;
+    o.ai = ai;
+    o.a0 = a0 * sqrt(2.0 * P.D);
+    o.a1 = a1;
+    return;
+  }
+  // ChimeraSystem.cpp:223-254
+  for (int j = 0; j < P.n; ++j) {
+    if (j == i) continue;
+    double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+    double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if ((r2 <= P.rho2) && (n != 0)) {
+      double cph = cos(gp[j] - pi - P.alpha);
+      a0 += cph / n;
+      a1 += (aphi[j] - ai) * cph / n;
+    }
+  }
+  o.phase = pi;
+  o.ai = ai;
+  o.a0 = a0 * -sqrt(2.0 * P.D);
+  o.a1 = a1 + -2.0 * P.D * ai;
+}
+
 }  // namespace
 }  // namespace spc2

@@ -63,69 +63,9 @@ __global__ __launch_bounds__(kBlock) void k_pass2_allpairs(KParams P, const doub
                                                            const double *gp, EpiCtx e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  const double xi = gx[i], yi = gy[i], pi = gp[i];
-  const double ai = e.aphi_slot[i], n = e.n_slot[i];
-  double a0 = 0.0, a1 = 0.0;
-  if (EPI == EPI_ITO2) {
-    a0 = e.a0[i];
-    a1 = e.a1[i];
-  }
-  if (SYS == SPC2_VORTEX) {
-    // VortexArraysSystemPtr.cpp:305-353 (FP64 form for every precision)
-    const double n2 = e.n2_slot[i];
-    const double vxi = P.v0 * cos(pi), vyi = P.v0 * sin(pi);
-    for (int j = 0; j < P.n; ++j) {
-      if (j == i) continue;
-      double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
-      double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
-      double dr = sqrt(dx * dx + dy * dy);
-      const bool al = (dr <= P.rho) && (n > 0.0), co = (dr <= P.xi_r) && (n2 > 0.0);
-      if (!al && !co) continue;
-      const double pj = gp[j];
-      const double vxj = P.v0 * cos(pj), vyj = P.v0 * sin(pj);
-      if (al) {
-        double cph = cos(pj - pi);
-        double sph = sin(pj - pi);
-        double w = vortex_w(P, dr);
-        double dw = vortex_dw(P, dr);
-        a0 += -w * cph / n;
-        a1 += ((vxj - vxi) * dw * dx / dr + (vyj - vyi) * dw * dy / dr) * sph / n + (e.aphi_slot[j] - ai) * w * cph / n -
-              2.0 * P.D * sph * w / n;
-      }
-      if (co) {
-        double th = atan2(dy, dx);
-        double cpp = cos(th - pi);
-        double spp = sin(th - pi);
-        a0 += P.kappa * cpp / n2;
-        a1 += -P.kappa * ((vxj - vxi) * (-dy) / dr / dr + (vyj - vyi) * dx / dr / dr - ai) * cpp / n2 +
-              P.kappa * P.D * spp / n2;
-      }
-    }
-    Pass2Out o;
-    o.phase = pi;
-    o.ai = ai;
-    o.a0 = a0 * sqrt(2.0 * P.D);
-    o.a1 = a1;
-    epilogue2<EPI>(P, e, i, i, o);
-    return;
-  }
-  // ChimeraSystem.cpp:223-254 (FP64 form for every precision: small-N path)
-  for (int j = 0; j < P.n; ++j) {
-    if (j == i) continue;
-    double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
-    double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
-    double r2 = dx * dx + dy * dy;
-    if ((r2 <= P.rho2) && (n != 0)) {
-      double cph = cos(gp[j] - pi - P.alpha);
-      a0 += cph / n;
-      a1 += (e.aphi_slot[j] - ai) * cph / n;
-    }
-  }
   Pass2Out o;
-  o.phase = pi;
-  o.ai = ai;
-  o.a0 = a0 * -sqrt(2.0 * P.D);
-  o.a1 = a1 + -2.0 * P.D * ai;
+  allpairs_pass2<SYS>(P, i, gx, gy, gp, e.aphi_slot, e.n_slot, e.n2_slot, EPI == EPI_ITO2 ? e.a0[i] : 0.0,
+                      EPI == EPI_ITO2 ? e.a1[i] : 0.0, o);
   epilogue2<EPI>(P, e, i, i, o);
 }
 

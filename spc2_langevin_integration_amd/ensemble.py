@@ -7,7 +7,8 @@ Sweep (one replica per grid point, 16 x 4 x 8 = 512):
   D 8 points log-spaced in [0.001, 0.1];
   N log-uniform in [500, 32768] (numpy default_rng(42)), density 4000
   (L = sqrt(N / 4000)), rho = 0.1, v0 = 0.01, omega ~ N(0, 0.1);
-  Stochastic Euler, dt = 0.01, FP32-compensated pair sums, seed 42 + replica.
+  Stochastic Euler (or, with --method srk, the stochastic Runge-Kutta step),
+  dt = 0.01, FP32-compensated pair sums, seed 42 + replica.
 Replicas with N < 640 have rho > L/4 and take the all-pairs path, as the
 reference dispatches.
 
@@ -16,7 +17,7 @@ the same number of steps and estimates the 512-replica sequential time by
 interpolating time(N) in log-log; every sampled replica's final state must
 equal the batch's bit for bit.
 
-    python -m spc2_langevin_integration_amd.ensemble [--steps 5000] [--sample 24]
+    python -m spc2_langevin_integration_amd.ensemble [--steps 5000] [--sample 24] [--method se|srk]
 """
 from __future__ import annotations
 
@@ -56,6 +57,7 @@ def main(argv=None):
     ap.add_argument("--sample", type=int, default=24)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--trial-steps", type=int, default=3)
+    ap.add_argument("--method", default="se", choices=["se", "srk"])
     args = ap.parse_args(argv)
     reps = sweep(args.replicas)
     ns = np.array([p.n for p in reps])
@@ -65,9 +67,9 @@ def main(argv=None):
         b.init_random_state(omega_std)
         tuned = None if args.no_tune else b.autotune(args.dt, args.trial_steps)
         targets = b.tile_targets()
-        b.step(args.dt, 1)  # warm-up: the timed leg starts at step 1 for every replica
+        b.step(args.dt, 1, method=args.method)  # warm-up: the timed leg starts at step 1 for every replica
         t0 = time.perf_counter()
-        b.step(args.dt, args.steps)
+        b.step(args.dt, args.steps, method=args.method)
         batch_sec = time.perf_counter() - t0
         order = np.argsort(ns)
         pick = order[np.linspace(0, len(order) - 1, min(args.sample, len(order))).astype(int)]
@@ -80,10 +82,10 @@ def main(argv=None):
         p = reps[int(r)]
         with spc2.Sim(p) as s:
             s.init_random_state(omega_std)
-            s.step("se", args.dt, 0, 1)
+            s.step(args.method, args.dt, 0, 1)
             s.synchronize()
             t0 = time.perf_counter()
-            s.step("se", args.dt, 1, args.steps)
+            s.step(args.method, args.dt, 1, args.steps)
             s.synchronize()
             seq.append((int(p.n), time.perf_counter() - t0))
             mismatches += int(not np.array_equal(s.get_state(), finals[int(r)]))
@@ -95,7 +97,7 @@ def main(argv=None):
     done_times = np.cumsum(est[np.argsort(-ns)])  # sequential order: any; completion times for latency
     line = {
         "metric": "ensemble solves/s, batched vs sequential launches",
-        "replicas": args.replicas, "steps": args.steps, "dt": args.dt, "method": "se", "dtype": "fp32k",
+        "replicas": args.replicas, "steps": args.steps, "dt": args.dt, "method": args.method, "dtype": "fp32k",
         "particles_total": int(ns.sum()), "all_pairs_replicas": int((ns < 640).sum()),
         "batch_seconds": batch_sec, "batch_solves_per_s": args.replicas / batch_sec,
         "batch_particle_steps_per_s": float(ns.sum()) * args.steps / batch_sec,

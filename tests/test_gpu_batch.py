@@ -1,9 +1,10 @@
 """Batched replicas (spc2_batch_*) against the oracle and against single handles.
 
 Every replica's Stochastic Euler step (StochasticEulerStepperPtr.hpp:32-76,
-with the natural frequency added as (omega + a_phi) dt for the chimera) must
-meet the single-step bars against the oracle's step_se with the same Philox
-normals: 1e-12 (FP64) and 1e-5 (FP32-compensated) of the increment scale
+with the natural frequency added as (omega + a_phi) dt for the chimera) and
+stochastic Runge-Kutta step (StochasticRungeKuttaStepperPtr.hpp:32-97) must
+meet the single-step bars against the oracle's step_se / step_srk with the same
+Philox normals: 1e-12 (FP64) and 1e-5 (FP32-compensated) of the increment scale
 (tests/_parity.py), replicas on the linked-cell and on the all-pairs path alike.
 
 A replica's arithmetic, summation order and noise keys do not depend on the
@@ -53,9 +54,10 @@ def oracle_sys(p):
                     xi_r=p.xi_r, kappa=p.kappa, D=p.D, threads=threads)
 
 
-def oracle_steps_match(b, reps, which, dt, steps, omega_std, prec):
+def oracle_steps_match(b, reps, which, dt, steps, omega_std, prec, method="se"):
     """Steps the batch `steps` times, each step from the oracle's states of the
-    replicas in `which`, and checks each replica's step against step_se."""
+    replicas in `which`, and checks each replica's step against step_se (or
+    step_srk, two normals per particle)."""
     cur, om, sysp = {}, {}, {}
     for r in which:
         p = reps[r]
@@ -66,11 +68,16 @@ def oracle_steps_match(b, reps, which, dt, steps, omega_std, prec):
     for k in range(steps):
         for r in which:
             b.set_state(r, cur[r])
-        b.step(dt, 1)
+        b.step(dt, 1, method=method)
         for r in which:
             p = reps[r]
             chim = p.system == spc2.CHIMERA
-            ref = O.step_se(sysp[r], cur[r], dt, O.philox_normals(p.seed, k, 0, p.n, 1), omega=om[r] if chim else None)
+            if method == "srk":
+                ref = O.step_srk(sysp[r], cur[r], dt, O.philox_normals(p.seed, k, 0, p.n, 2),
+                                 omega=om[r] if chim else None)
+            else:
+                ref = O.step_se(sysp[r], cur[r], dt, O.philox_normals(p.seed, k, 0, p.n, 1),
+                                omega=om[r] if chim else None)
             worst = max(worst, assert_state(b.get_state(r), ref, cur[r], p.Lx, TOL[prec],
                                             f"replica {r} ({'chimera' if chim else 'vortex'} N={p.n}) step {k}"))
             cur[r] = ref
@@ -87,6 +94,44 @@ def test_batch_replicas_match_oracle(gpu, prec):
         b.init_random_state(0.1)
         worst = oracle_steps_match(b, reps, range(len(reps)), 0.01, 3, 0.1, prec)
     print(f"batch/{prec}: largest error / bar {worst:.3e}")
+
+
+@pytest.mark.parametrize("prec", ["fp32k", "fp64"])
+def test_batch_srk_replicas_match_oracle(gpu, prec):
+    """Batched SRK steps (StochasticRungeKuttaStepperPtr.hpp:32-97): every
+    replica of the mix, linked and all-pairs, against the oracle's step_srk,
+    three steps."""
+    reps = [replica(sy, n, prec, seed, sg, al, D) for sy, n, seed, sg, al, D in MIX]
+    with spc2.Batch(reps) as b:
+        b.init_random_state(0.1)
+        worst = oracle_steps_match(b, reps, range(len(reps)), 0.01, 3, 0.1, prec, "srk")
+    print(f"batch srk/{prec}: largest error / bar {worst:.3e}")
+
+
+@pytest.mark.parametrize("prec", ["fp32k", "fp64"])
+def test_batch_srk_equals_single_handles(gpu, prec):
+    """A replica's SRK trajectory in the batch, with per-replica step limits
+    and a re-pack, equals the same replica stepped alone, bit for bit."""
+    reps = [replica(sy, n, prec, seed, sg, al, D) for sy, n, seed, sg, al, D in MIX]
+    dt, omega_std = 0.01, 0.1
+    limit = np.array([4, 6, 5, 6, 3, 6, 6, 2], np.int64)
+    with spc2.Batch(reps) as b:
+        b.init_random_state(omega_std)
+        b.step(dt, 6, limit=limit, method="srk")
+        assert np.array_equal(b.steps_done(), limit)
+        got = [b.get_state(r) for r in range(len(reps))]
+        keep = [0, 2, 4, 7]
+        with spc2.Batch([reps[r] for r in keep], prev=b, prev_index=keep) as c:
+            c.step(dt, 2, method="srk")
+            got2 = [c.get_state(q) for q in range(len(keep))]
+    for r, p in enumerate(reps):
+        with spc2.Sim(p) as s:
+            s.init_random_state(omega_std)
+            s.step("srk", dt, 0, int(limit[r]))
+            assert np.array_equal(s.get_state(), got[r]), f"replica {r} (N={p.n})"
+            if r in keep:
+                s.step("srk", dt, int(limit[r]), 2)
+                assert np.array_equal(s.get_state(), got2[keep.index(r)]), f"replica {r} after the re-pack"
 
 
 def test_batch_full_sweep(gpu):
