@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # <VortAcc<P32K>>) on this workload (skin 0.2 rho), from the FETCH_SIZE (x2.000)
 # and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
 # copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 240.2e6
+TRAFFIC_VERLET_FP32K = 250.3e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -338,13 +338,14 @@ def cpu_baseline(w, target_s=12.0, max_steps=40):
             "seconds": sec}
 
 
-# Survey section 8(d) byte convention for the vortex RK4 + Verlet workload:
-# 1,108 B per particle-step (4 x [24 + 12 + 12 + 45.2 x 4 list + 12 + 12] + 96
-# combine) over the step's 4 pair-kernel launches.
-SURVEY_BYTES_PER_PARTICLE_STEP = 1108.0
+# Algorithmic bytes of the vortex RK4 + Verlet workload: 1,108 B per
+# particle-step (4 x [24 + 12 + 12 + 45.2 x 4 list + 12 + 12] + 96 combine) over
+# the step's 4 pair-kernel launches.
+ALG_BYTES_PER_PARTICLE_STEP = 1108.0
 # VALU issue of the same kernel from the SQ counter pass (SQ_INSTS_VALU x 4
-# cycles over the 1024 SIMDs during the launch); profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-VALU_ISSUE_FRAC_VERLET_FP32K = 0.83
+# cycles over the 1024 SIMDs during the launch, stages weighted 3:1);
+# profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
+VALU_ISSUE_FRAC_VERLET_FP32K = 0.72
 
 
 def main():
@@ -425,8 +426,8 @@ def main():
         f" ({w['system']} pair sums, FP32 Kahan)"
     avg_s = prof["avg_ms"] / 1e3
     if headline:
-        # survey 8(d): 1,108 B per particle-step over 4 launches
-        alg_bytes = SURVEY_BYTES_PER_PARTICLE_STEP / 4.0 * w["n"]
+        # 1,108 B per particle-step over 4 launches
+        alg_bytes = ALG_BYTES_PER_PARTICLE_STEP / 4.0 * w["n"]
         roof = {"bound": "hbm", "kernel": kname, "achieved": alg_bytes / avg_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": alg_bytes / avg_s / 1e9 / HBM_PEAK_GBS,
                 "traffic": TRAFFIC_VERLET_FP32K, "traffic_GBs": TRAFFIC_VERLET_FP32K / avg_s / 1e9,

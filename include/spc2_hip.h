@@ -77,7 +77,7 @@ typedef struct spc2_params {
 
 typedef struct spc2_sim *spc2_sim_t;
 
-/* Fills *p with the chimera defaults of the survey configs (v0 = 0.01,
+/* Fills *p with the chimera defaults of the benchmark workloads (v0 = 0.01,
  * sigma = 1, rho = 0.1, alpha = 1.45, D = 0.01, L = 1, N = 1000, seed 42). */
 void spc2_default_params(spc2_params *p);
 
