@@ -391,22 +391,40 @@ __global__ __launch_bounds__(64) void k_block_halo(KParams P, const int *start, 
 // long (see verlet_build).  vslot[slot] = home (null: slot = home);
 // wmax_s[w] = wavefront w's group count (groups of eight), from the length at
 // its first slot.
+// The order is a bitonic sort of the workgroup's 256 keys (0xffff - length) << 8
+// | lane in LDS, ascending: decreasing length, ties by home (lengths clamped to
+// 0xffff for the key; missing homes of the last workgroup sort last).  A
+// wavefront's group count is the maximum of its lanes' lengths, whatever the
+// order.
 __global__ __launch_bounds__(kBlock) void k_verlet_order(int n, const int *vcnt, int *vslot, int *wmax_s) {
-  __shared__ int c[kBlock];
+  static_assert(kBlock == 256, "the key holds the lane in 8 bits");
+  __shared__ uint32_t key[kBlock];
   const int base = blockIdx.x * kBlock, m = min(kBlock, n - base);
   const int me = threadIdx.x;
-  c[me] = me < m ? vcnt[base + me] : 0;
+  key[me] = me < m ? ((uint32_t)(0xffff - min(vcnt[base + me], 0xffff)) << 8) | (uint32_t)me : 0xffffffffu;
   __syncthreads();
-  if (me >= m) return;
-  const int mine = c[me];
-  int rank = 0;
-  for (int k = 0; k < m; ++k) {
-    const int ck = c[k];
-    rank += (ck > mine) | ((ck == mine) & (k < me));
+  for (int k = 2; k <= kBlock; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int p = me ^ j;
+      if (p > me) {
+        const uint32_t a = key[me], b = key[p];
+        if (((me & k) == 0) == (a > b)) {
+          key[me] = b;
+          key[p] = a;
+        }
+      }
+      __syncthreads();
+    }
   }
-  const int slot = base + rank;
-  vslot[slot] = base + me;
-  if ((slot & 63) == 0) wmax_s[slot >> 6] = (mine + 7) >> 3;
+  const uint32_t kv = key[me];  // position me of the order
+  const int home = base + (int)(kv & 0xffu);
+  int len = me < m ? vcnt[home] : 0;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) len = max(len, __shfl_xor(len, d, 64));
+  if (me >= m) return;
+  const int slot = base + me;
+  vslot[slot] = home;
+  if ((slot & 63) == 0) wmax_s[slot >> 6] = (len + 7) >> 3;
 }
 
 // FP32 list entry formats.
