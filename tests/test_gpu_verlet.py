@@ -454,3 +454,62 @@ def test_chimera_pass2_from_rho_bits_equals_recomputed(gpu, monkeypatch):
     assert va == vb and va[0] >= 2
     assert np.array_equal(ia[0], ib[0]) and np.array_equal(ia[1], ib[1])
     assert np.array_equal(sa, sb), np.abs(sa - sb).max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
+@pytest.mark.parametrize("init", [0, 1])
+def test_verlet_chimera_rhs_ito(gpu, prec, init):
+    """The chimera Ito evaluation on a Verlet list (ChimeraSystemPtr.cpp:448-639):
+    pass 1 and pass 2 from the staged halo (the FP32 pass 2 from pass 1's
+    within-rho bits) against the oracle's rhs_ito: the derivative at the
+    per-particle bars, each additional output against its largest reference
+    value, and exact pair counts."""
+    n, v0, D = 30000, 0.3, 0.05
+    rho = 0.08 if prec == "fp64f" else 0.1  # the factored form's 40-byte pass-2 halo fits the LDS at 0.08
+    L = math.sqrt(n / 4000.0)
+    st = O.make_state(29, 0, n, L, rho)
+    d0 = O.uniform(290, 3 * n, -1.0, 1.0) if init else np.zeros(3 * n)
+    a0 = O.uniform(291, 2 * n, -1.0, 1.0) if init else np.zeros(2 * n)
+    sysp = O.chimera(n, L, rho, v0, 1.2, 1.4, D, threads=8)
+    rd, ra, rc = O.rhs_ito(sysp, st, d0, a0)
+    _, _, ts = O.rhs_k(sysp, st, np.zeros(3 * n), 0.0, 0.0, d0, term_scale=True)
+    p = spc2.params("chimera", prec, "verlet", n=n, L=L, rho=rho, v0=v0, sigma=1.2, alpha=1.4, D=D)
+    with spc2.Sim(p) as s:
+        s.set_state(st)
+        s.set_counting(True)
+        gd, ga = s.eval_rhs_ito(d0, a0)
+        cnt = s.counts()
+        assert s.verlet_layout()[0] == 1
+    tol = TOL[prec]
+    assert_rhs_terms(gd, rd, ts, v0, tol, f"chimera verlet ito/{prec}", rd[2::3] == d0[2::3])
+    for c in (0, 1):
+        scale = np.abs(ra[c::2]).max()
+        err = np.abs(ga[c::2] - ra[c::2]).max()
+        assert err <= tol * scale, f"chimera verlet ito/{prec} additional[{c}]: {err:.3e} > {tol} x {scale:.3e}"
+    assert np.array_equal(cnt[1:], rc[1:]), (cnt, rc)
+
+
+@pytest.mark.parametrize("prec", ["fp32k", "fp64"])
+def test_verlet_chimera_srk_steps_match_oracle(gpu, prec):
+    """The chimera SRK workload's method on a Verlet list (skin 0.2 rho, dt =
+    0.005), each step from the oracle's state with the same Philox normals,
+    against the oracle's step_srk at the single-step bars; the handle keeps its
+    list across the steps, and pass 2 of the FP32 form reads pass 1's rho bits."""
+    from _parity import assert_state
+    n, rho, dt, steps = 30000, 0.1, 0.005, 6
+    L = math.sqrt(n / 4000.0)
+    st, om = O.init_state(7, n, L, L, 0.1)
+    sysp = O.chimera(n, L, rho, 0.01, 1.0, 1.45, 0.01, threads=8)
+    p = spc2.params("chimera", prec, "verlet", n=n, L=L, rho=rho, v0=0.01, sigma=1.0, alpha=1.45, D=0.01, seed=7)
+    cur = st
+    worst = 0.0
+    with spc2.Sim(p) as s:
+        s.set_omega(om)
+        for k in range(steps):
+            s.set_state(cur)
+            s.step("srk", dt, k, 1)
+            ref = O.step_srk(sysp, cur, dt, O.philox_normals(7, k, 0, n, 2), omega=om)
+            worst = max(worst, assert_state(s.get_state(), ref, cur, L, TOL[prec], f"chimera verlet srk/{prec} step {k}"))
+            cur = ref
+        assert s.verlet_layout()[0] == 1
+    print(f"chimera verlet srk/{prec}: largest error / bar {worst:.3e}")
