@@ -161,7 +161,9 @@ __device__ __forceinline__ int visit_lds(Acc &acc, const S &src, const SegShared
   int cand = 0;
   if constexpr (std::is_same<S, Src32>::value) {
     // FP32 form: one loop per stencil cell, its column offset computed once
-    // (the value the per-candidate selection gives inside that cell)
+    // (the value the per-candidate selection gives inside that cell), its
+    // terms summed in FP32 partials that are added to the compensated sums
+    // after the cell
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const int base = r * ncol + qi;
@@ -169,19 +171,25 @@ __device__ __forceinline__ int visit_lds(Acc &acc, const S &src, const SegShared
       acc.row(r - 1, b1, b2, P);
       cand += hi - lo;
       const float ol = acc.co.col(-1), oc = acc.co.col(0), orr = acc.co.col(1);
+      acc.cell_begin();
 #pragma unroll 2
-      for (int m = lo; m < b1; ++m) acc.pair_x(src, m, ol, P);
+      for (int m = lo; m < b1; ++m) acc.template pair_x<SUM_GROUP>(src, m, ol, P);
+      acc.flush();
+      acc.cell_begin();
       if (r == 1) {
 #pragma unroll 2
-        for (int m = b1; m < ei; ++m) acc.pair_x(src, m, oc, P);
+        for (int m = b1; m < ei; ++m) acc.template pair_x<SUM_GROUP>(src, m, oc, P);
 #pragma unroll 2
-        for (int m = ei + 1; m < b2; ++m) acc.pair_x(src, m, oc, P);
+        for (int m = ei + 1; m < b2; ++m) acc.template pair_x<SUM_GROUP>(src, m, oc, P);
       } else {
 #pragma unroll 2
-        for (int m = b1; m < b2; ++m) acc.pair_x(src, m, oc, P);
+        for (int m = b1; m < b2; ++m) acc.template pair_x<SUM_GROUP>(src, m, oc, P);
       }
+      acc.flush();
+      acc.cell_begin();
 #pragma unroll 2
-      for (int m = b2; m < hi; ++m) acc.pair_x(src, m, orr, P);
+      for (int m = b2; m < hi; ++m) acc.template pair_x<SUM_GROUP>(src, m, orr, P);
+      acc.flush();
     }
     return cand - 1;
   }
@@ -219,8 +227,16 @@ __device__ __forceinline__ int visit_global(Acc &acc, const S &src, const int *s
       else if (dxo == 0) acc.row(dyo, m0, m1, P);
       else acc.row(dyo, m0, m0, P);
       cand += m1 - m0;
-      for (int m = m0; m < m1; ++m)
-        if (m != k) acc.pair(src, m, P);
+      if constexpr (std::is_same<S, Src32>::value) {
+        // per cell FP32 partials, as visit_lds sums them
+        acc.cell_begin();
+        for (int m = m0; m < m1; ++m)
+          if (m != k) acc.template pair<SUM_GROUP>(src, m, P);
+        acc.flush();
+      } else {
+        for (int m = m0; m < m1; ++m)
+          if (m != k) acc.pair(src, m, P);
+      }
     }
   }
   return cand - 1;

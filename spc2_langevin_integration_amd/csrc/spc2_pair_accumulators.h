@@ -142,11 +142,14 @@ struct ColOffsets {
   __device__ __forceinline__ float col(int d) const { return __builtin_fmaf((float)d, hxf, oM); }
 };
 
-// How an FP32 pair term is summed. The cell walks add each term to the Kahan
-// sums. The Verlet walks sum every eight list entries in plain FP32 partials
-// (the group's first entry starts them) and add the partials to the Kahan sums
-// once per group (flush()): a compensated sum over groups, whose error bound is
-// the Kahan bound plus 7 ulp of each group's absolute sum.
+// How an FP32 pair term is summed. The Verlet walks sum every eight list
+// entries in plain FP32 partials (the group's first entry starts them) and add
+// the partials to the Kahan sums once per group (flush()): a compensated sum
+// over groups, whose error bound is the Kahan bound plus 7 ulp of each group's
+// absolute sum.  The cell walks do the same per stencil cell (cell_begin() zeroes
+// the partials, SUM_GROUP adds, flush() after the cell): the Kahan bound plus
+// (m - 1) ulp of each cell's absolute sum, m its candidates.  SUM_KAHAN adds
+// each term to the Kahan sums.
 enum SumMode { SUM_KAHAN = 0, SUM_GROUP_FIRST = 1, SUM_GROUP = 2 };
 
 template <int MODE>
@@ -272,15 +275,19 @@ struct ChimAcc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
     const float4 v = s.f4[m];
-    pair_d(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
+    pair_d<MODE>(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
   }
   // pair() with the column offset known (a run of candidates in one stencil column)
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
     const float4 v = s.f4[m];
-    pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
+    pair_d<MODE>(s, m, v, v.x + ox, v.y + co.oy, P);
   }
+  // the cell walks' partial sums over one stencil cell start at zero
+  __device__ __forceinline__ void cell_begin() { ps = pc = 0.0f; }
   // a list entry with the partner's tile already loaded (aj: pass-2 input,
   // unused here); returns whether the partner is within rho
   template <int MODE>
@@ -467,15 +474,19 @@ struct VortAcc<P32K> {
   }
   // Branch-free except for the rare cases (cutoff band, cohesion range): every
   // lane evaluates W and adds W sin, W cos masked to zero outside rho.
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
     const float4 v = s.f4[m];
-    pair_d(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
+    pair_d<MODE>(s, m, v, v.x + co.ox(m), v.y + co.oy, P);
   }
   // pair() with the column offset known (a run of candidates in one stencil column)
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
     const float4 v = s.f4[m];
-    pair_d(s, m, v, v.x + ox, v.y + co.oy, P);
+    pair_d<MODE>(s, m, v, v.x + ox, v.y + co.oy, P);
   }
+  // the cell walks' partial sums over one stencil cell start at zero
+  __device__ __forceinline__ void cell_begin() { psw = pcw = 0.0f; }
   // a list entry with the partner's tile already loaded (aj: pass-2 input,
   // unused here).  DEFER: the cohesion term is left to the caller, who gets
   // whether the partner lies within xi_r's band and calls cohesion() later, with
@@ -690,16 +701,22 @@ struct Chim2Acc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
-  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_x(s, m, co.ox(m), P); }
+  template <int MODE = SUM_KAHAN>
+  __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) {
+    pair_x<MODE>(s, m, co.ox(m), P);
+  }
   // pair() with the column offset known (a run of candidates in one stencil column)
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
     float4 v = s.f4[m];
     float dx = v.x + ox, dy = v.y + co.oy;
     float r2 = __builtin_fmaf(dx, dx, dy * dy);
     bool in = r2 <= P.rho2f_hi;  // outside the band, the same as r2 <= rho2f
     if ((r2 >= P.rho2f_lo) & in) in = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
-    sum(v, in, in ? s.aj[m] : 0.0f);
+    sum<MODE>(v, in, in ? s.aj[m] : 0.0f);
   }
+  // the cell walks' partial sums over one stencil cell start at zero
+  __device__ __forceinline__ void cell_begin() { pc = ps = ppc = pqs = 0.0f; }
   // a list entry with the partner's tile and pass-1 derivative already loaded
   template <int MODE>
   __device__ __forceinline__ void pair_c(const Src32 &s, int m, float fx, float fy, const float4 &v, float ajv,
@@ -918,12 +935,17 @@ struct Vort2Acc<P32K> {
     co.b1 = b1;
     co.b2 = b2;
   }
+  // (few candidates reach the sums, which stay per term: MODE, cell_begin()
+  // and flush() have nothing to do)
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair(const Src32 &s, int m, const KParams &P) { pair_x(s, m, co.ox(m), P); }
   // pair() with the column offset known (a run of candidates in one stencil column)
+  template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void pair_x(const Src32 &s, int m, float ox, const KParams &P) {
     const float4 v = s.f4[m];
     pair_t(s, m, v, v.x + ox, v.y + co.oy, 0.0f, false, P);
   }
+  __device__ __forceinline__ void cell_begin() {}
   // a list entry with the partner's tile and pass-1 derivative already loaded
   // (few entries reach the sums, which stay per entry: flush() has nothing to add)
   template <int MODE>
