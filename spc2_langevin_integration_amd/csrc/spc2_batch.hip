@@ -175,7 +175,8 @@ __global__ __launch_bounds__(kB) void k_b_pass2_srk(const Rep *reps, const int *
 // order (one cell, sorted by id), so a particle's slot is its home.
 template <int PREC, int EPI>
 __global__ __launch_bounds__(kB) void k_b_allpairs1(const Rep *reps, const int *ap_rep, const double *gx,
-                                                    const double *gy, const double *gp, EpiCtx e, int k) {
+                                                    const double *gy, const double *gp, const float4 *f4, EpiCtx e,
+                                                    int k) {
   const Rep &R = reps[ap_rep[blockIdx.x]];
   if (k >= R.steps) return;
   const int i = (blockIdx.x - R.apoff) * kB + threadIdx.x;
@@ -183,8 +184,9 @@ __global__ __launch_bounds__(kB) void k_b_allpairs1(const Rep *reps, const int *
   const KParams P = R.kp;
   const EpiCtx c = rep_ctx(e, R, k);
   Pass1Out o{};
-  if (R.system == SPC2_CHIMERA)
-    allpairs_pass1<PREC, SPC2_CHIMERA>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
+  if (R.system == SPC2_CHIMERA)  // the replica's one cell holds its particles in id order: tile j is id j
+    allpairs_pass1<PREC, SPC2_CHIMERA>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o,
+                                       PREC == P32K ? f4 + R.poff : nullptr);
   else
     allpairs_pass1<PREC, SPC2_VORTEX>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
   epilogue1<EPI>(P, c, R.poff + i, i, R.poff + i, o);
@@ -508,8 +510,8 @@ void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk) {
                          (const int *)b.slot_id, (const double *)b.gp, e, k);
     if (b.nap > 0)
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_allpairs1<PREC, EPI_SE>), dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
-                         (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e,
-                         k);
+                         (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp,
+                         (const float4 *)b.t.f4, e, k);
     return;
   }
   // SRK (StochasticRungeKuttaStepperPtr.hpp:32-97): both passes read the same
@@ -522,8 +524,8 @@ void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk) {
                        (const int *)b.slot_id, (const double *)b.gp, e1, k);
   if (b.nap > 0)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_allpairs1<PREC, EPI_ITO1>), dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
-                       (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e1,
-                       k);
+                       (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp,
+                       (const float4 *)b.t.f4, e1, k);
   if (b.nseg > 0)
     hipLaunchKernelGGL(k_b_pass2_srk<PREC>, dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
                        (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,

@@ -1280,9 +1280,12 @@ void launch_pair(Sim &s, void (*kern)(KArgs...), dim3 grid, dim3 blk, size_t sh,
 // All pairs (rho > L/4): thread per particle, j = 0..N-1 ascending, stage
 // positions not wrapped (ChimeraSystem.cpp:131-167, :184-254); arrays by
 // particle id.
+// tiles (FP32 form, optional): FP32 tiles by particle id whose cos and sin are
+// (float) of the FP64 sincos of gp, read instead of forming them per pair (the
+// same values).
 template <int PREC, int SYS>
 __device__ __forceinline__ void allpairs_pass1(const KParams &P, int i, const double *gx, const double *gy,
-                                               const double *gp, Pass1Out &o) {
+                                               const double *gp, Pass1Out &o, const float4 *tiles = nullptr) {
   const double xi = gx[i], yi = gy[i], pi = gp[i];
   if (SYS == SPC2_CHIMERA) {
     if (PREC == P64) {
@@ -1314,6 +1317,13 @@ __device__ __forceinline__ void allpairs_pass1(const KParams &P, int i, const do
         double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
         double r2 = dx * dx + dy * dy;
         if (r2 <= P.rho2) {
+          if (PREC == P32K && tiles) {
+            const float4 tj = tiles[j];
+            Sf.add(tj.w);
+            Cf.add(tj.z);
+            ++n;
+            continue;
+          }
           double sj, cj;
           sincos(gp[j], &sj, &cj);
           if (PREC == P32K) {
