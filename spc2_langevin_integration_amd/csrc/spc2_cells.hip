@@ -146,13 +146,20 @@ __global__ __launch_bounds__(kBlock) void k_rank_tiles(int n, int nx, double hx,
                                                        const int *tmpid, const int *cell_of, const int *start,
                                                        const double *gx, const double *gy, const double *gp,
                                                        int *slot_h, int *slot_id, Tiles t) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int ids[kRankStage];
+  __shared__ int span[2];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int h = 0, id = 0, c = 0, lo = 0, hi = 0;
+  if (k < n) {
+    h = tmp[k];
+    id = tmpid[k];
+    c = cell_of[h];
+    lo = start[c];
+    hi = start[c + 1];
+  }
+  const int s0 = stage_cell_ids(ids, span, tmpid, n, lo, hi);
   if (k >= n) return;
-  const int h = tmp[k], id = tmpid[k];
-  const int c = cell_of[h];
-  const int lo = start[c], hi = start[c + 1];
-  int r = 0;
-  for (int m = lo; m < hi; ++m) r += tmpid[m] < id;
+  const int r = s0 >= 0 ? rank_in_cell(ids + (lo - s0), hi - lo, id) : rank_in_cell(tmpid + lo, hi - lo, id);
   const int dst = lo + r;
   slot_h[dst] = h;
   slot_id[dst] = id;

@@ -50,6 +50,35 @@ __device__ __forceinline__ int cell_coord(double x, double L, int n) {
   return (int)f;
 }
 
+// ---- stable rank inside a cell ----------------------------------------------
+// The counting sort places a workgroup's slots in cells of increasing index, so
+// their cells span [start of the first slot's cell, end of the last slot's
+// cell).  stage_cell_ids copies the placed ids of that span to LDS when it holds
+// at most kRankStage of them; every thread of the workgroup calls it (lo, hi:
+// the thread's cell; unused past n).  Returns the span's first slot, or -1 when
+// the ids stay in global memory.
+constexpr int kRankStage = 2048;
+__device__ __forceinline__ int stage_cell_ids(int *ids, int *span, const int *tmpid, int n, int lo, int hi) {
+  const int k0 = blockIdx.x * blockDim.x, k1 = min(n, k0 + (int)blockDim.x) - 1;
+  const int k = k0 + (int)threadIdx.x;
+  if (k == k0) span[0] = lo;
+  if (k == k1) span[1] = hi;
+  __syncthreads();
+  const int s0 = span[0], s1 = span[1];
+  const bool staged = s1 - s0 <= kRankStage;
+  if (staged)
+    for (int m = s0 + (int)threadIdx.x; m < s1; m += (int)blockDim.x) ids[m - s0] = tmpid[m];
+  __syncthreads();
+  return staged ? s0 : -1;
+}
+
+// The rank of id among the cnt ids at a: how many are smaller.
+__device__ __forceinline__ int rank_in_cell(const int *a, int cnt, int id) {
+  int r = 0;
+  for (int m = 0; m < cnt; ++m) r += a[m] < id;
+  return r;
+}
+
 // ---- Kahan-compensated FP32 sum ----------------------------------------------
 
 struct Kahan {
