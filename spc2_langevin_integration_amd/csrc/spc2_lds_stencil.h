@@ -22,6 +22,8 @@ namespace {
 
 constexpr int kMaxSegW = 64;
 constexpr int kMaxSlots = 3 * (kMaxSegW + 2);
+// x offset that puts a candidate beyond every cutoff (r^2 about 1e12)
+constexpr float kSelfOff = 1.0e6f;
 
 
 // =============================================================================
@@ -177,10 +179,13 @@ __device__ __forceinline__ int visit_lds(Acc &acc, const S &src, const SegShared
       acc.flush();
       acc.cell_begin();
       if (r == 1) {
+        // the own cell in one loop: the particle itself is moved kSelfOff
+        // away, beyond every cutoff, so it adds the +0 terms of any candidate
+        // outside rho (partials start at +0 and never become -0, so the sums
+        // equal those that skip it).  Two loops split at ei would cost the
+        // wavefront twice the cell, since lanes split at different places.
 #pragma unroll 2
-        for (int m = b1; m < ei; ++m) acc.template pair_x<SUM_GROUP>(src, m, oc, P);
-#pragma unroll 2
-        for (int m = ei + 1; m < b2; ++m) acc.template pair_x<SUM_GROUP>(src, m, oc, P);
+        for (int m = b1; m < b2; ++m) acc.template pair_x<SUM_GROUP>(src, m, m == ei ? kSelfOff : oc, P);
       } else {
 #pragma unroll 2
         for (int m = b1; m < b2; ++m) acc.template pair_x<SUM_GROUP>(src, m, oc, P);
