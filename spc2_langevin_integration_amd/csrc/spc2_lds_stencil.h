@@ -155,6 +155,38 @@ __device__ __forceinline__ typename SrcOf<PREC>::type global_src(const Tiles &t,
   }
 }
 
+// Accumulators with a two-candidate step for the FP32 cell walks.
+template <class Acc>
+struct HasPair2 : std::false_type {};
+template <>
+struct HasPair2<ChimAcc<P32K>> : std::true_type {};
+
+// The FP32 walk of one stencil cell, LDS indices [a, b), column offset ox, in
+// FP32 partials added to the compensated sums after the cell.  SELF: the cell
+// holds the particle itself (LDS index self), which is moved kSelfOff away,
+// beyond every cutoff, so it adds the +0 terms of any candidate outside rho
+// (partials start at +0 and never become -0, so the sums equal those that skip
+// it).  One loop over the cell, not two split at the particle: a wavefront's
+// lanes split at different places, so two loops would run the cell twice.
+template <bool SELF, class Acc>
+__device__ __forceinline__ void cell_run(Acc &acc, const Src32 &src, int a, int b, float ox, int self,
+                                         const KParams &P) {
+  acc.cell_begin();
+  if constexpr (HasPair2<Acc>::value) {
+    int m = a;
+    for (; m + 1 < b; m += 2) {
+      const float oxa = SELF && m == self ? kSelfOff : ox, oxb = SELF && m + 1 == self ? kSelfOff : ox;
+      acc.pair2_x(src, m, oxa, oxb, P);
+    }
+    if (m < b) acc.template pair_x<SUM_GROUP>(src, m, SELF && m == self ? kSelfOff : ox, P);
+    acc.flush_cell();
+  } else {
+#pragma unroll 2
+    for (int m = a; m < b; ++m) acc.template pair_x<SUM_GROUP>(src, m, SELF && m == self ? kSelfOff : ox, P);
+    acc.flush();
+  }
+}
+
 // Visits the candidates of the centre particle at LDS index ei (halo column qi)
 // in reference order; returns the number of candidates.
 template <class Acc, class S>
@@ -173,28 +205,10 @@ __device__ __forceinline__ int visit_lds(Acc &acc, const S &src, const SegShared
       acc.row(r - 1, b1, b2, P);
       cand += hi - lo;
       const float ol = acc.co.col(-1), oc = acc.co.col(0), orr = acc.co.col(1);
-      acc.cell_begin();
-#pragma unroll 2
-      for (int m = lo; m < b1; ++m) acc.template pair_x<SUM_GROUP>(src, m, ol, P);
-      acc.flush();
-      acc.cell_begin();
-      if (r == 1) {
-        // the own cell in one loop: the particle itself is moved kSelfOff
-        // away, beyond every cutoff, so it adds the +0 terms of any candidate
-        // outside rho (partials start at +0 and never become -0, so the sums
-        // equal those that skip it).  Two loops split at ei would cost the
-        // wavefront twice the cell, since lanes split at different places.
-#pragma unroll 2
-        for (int m = b1; m < b2; ++m) acc.template pair_x<SUM_GROUP>(src, m, m == ei ? kSelfOff : oc, P);
-      } else {
-#pragma unroll 2
-        for (int m = b1; m < b2; ++m) acc.template pair_x<SUM_GROUP>(src, m, oc, P);
-      }
-      acc.flush();
-      acc.cell_begin();
-#pragma unroll 2
-      for (int m = b2; m < hi; ++m) acc.template pair_x<SUM_GROUP>(src, m, orr, P);
-      acc.flush();
+      cell_run<false>(acc, src, lo, b1, ol, ei, P);
+      if (r == 1) cell_run<true>(acc, src, b1, b2, oc, ei, P);
+      else cell_run<false>(acc, src, b1, b2, oc, ei, P);
+      cell_run<false>(acc, src, b2, hi, orr, ei, P);
     }
     return cand - 1;
   }

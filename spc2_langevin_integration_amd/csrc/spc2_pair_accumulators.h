@@ -254,6 +254,7 @@ struct ChimAcc<P32K> {
   ColOffsets co;
   Kahan S, C;
   float ps, pc;  // the current list group's partial sums
+  float pn;      // pair2_x's in-range count in the current cell
   int n, k;
   const double *tx, *ty;
   __device__ __forceinline__ void init(const Src32 &s, int e, int kk, const Tiles &t, const KParams &P) {
@@ -265,6 +266,7 @@ struct ChimAcc<P32K> {
     S.init();
     C.init();
     ps = pc = 0.0f;
+    pn = 0.0f;
     n = 0;
     k = kk;
     tx = t.x;
@@ -287,7 +289,7 @@ struct ChimAcc<P32K> {
     pair_d<MODE>(s, m, v, v.x + ox, v.y + co.oy, P);
   }
   // the cell walks' partial sums over one stencil cell start at zero
-  __device__ __forceinline__ void cell_begin() { ps = pc = 0.0f; }
+  __device__ __forceinline__ void cell_begin() { ps = pc = pn = 0.0f; }
   // a list entry with the partner's tile already loaded (aj: pass-2 input,
   // unused here); returns whether the partner is within rho
   template <int MODE>
@@ -298,6 +300,32 @@ struct ChimAcc<P32K> {
   __device__ __forceinline__ void flush() {
     S.add(ps);
     C.add(pc);
+  }
+  // flush() after a cell walked with pair2_x, with its count
+  __device__ __forceinline__ void flush_cell() {
+    flush();
+    n += (int)pn;
+  }
+  // Two consecutive cell candidates m, m + 1 (column offsets oxa, oxb) into the
+  // cell's partials, with one branch for both band re-checks.  An in-range
+  // flag f in {0, 1} enters as fma(f, x, p): p + x when in range and p + 0
+  // otherwise, the same values as pair_x<SUM_GROUP> for each in turn.
+  __device__ __forceinline__ void pair2_x(const Src32 &s, int m, float oxa, float oxb, const KParams &P) {
+    const float4 a = s.f4[m], b = s.f4[m + 1];
+    const float dxa = a.x + oxa, dya = a.y + co.oy, dxb = b.x + oxb, dyb = b.y + co.oy;
+    const float r2a = __builtin_fmaf(dxa, dxa, dya * dya), r2b = __builtin_fmaf(dxb, dxb, dyb * dyb);
+    bool ina = r2a <= P.rho2f_hi, inb = r2b <= P.rho2f_hi;
+    const bool ba = (r2a >= P.rho2f_lo) & ina, bb = (r2b >= P.rho2f_lo) & inb;
+    if (ba | bb) {
+      if (ba) ina = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+      if (bb) inb = exact_r2(tx, ty, k, s.gslot(m + 1), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    }
+    const float fa = ina ? 1.0f : 0.0f, fb = inb ? 1.0f : 0.0f;
+    ps = __builtin_fmaf(fa, a.w, ps);
+    pc = __builtin_fmaf(fa, a.z, pc);
+    ps = __builtin_fmaf(fb, b.w, ps);
+    pc = __builtin_fmaf(fb, b.z, pc);
+    pn += fa + fb;  // the cell's count so far, exact in FP32
   }
   template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ bool pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
