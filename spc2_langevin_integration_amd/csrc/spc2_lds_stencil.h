@@ -160,6 +160,10 @@ template <class Acc>
 struct HasPair2 : std::false_type {};
 template <>
 struct HasPair2<ChimAcc<P32K>> : std::true_type {};
+template <>
+struct HasPair2<Chim2Acc<P32K>> : std::true_type {};
+template <>
+struct HasPair2<VortAcc<P32K>> : std::true_type {};
 
 // The FP32 walk of one stencil cell, LDS indices [a, b), column offset ox, in
 // FP32 partials added to the compensated sums after the cell.  SELF: the cell

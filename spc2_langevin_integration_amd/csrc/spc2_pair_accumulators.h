@@ -474,6 +474,7 @@ struct VortAcc<P32K> {
   ColOffsets co;
   Kahan SW, CW, Tx, Ty;
   float psw, pcw;    // the current list group's partial sums
+  float pn;          // pair2_x's within-rho count in the current cell
   float mu_pv, wav;  // mu_pf and w_af in vector registers
   int n01, n2, k;
   const double *tx, *ty;
@@ -487,7 +488,7 @@ struct VortAcc<P32K> {
     CW.init();
     Tx.init();
     Ty.init();
-    psw = pcw = 0.0f;
+    psw = pcw = pn = 0.0f;
     mu_pv = in_vgpr(P.mu_pf);
     wav = in_vgpr(P.w_af);
     n01 = n2 = 0;
@@ -514,7 +515,7 @@ struct VortAcc<P32K> {
     pair_d<MODE>(s, m, v, v.x + ox, v.y + co.oy, P);
   }
   // the cell walks' partial sums over one stencil cell start at zero
-  __device__ __forceinline__ void cell_begin() { psw = pcw = 0.0f; }
+  __device__ __forceinline__ void cell_begin() { psw = pcw = pn = 0.0f; }
   // a list entry with the partner's tile already loaded (aj: pass-2 input,
   // unused here).  DEFER: the cohesion term is left to the caller, who gets
   // whether the partner lies within xi_r's band and calls cohesion() later, with
@@ -527,6 +528,45 @@ struct VortAcc<P32K> {
   __device__ __forceinline__ void flush() {
     SW.add(psw);
     CW.add(pcw);
+  }
+  // flush() after a cell walked with pair2_x, with its count
+  __device__ __forceinline__ void flush_cell() {
+    flush();
+    n01 += (int)pn;
+  }
+  // W(r) of an FP32 r^2, masked to zero outside rho
+  __device__ __forceinline__ float weight(float r2, bool inr, const KParams &P) const {
+    float r = __builtin_amdgcn_sqrtf(r2);
+    float w_in = __builtin_fmaf(-P.mu_p_over_xa2f, r2, mu_pv);
+    float w_an = __builtin_fmaf(r, __builtin_fmaf(-P.c4f, r, wav), P.w_bf);
+    float w = r2 <= P.xa2f ? w_in : w_an;
+    return inr ? w : 0.0f;
+  }
+  // Two consecutive cell candidates m, m + 1 (column offsets oxa, oxb), with
+  // one branch for both rho band re-checks and one for both cohesion tests.
+  // The terms, counts and cohesion sums are those of pair_x<SUM_GROUP> for
+  // each in turn (the W partials and the cohesion sums are separate chains).
+  __device__ __forceinline__ void pair2_x(const Src32 &s, int m, float oxa, float oxb, const KParams &P) {
+    const float4 a = s.f4[m], b = s.f4[m + 1];
+    const float dxa = a.x + oxa, dya = a.y + co.oy, dxb = b.x + oxb, dyb = b.y + co.oy;
+    const float r2a = __builtin_fmaf(dxa, dxa, dya * dya), r2b = __builtin_fmaf(dxb, dxb, dyb * dyb);
+    bool ina = r2a <= P.rho2f_hi, inb = r2b <= P.rho2f_hi;
+    const bool ba = (r2a >= P.rho2f_lo) & ina, bb = (r2b >= P.rho2f_lo) & inb;
+    if (ba | bb) {
+      if (ba) ina = sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy)) <= P.rho;
+      if (bb) inb = sqrt(exact_r2(tx, ty, k, s.gslot(m + 1), P.Lx, P.rLx, P.Ly, P.rLy)) <= P.rho;
+    }
+    const float wa = weight(r2a, ina, P), wb = weight(r2b, inb, P);
+    psw = __builtin_fmaf(wa, a.w, psw);
+    pcw = __builtin_fmaf(wa, a.z, pcw);
+    psw = __builtin_fmaf(wb, b.w, psw);
+    pcw = __builtin_fmaf(wb, b.z, pcw);
+    pn += (ina ? 1.0f : 0.0f) + (inb ? 1.0f : 0.0f);
+    const bool ca = r2a <= P.xr2f_hi, cb = r2b <= P.xr2f_hi;
+    if (ca | cb) {
+      if (ca) cohesion(s, m, dxa, dya, r2a, P);
+      if (cb) cohesion(s, m + 1, dxb, dyb, r2b, P);
+    }
   }
   template <int MODE = SUM_KAHAN, bool DEFER = false>
   __device__ __forceinline__ bool pair_d(const Src32 &s, int m, const float4 &v, float dx, float dy,
@@ -769,6 +809,31 @@ struct Chim2Acc<P32K> {
     S.add(ps);
     Pc.add(ppc);
     Qs.add(pqs);
+  }
+  __device__ __forceinline__ void flush_cell() { flush(); }
+  // Two consecutive cell candidates, as ChimAcc<P32K>::pair2_x: the in-range
+  // flag f and f * aj enter as fma factors, the same values as pair_x<SUM_GROUP>
+  // for each in turn (the halo's pass-1 derivatives are finite).
+  __device__ __forceinline__ void pair2_x(const Src32 &s, int m, float oxa, float oxb, const KParams &P) {
+    const float4 a = s.f4[m], b = s.f4[m + 1];
+    const float dxa = a.x + oxa, dya = a.y + co.oy, dxb = b.x + oxb, dyb = b.y + co.oy;
+    const float r2a = __builtin_fmaf(dxa, dxa, dya * dya), r2b = __builtin_fmaf(dxb, dxb, dyb * dyb);
+    bool ina = r2a <= P.rho2f_hi, inb = r2b <= P.rho2f_hi;
+    const bool ba = (r2a >= P.rho2f_lo) & ina, bb = (r2b >= P.rho2f_lo) & inb;
+    if (ba | bb) {
+      if (ba) ina = exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+      if (bb) inb = exact_r2(tx, ty, k, s.gslot(m + 1), P.Lx, P.rLx, P.Ly, P.rLy) <= P.rho2;
+    }
+    const float fa = ina ? 1.0f : 0.0f, fb = inb ? 1.0f : 0.0f;
+    const float ga = fa * s.aj[m], gb = fb * s.aj[m + 1];
+    pc = __builtin_fmaf(fa, a.z, pc);
+    ps = __builtin_fmaf(fa, a.w, ps);
+    ppc = __builtin_fmaf(ga, a.z, ppc);
+    pqs = __builtin_fmaf(ga, a.w, pqs);
+    pc = __builtin_fmaf(fb, b.z, pc);
+    ps = __builtin_fmaf(fb, b.w, ps);
+    ppc = __builtin_fmaf(gb, b.z, ppc);
+    pqs = __builtin_fmaf(gb, b.w, pqs);
   }
   template <int MODE = SUM_KAHAN>
   __device__ __forceinline__ void sum(const float4 &v, bool in, float aj) {
