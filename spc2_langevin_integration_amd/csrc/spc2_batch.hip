@@ -7,8 +7,9 @@
 // array, and its linked-cell row segments soff[r] .. in one global segment
 // numbering.  A step bins every particle into its replica's cells (one
 // histogram), scans all cells at once, counting-sorts all particles, then one
-// pair launch covers every row segment of every linked replica (a block looks
-// its replica up in a table) and one more covers the all-pairs replicas.  The
+// pair launch per pass covers the all-pairs replicas' blocks, first, and every
+// row segment of every linked replica (a block looks its replica up in a
+// table).  The
 // per-replica arithmetic, summation order and noise keys (replica seed, the
 // replica's own step, particle id within the replica) are those of a single
 // handle, so a replica's trajectory is bit-identical alone or in any batch.
@@ -141,14 +142,68 @@ __device__ __forceinline__ EpiCtx rep_ctx(const EpiCtx &e, const Rep &R, int k) 
   return c;
 }
 
+// All-pairs replicas (rho > L/4): thread per particle; their homes are in id
+// order (one cell, sorted by id), so a particle's slot is its home.  Block ab
+// of the all-pairs blocks.
+template <int PREC, int EPI>
+__device__ __forceinline__ void b_allpairs1(const Rep *reps, const int *ap_rep, int ab, const double *gx,
+                                            const double *gy, const double *gp, const float4 *f4, const EpiCtx &e,
+                                            int k) {
+  const Rep &R = reps[ap_rep[ab]];
+  if (k >= R.steps) return;
+  const int i = (ab - R.apoff) * kB + threadIdx.x;
+  if (i >= R.n) return;
+  const KParams P = R.kp;
+  const EpiCtx c = rep_ctx(e, R, k);
+  Pass1Out o{};
+  if (R.system == SPC2_CHIMERA)  // the replica's one cell holds its particles in id order: tile j is id j
+    allpairs_pass1<PREC, SPC2_CHIMERA>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o,
+                                       PREC == P32K ? f4 + R.poff : nullptr);
+  else
+    allpairs_pass1<PREC, SPC2_VORTEX>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
+  epilogue1<EPI>(P, c, R.poff + i, i, R.poff + i, o);
+}
+
+__device__ __forceinline__ void b_allpairs2_srk(const Rep *reps, const int *ap_rep, int ab, const double *gx,
+                                                const double *gy, const double *gp, const EpiCtx &e, int k) {
+  const Rep &R = reps[ap_rep[ab]];
+  if (k >= R.steps) return;
+  const int i = (ab - R.apoff) * kB + threadIdx.x;
+  if (i >= R.n) return;
+  const KParams P = R.kp;
+  const EpiCtx c = rep_ctx(e, R, k);
+  const int o0 = R.poff;
+  Pass2Out o;
+  if (R.system == SPC2_CHIMERA)
+    allpairs_pass2<SPC2_CHIMERA>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
+                                 0.0, 0.0, o);
+  else
+    allpairs_pass2<SPC2_VORTEX>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
+                                0.0, 0.0, o);
+  epilogue2<EPI_SRK>(P, c, o0 + i, i, o);
+}
+
+// The all-pairs blocks lead the pass launches: the first ap.pad blocks (ap.n
+// of them used, ap.pad a multiple of 8 so that the row segments keep their XCD
+// mapping) take all-pairs replicas.  They are few and long, and starting first
+// they run beside the row segments instead of after them.
+struct ApBlocks {
+  const int *rep;
+  int n, pad;
+};
+
 // Pass 1 over every row segment of every linked replica: pair sums and the SE
 // update (EPI_SE), or the SRK step's Ito intermediates by slot (EPI_ITO1).
 template <int PREC, int EPI>
 __global__ __launch_bounds__(kB) void k_b_pass1(const Rep *reps, const int *seg_rep, const int *start, Tiles t,
-                                                const int *slot_h, const int *slot_id, const double *gp, EpiCtx e,
-                                                int k) {
+                                                const int *slot_h, const int *slot_id, const double *gx,
+                                                const double *gy, const double *gp, EpiCtx e, int k, ApBlocks ap) {
   extern __shared__ float4 dyn_lds[];
-  const int sg = xcd_segment(blockIdx.x, gridDim.x);
+  if ((int)blockIdx.x < ap.pad) {
+    if ((int)blockIdx.x < ap.n) b_allpairs1<PREC, EPI>(reps, ap.rep, blockIdx.x, gx, gy, gp, t.f4, e, k);
+    return;
+  }
+  const int sg = xcd_segment(blockIdx.x - ap.pad, gridDim.x - ap.pad);
   const Rep &R = reps[seg_rep[sg]];
   if (k >= R.steps) return;  // the whole workgroup: the replica is done in this call
   const KParams P = R.kp;
@@ -164,10 +219,15 @@ __global__ __launch_bounds__(kB) void k_b_pass1(const Rep *reps, const int *seg_
 // Pass 2 of the SRK step over every row segment of every linked replica.
 template <int PREC>
 __global__ __launch_bounds__(kB) void k_b_pass2_srk(const Rep *reps, const int *seg_rep, const int *start, Tiles t,
-                                                    const int *slot_h, const int *slot_id, const double *gp,
-                                                    EpiCtx e, int k) {
+                                                    const int *slot_h, const int *slot_id, const double *gx,
+                                                    const double *gy, const double *gp, EpiCtx e, int k,
+                                                    ApBlocks ap) {
   extern __shared__ float4 dyn_lds[];
-  const int sg = xcd_segment(blockIdx.x, gridDim.x);
+  if ((int)blockIdx.x < ap.pad) {
+    if ((int)blockIdx.x < ap.n) b_allpairs2_srk(reps, ap.rep, blockIdx.x, gx, gy, gp, e, k);
+    return;
+  }
+  const int sg = xcd_segment(blockIdx.x - ap.pad, gridDim.x - ap.pad);
   const Rep &R = reps[seg_rep[sg]];
   if (k >= R.steps) return;
   const KParams P = R.kp;
@@ -176,46 +236,6 @@ __global__ __launch_bounds__(kB) void k_b_pass2_srk(const Rep *reps, const int *
     pass2_segment<Chim2Acc<PREC>, PREC, EPI_SRK>(P, start + R.coff, t, slot_h, slot_id, gp, c, sg - R.soff, dyn_lds);
   else
     pass2_segment<Vort2Acc<PREC>, PREC, EPI_SRK>(P, start + R.coff, t, slot_h, slot_id, gp, c, sg - R.soff, dyn_lds);
-}
-
-// All-pairs replicas (rho > L/4): thread per particle; their homes are in id
-// order (one cell, sorted by id), so a particle's slot is its home.
-template <int PREC, int EPI>
-__global__ __launch_bounds__(kB) void k_b_allpairs1(const Rep *reps, const int *ap_rep, const double *gx,
-                                                    const double *gy, const double *gp, const float4 *f4, EpiCtx e,
-                                                    int k) {
-  const Rep &R = reps[ap_rep[blockIdx.x]];
-  if (k >= R.steps) return;
-  const int i = (blockIdx.x - R.apoff) * kB + threadIdx.x;
-  if (i >= R.n) return;
-  const KParams P = R.kp;
-  const EpiCtx c = rep_ctx(e, R, k);
-  Pass1Out o{};
-  if (R.system == SPC2_CHIMERA)  // the replica's one cell holds its particles in id order: tile j is id j
-    allpairs_pass1<PREC, SPC2_CHIMERA>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o,
-                                       PREC == P32K ? f4 + R.poff : nullptr);
-  else
-    allpairs_pass1<PREC, SPC2_VORTEX>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
-  epilogue1<EPI>(P, c, R.poff + i, i, R.poff + i, o);
-}
-
-__global__ __launch_bounds__(kB) void k_b_allpairs2_srk(const Rep *reps, const int *ap_rep, const double *gx,
-                                                        const double *gy, const double *gp, EpiCtx e, int k) {
-  const Rep &R = reps[ap_rep[blockIdx.x]];
-  if (k >= R.steps) return;
-  const int i = (blockIdx.x - R.apoff) * kB + threadIdx.x;
-  if (i >= R.n) return;
-  const KParams P = R.kp;
-  const EpiCtx c = rep_ctx(e, R, k);
-  const int o0 = R.poff;
-  Pass2Out o;
-  if (R.system == SPC2_CHIMERA)
-    allpairs_pass2<SPC2_CHIMERA>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
-                                 0.0, 0.0, o);
-  else
-    allpairs_pass2<SPC2_VORTEX>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
-                                0.0, 0.0, o);
-  epilogue2<EPI_SRK>(P, c, o0 + i, i, o);
 }
 
 // Re-homing of the SE arrays (x, y, phi, omega): slot k becomes home k.
@@ -510,36 +530,27 @@ void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk) {
   hipLaunchKernelGGL(k_b_rank_tiles<PREC>, g, blk, 0, b.stream, b.nt, (const Rep *)b.d_reps, (const int *)b.rep_of,
                      (const int *)b.tmp, (const int *)b.tmpid, (const int *)b.cell_of, (const int *)b.start,
                      (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, b.slot_h, b.slot_id, b.t);
+  const ApBlocks ap{b.ap_rep, b.nap, (b.nap + 7) / 8 * 8};
+  const dim3 gpass(ap.pad + b.nseg);
+  if (gpass.x == 0) return;
   if (!srk) {
-    if (b.nseg > 0)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_pass1<PREC, EPI_SE>), dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
-                         (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
-                         (const int *)b.slot_id, (const double *)b.gp, e, k);
-    if (b.nap > 0)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_allpairs1<PREC, EPI_SE>), dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
-                         (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp,
-                         (const float4 *)b.t.f4, e, k);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_pass1<PREC, EPI_SE>), gpass, blk, b.lds, b.stream, (const Rep *)b.d_reps,
+                       (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
+                       (const int *)b.slot_id, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e, k,
+                       ap);
     return;
   }
   // SRK (StochasticRungeKuttaStepperPtr.hpp:32-97): both passes read the same
   // stage tiles; pass 2 reads every replica's pass-1 values
   EpiCtx e1 = e;
   e1.sa.zero_in = 1;  // derivative zero-filled each step
-  if (b.nseg > 0)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_pass1<PREC, EPI_ITO1>), dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
-                       (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
-                       (const int *)b.slot_id, (const double *)b.gp, e1, k);
-  if (b.nap > 0)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_allpairs1<PREC, EPI_ITO1>), dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
-                       (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp,
-                       (const float4 *)b.t.f4, e1, k);
-  if (b.nseg > 0)
-    hipLaunchKernelGGL(k_b_pass2_srk<PREC>, dim3(b.nseg), blk, b.lds, b.stream, (const Rep *)b.d_reps,
-                       (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
-                       (const int *)b.slot_id, (const double *)b.gp, e, k);
-  if (b.nap > 0)
-    hipLaunchKernelGGL(k_b_allpairs2_srk, dim3(b.nap), blk, 0, b.stream, (const Rep *)b.d_reps,
-                       (const int *)b.ap_rep, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e, k);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_pass1<PREC, EPI_ITO1>), gpass, blk, b.lds, b.stream, (const Rep *)b.d_reps,
+                     (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
+                     (const int *)b.slot_id, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e1, k,
+                     ap);
+  hipLaunchKernelGGL(k_b_pass2_srk<PREC>, gpass, blk, b.lds, b.stream, (const Rep *)b.d_reps, (const int *)b.seg_rep,
+                     (const int *)b.start, b.t, (const int *)b.slot_h, (const int *)b.slot_id, (const double *)b.gx,
+                     (const double *)b.gy, (const double *)b.gp, e, k, ap);
 }
 
 void launch_rehome(spc2_batch &b) {
