@@ -272,8 +272,10 @@ spc2_status spc2_batch_snapshot_async(spc2_batch_t b, float *dst);
 spc2_status spc2_batch_snapshot_wait(spc2_batch_t b);
 spc2_status spc2_batch_layout(spc2_batch_t b, int32_t *first_particle, int32_t *particles);
 
-/* Row-segment tile size of each linked replica's pair kernel, in particles
- * per workgroup segment (the autotuner's knob). */
+/* Tile size of each linked replica's pair kernel, in particles per
+ * workgroup (the autotuner's knob): rows with at least 8 particles per cell
+ * are cut into chunks of that many slots, rounded to whole 64-lane
+ * wavefronts; sparser rows into segments of about that many particles. */
 spc2_status spc2_batch_set_tile_targets(spc2_batch_t b, const double *particles_per_segment);
 spc2_status spc2_batch_get_tile_targets(spc2_batch_t b, double *particles_per_segment);
 

@@ -873,21 +873,21 @@ spc2_status spc2_batch_set_tile_targets(spc2_batch_t b, const double *particles_
   if (check_batch(b) || !particles_per_segment) return set_error(SPC2_ERR_INVALID, "null argument");
   for (int r = 0; r < b->count; ++r) {
     if (!(particles_per_segment[r] >= 16.0 && particles_per_segment[r] <= 4096.0))
-      return set_error(SPC2_ERR_INVALID, "tile targets must lie in [16, 4096] particles per segment");
+      return set_error(SPC2_ERR_INVALID, "tile targets must lie in [16, 4096] particles per workgroup");
   }
   b->target.assign(particles_per_segment, particles_per_segment + b->count);
   return upload_layout(*b);
 }
 
 // Autotuner: per bucket of linked replicas with the same floor(log2 N), the
-// row-segment tile (particles per workgroup segment) that minimises the step
+// pair-kernel tile (particles per workgroup) that minimises the step
 // time of the whole batch, by coordinate descent over the buckets.  Trials run
 // on a copy of the batch, so the caller's replicas do not move.
 spc2_status spc2_batch_autotune(spc2_batch_t b, double dt, int32_t trial_steps, double *ms_per_step) {
   if (check_batch(b)) return SPC2_ERR_INVALID;
   if (!(dt > 0.0) || trial_steps < 1 || trial_steps > 1000)
     return set_error(SPC2_ERR_INVALID, "dt must be > 0 and trial_steps in [1, 1000]");
-  static const double kCandidates[] = {96.0, 160.0, 218.0, 320.0, 480.0, 720.0};
+  static const double kCandidates[] = {128.0, 192.0, 256.0, 320.0, 448.0, 704.0};
   std::vector<int32_t> same(b->count);
   for (int r = 0; r < b->count; ++r) same[r] = r;
   spc2_batch_t trial = nullptr;

@@ -50,9 +50,11 @@ struct KParams {
   float rho2f_lo, rho2f_hi, xr2f_hi;  // rho2f -/+ band_rho2f, xr2f + band_xr2f
   // FP32 cohesion pairs closer than this (r^2) take FP64 geometry
   float close2f;
-  // pair-kernel decomposition: row segments of seg_w cells, segs per row,
-  // LDS capacity in tile elements per workgroup
-  int seg_w, segs, lds_cap;
+  // pair-kernel decomposition, segs workgroups per cell row: row chunks of
+  // `chunk` slots (chunk > 0; the row's last workgroup takes the rest of the
+  // row) or, with chunk 0, row segments of seg_w cells; LDS capacity in tile
+  // elements per workgroup
+  int seg_w, segs, lds_cap, chunk;
 };
 
 // Tile arrays in cell-sorted slot order, written by the rank kernel.
@@ -222,7 +224,7 @@ void launch_place_raw(hipStream_t st, int n, const int *cell_of, const int *loca
 void launch_place(Sim &s);
 void launch_rank_tiles(Sim &s);
 // spc2_pairs.hip
-constexpr double kSegTarget = 218.0;  // default particles per row segment
+constexpr double kSegTarget = 256.0;  // default particles per pair workgroup (row chunk or segment)
 constexpr int kLdsBytes = 60 * 1024;  // dynamic LDS of one pair workgroup at most
 int lds_elem_bytes(int prec);
 void configure_segments_kp(KParams &k, int prec, double target);

@@ -20,8 +20,9 @@
 namespace spc2 {
 namespace {
 
-constexpr int kMaxSegW = 64;
+constexpr int kMaxSegW = 64;  // 3 (kMaxSegW + 2) halo cells, one per lane of the setup scan
 constexpr int kMaxSlots = 3 * (kMaxSegW + 2);
+constexpr double kChunkPerCell = 8.0;  // mean particles per cell from which rows are cut into chunks
 // x offset that puts a candidate beyond every cutoff (r^2 about 1e12)
 constexpr float kSelfOff = 1.0e6f;
 
@@ -56,11 +57,13 @@ struct Seg {
   int cy, cx0, W, ncol, ns, total;
 };
 
-__device__ __forceinline__ Seg seg_setup(const KParams &P, const int *start, int seg, SegShared &sh) {
+// Stages the halo table of centre columns cx0 .. cx0 + W - 1 of row cy (W <=
+// kMaxSegW): the 3 x (W + 2) cells' first slots and LDS offsets.
+__device__ __forceinline__ Seg halo_setup(const KParams &P, const int *start, int cy, int cx0, int W, SegShared &sh) {
   Seg g;
-  g.cy = seg / P.segs;
-  g.cx0 = (seg - g.cy * P.segs) * P.seg_w;
-  g.W = min(P.seg_w, P.nx - g.cx0);
+  g.cy = cy;
+  g.cx0 = cx0;
+  g.W = W;
   g.ncol = g.W + 2;
   g.ns = 3 * g.ncol;
   int cnt = 0;
@@ -81,6 +84,32 @@ __device__ __forceinline__ Seg seg_setup(const KParams &P, const int *start, int
   __syncthreads();
   g.total = sh.off[g.ns];
   return g;
+}
+
+// The centre slots [k0, k1) of pair workgroup `seg`, their row cy and the
+// columns cx0 .. cx0 + W - 1 they occupy.  Row chunks (P.chunk > 0): slots
+// j * chunk .. of the row, the row's last workgroup (j = segs - 1) taking the
+// rest; false if the chunk starts past the row.  Row segments: seg_w cells.
+struct Block {
+  int k0, k1, cy, cx0, W;
+};
+__device__ __forceinline__ bool block_range(const KParams &P, const int *start, const Tiles &t, int seg, Block &b) {
+  b.cy = seg / P.segs;
+  const int j = seg - b.cy * P.segs, row = b.cy * P.nx;
+  if (P.chunk == 0) {
+    b.cx0 = j * P.seg_w;
+    b.W = min(P.seg_w, P.nx - b.cx0);
+    b.k0 = start[row + b.cx0];
+    b.k1 = start[row + b.cx0 + b.W];
+    return true;
+  }
+  const int re = start[row + P.nx];
+  b.k0 = start[row] + j * P.chunk;
+  if (b.k0 >= re) return false;
+  b.k1 = j == P.segs - 1 ? re : min(re, b.k0 + P.chunk);
+  b.cx0 = t.cell[b.k0] - row;
+  b.W = t.cell[b.k1 - 1] - row - b.cx0 + 1;
+  return true;
 }
 
 // largest halo slot s with off[s] <= e
@@ -275,8 +304,12 @@ __device__ __forceinline__ void pass1_segment(const KParams &P, const int *__res
                                               const double *gp, const EpiCtx &e, unsigned long long *counts, int seg,
                                               void *dyn_lds) {
   __shared__ SegShared sh;
-  Seg g = seg_setup(P, start, seg, sh);
-  const bool staged = g.total <= P.lds_cap;
+  Block b;
+  if (!block_range(P, start, t, seg, b)) return;  // the whole workgroup
+  const bool fits = b.W <= kMaxSegW;
+  Seg g{};
+  if (fits) g = halo_setup(P, start, b.cy, b.cx0, b.W, sh);
+  const bool staged = fits && g.total <= P.lds_cap;
   LdsTile<PREC, 1> lt(dyn_lds, P.lds_cap);
   if (staged) {
     for (int el = threadIdx.x; el < g.total; el += kPairBlock) {
@@ -285,9 +318,11 @@ __device__ __forceinline__ void pass1_segment(const KParams &P, const int *__res
     }
   }
   __syncthreads();
-  const int e0 = sh.off[g.ncol + 1];
-  const int ncen = sh.off[g.ncol + g.W + 1] - e0;
-  const int k0 = sh.gst[g.ncol + 1];
+  // the centre slots are contiguous from the first centre cell's, k0 its slot
+  // (k0 - gst) of the cell
+  const int e0 = staged ? sh.off[g.ncol + 1] + (b.k0 - sh.gst[g.ncol + 1]) : 0;
+  const int ncen = b.k1 - b.k0;
+  const int k0 = b.k0;
   long long cand_sum = 0, n_sum = 0, n2_sum = 0;
   for (int p = threadIdx.x; p < ncen; p += kPairBlock) {
     const int k = k0 + p, ei = e0 + p;
@@ -323,8 +358,12 @@ __device__ __forceinline__ void pass2_segment(const KParams &P, const int *__res
                                               const int *__restrict__ slot_h, const int *__restrict__ slot_id,
                                               const double *gp, const EpiCtx &e, int seg, void *dyn_lds) {
   __shared__ SegShared sh;
-  Seg g = seg_setup(P, start, seg, sh);
-  const bool staged = g.total <= P.lds_cap;
+  Block b;
+  if (!block_range(P, start, t, seg, b)) return;
+  const bool fits = b.W <= kMaxSegW;
+  Seg g{};
+  if (fits) g = halo_setup(P, start, b.cy, b.cx0, b.W, sh);
+  const bool staged = fits && g.total <= P.lds_cap;
   LdsTile<PREC, 2> lt(dyn_lds, P.lds_cap);
   if (staged) {
     for (int el = threadIdx.x; el < g.total; el += kPairBlock) {
@@ -333,9 +372,9 @@ __device__ __forceinline__ void pass2_segment(const KParams &P, const int *__res
     }
   }
   __syncthreads();
-  const int e0 = sh.off[g.ncol + 1];
-  const int ncen = sh.off[g.ncol + g.W + 1] - e0;
-  const int k0 = sh.gst[g.ncol + 1];
+  const int e0 = staged ? sh.off[g.ncol + 1] + (b.k0 - sh.gst[g.ncol + 1]) : 0;
+  const int ncen = b.k1 - b.k0;
+  const int k0 = b.k0;
   for (int p = threadIdx.x; p < ncen; p += kPairBlock) {
     const int k = k0 + p, ei = e0 + p;
     const int h = slot_h[k], id = slot_id[k];

@@ -149,21 +149,42 @@ void pass2_sys(Sim &s, int epi, const EpiCtx &e) {
 
 }  // namespace
 
-// Row segments of about `target` particles (seg_w cells, at most kMaxSegW) and
-// an LDS capacity of 1.6 x the mean halo plus slack, within 60 KB.
+// The pair workgroups of a linked system: about `target` particles each and an
+// LDS capacity for their halo.
+// * Rows of at least kChunkPerCell particles per cell are cut into chunks of
+//   `target` slots rounded to whole wavefronts, so that every wavefront but a
+//   row's last is full.  A row gets workgroups for its mean count plus four
+//   standard deviations; its last workgroup takes whatever remains.
+// * Sparser rows are cut into segments of seg_w cells (at most kMaxSegW), whose
+//   particle counts follow the cells.
+// The LDS capacity is the mean halo with a margin, within 60 KB; a workgroup
+// whose halo exceeds it reads the same tiles from global memory.
 void configure_segments_kp(KParams &k, int prec, double target) {
+  k.chunk = 0;
   if (!k.linked) {
     k.seg_w = k.segs = 1;
     k.lds_cap = 0;
     return;
   }
   const double per_cell = (double)k.n / ((double)k.nx * (double)k.ny);
-  int w = (int)std::floor(target / std::max(per_cell, 1e-9));
-  w = std::max(1, std::min({w, kMaxSegW, k.nx}));
-  k.seg_w = w;
-  k.segs = (k.nx + w - 1) / w;
-  const double halo = 3.0 * (w + 2) * per_cell;
-  int cap = (int)std::ceil((1.6 * halo + 160.0) / 64.0) * 64;
+  double halo;
+  if (per_cell >= kChunkPerCell) {
+    int chunk = 64 * std::max(1, (int)std::lround(target / 64.0));
+    // at most kMaxSegW - 3 cells' worth, so a chunk's columns fit the halo tables
+    chunk = std::min(chunk, 64 * std::max(1, (int)std::floor((kMaxSegW - 3) * per_cell / 64.0)));
+    const double row = (double)k.n / (double)k.ny;
+    k.chunk = chunk;
+    k.segs = std::max(1, (int)std::ceil((row + 4.0 * std::sqrt(row) + 8.0) / chunk));
+    k.seg_w = std::min(k.nx, (int)std::ceil(chunk / per_cell) + 1);
+    halo = 1.25 * 3.0 * (chunk + 3.0 * per_cell);  // chunk's cells, a partial cell at each end, and the sides
+  } else {
+    int w = (int)std::floor(target / std::max(per_cell, 1e-9));
+    w = std::max(1, std::min({w, kMaxSegW, k.nx}));
+    k.seg_w = w;
+    k.segs = (k.nx + w - 1) / w;
+    halo = 1.6 * 3.0 * (w + 2) * per_cell;
+  }
+  int cap = (int)std::ceil((halo + 160.0) / 64.0) * 64;
   cap = std::min(cap, kLdsBytes / lds_elem_bytes(prec));
   if (const char *env = std::getenv("SPC2_LDS_CAP")) cap = std::max(0, std::min(cap, std::atoi(env)));
   k.lds_cap = cap;
