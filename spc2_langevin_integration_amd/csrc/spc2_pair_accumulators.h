@@ -542,6 +542,23 @@ struct VortAcc<P32K> {
     float w = r2 <= P.xa2f ? w_in : w_an;
     return inr ? w : 0.0f;
   }
+  // An entry's stages for walks that test two entries before summing either:
+  // the FP32 rho test (returns whether the FP64 re-check rho_exact decides),
+  // then term() with the decision.  Together, the values of pair_d<MODE, true>.
+  __device__ __forceinline__ bool rho_test(float r2, bool &inr, const KParams &P) const {
+    inr = r2 <= P.rho2f_hi;
+    return (r2 >= P.rho2f_lo) & inr;
+  }
+  __device__ __forceinline__ bool rho_exact(const Src32 &s, int m, const KParams &P) const {
+    return sqrt(exact_r2(tx, ty, k, s.gslot(m), P.Lx, P.rLx, P.Ly, P.rLy)) <= P.rho;
+  }
+  template <int MODE>
+  __device__ __forceinline__ void term(const float4 &v, float r2, bool inr, const KParams &P) {
+    const float w = weight(r2, inr, P);
+    group_add_prod<MODE>(psw, w, v.w);
+    group_add_prod<MODE>(pcw, w, v.z);
+    n01 += inr;
+  }
   // Two consecutive cell candidates m, m + 1 (column offsets oxa, oxb), with
   // one branch for both rho band re-checks and one for both cohesion tests.
   // The terms, counts and cohesion sums are those of pair_x<SUM_GROUP> for

@@ -738,6 +738,49 @@ __device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 
         v[u] = F::tile(off);
         a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
       }
+      if constexpr (COH_QUEUE) {
+        // two entries at a time: one branch for both rho band re-checks and one
+        // for both queue tests
+#pragma unroll
+        for (int u = 0; u < 4; u += 2) {
+          const float dxa = v[u].x + acc.co.cdx(F::fx(e[u]), P.hxf), dya = v[u].y - acc.co.yf;
+          const float dxb = v[u + 1].x + acc.co.cdx(F::fx(e[u + 1]), P.hxf), dyb = v[u + 1].y - acc.co.yf;
+          const float r2a = __builtin_fmaf(dxa, dxa, dya * dya), r2b = __builtin_fmaf(dxb, dxb, dyb * dyb);
+          bool ina, inb;
+          const bool ba = acc.rho_test(r2a, ina, P), bb = acc.rho_test(r2b, inb, P);
+          if (ba | bb) {
+            if (ba) ina = acc.rho_exact(src, F::index(e[u]), P);
+            if (bb) inb = acc.rho_exact(src, F::index(e[u + 1]), P);
+          }
+          if (h == 0 && u == 0) acc.template term<SUM_GROUP_FIRST>(v[u], r2a, ina, P);
+          else acc.template term<SUM_GROUP>(v[u], r2a, ina, P);
+          acc.template term<SUM_GROUP>(v[u + 1], r2b, inb, P);
+          const bool na = r2a <= P.xr2f_hi, nb = r2b <= P.xr2f_hi;
+          if (na | nb) {
+            // the entries from an opaque copy of their word, so that the common
+            // path keeps its one-instruction decode of the high half
+            uint32_t wq = w[2 * h + (u >> 1)];
+            asm volatile("" : "+v"(wq));
+            if (na) {
+              if (qn == kCohQueue) {
+                coh_drain(acc, src, cq, qn, P);
+                qn = 0;
+              }
+              cq[qn * kBlock] = wq & 0xffffu;
+              ++qn;
+            }
+            if (nb) {
+              if (qn == kCohQueue) {
+                coh_drain(acc, src, cq, qn, P);
+                qn = 0;
+              }
+              cq[qn * kBlock] = wq >> 16;
+              ++qn;
+            }
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if constexpr (RHO_BITS) {
@@ -746,25 +789,6 @@ __device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 
                   ? acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P)
                   : acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
           bits |= (uint32_t)in << (4 * h + u);
-        } else if constexpr (COH_QUEUE) {
-          const bool near =
-              h == 0 && u == 0
-                  ? acc.template pair_c<SUM_GROUP_FIRST, true>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u],
-                                                               a[u], P)
-                  : acc.template pair_c<SUM_GROUP, true>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u],
-                                                         P);
-          if (near) {
-            if (qn == kCohQueue) {
-              coh_drain(acc, src, cq, qn, P);
-              qn = 0;
-            }
-            // the entry from an opaque copy of its word, so that the common path
-            // keeps its one-instruction decode of the high half
-            uint32_t wq = w[2 * h + (u >> 1)];
-            asm volatile("" : "+v"(wq));
-            cq[qn * kBlock] = (u & 1) ? wq >> 16 : wq & 0xffffu;
-            ++qn;
-          }
         } else if (h == 0 && u == 0) {
           acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
         } else {
