@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # <VortAcc<P32K>>) on this workload (skin 0.2 rho), from the FETCH_SIZE (x2.000)
 # and WRITE_SIZE (x1.000) counter passes calibrated against a 512 MiB device
 # copy, stages 1-3 and 4 weighted 3:1; profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-TRAFFIC_VERLET_FP32K = 250.3e6
+TRAFFIC_VERLET_FP32K = 240.6e6
 
 # algorithmic flop convention per RHS evaluation (T = sin/cos/sqrt/atan2 counted as 1)
 FLOP_CANDIDATE = 14      # 2 sub, min-image 2x4, r^2 3, compare 1
@@ -345,7 +345,7 @@ ALG_BYTES_PER_PARTICLE_STEP = 1108.0
 # VALU issue of the same kernel from the SQ counter pass (SQ_INSTS_VALU x 4
 # cycles over the 1024 SIMDs during the launch, stages weighted 3:1);
 # profiles/pair_kernel_vortex_rk4_n1m_fp32k_verlet.md.
-VALU_ISSUE_FRAC_VERLET_FP32K = 0.72
+VALU_ISSUE_FRAC_VERLET_FP32K = 0.74
 
 
 def main():
