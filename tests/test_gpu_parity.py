@@ -303,6 +303,60 @@ def test_lds_overflow_fallback_matches(gpu, prec, cap, monkeypatch):
     assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
 
 
+def clustered_rows_state(n, L, cell, seed=9):
+    """Uniform particles except two cell rows: 80% of row 5's particles moved into
+    row 10.  Row 10 then holds about 1.8x a row's mean, beyond the workgroups
+    its row chunks are sized for (mean + 4 sigma), so the row's last workgroup
+    takes the rest; row 5 keeps a few particles per cell, so 256 of its slots
+    span more than 64 cells and its chunk reads its tiles from global memory."""
+    st = O.make_state(seed, 0, n, L, 0.05)
+    y = st[1::3].copy()
+    row = np.floor(y / cell).astype(int)
+    idx = np.flatnonzero(row == 5)
+    move = idx[: int(0.8 * idx.size)]
+    y[move] += 5 * cell
+    st[1::3] = y
+    return st
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+def test_cell_walk_overfull_and_sparse_rows(gpu, prec):
+    """Cell-stencil sums with an overfull and a sparse cell row against the
+    oracle: the row-chunk workgroups' overflow and global-memory paths."""
+    n, rho = 160000, 0.05
+    L = math.sqrt(n / 4000.0)
+    nc = O.num_cells(L, rho)
+    st = clustered_rows_state(n, L, L / nc)
+    rows = np.bincount(np.floor(st[1::3] / (L / nc)).astype(int), minlength=nc)
+    mean = n / nc
+    assert rows[10] > mean + 4 * math.sqrt(mean) + 8 + 256 and rows[5] < 0.25 * mean, (rows[5], rows[10], mean)
+    kp = O.uniform(72, 3 * n, -1.0, 1.0)
+    kn = O.uniform(73, 3 * n, 5.0, 6.0)
+    ref, rc, ts = O.rhs_k(orc_vortex(n, L), st, kp, 0.5, 0.01, kn, term_scale=True)
+    with spc2.Sim(vort_params(n, L, precision=prec, neighbour="cells")) as s:
+        s.set_state(st)
+        s.set_counting(True)
+        out = s.eval_rhs(kp, 0.5, 0.01, kn)
+        cnt = s.counts()
+    assert_rhs_terms(out, ref, ts, 0.025, TOL[prec], f"vortex clustered/{prec}", ref[2::3] == kn[2::3])
+    assert np.array_equal(cnt, rc), (cnt, rc)
+    # chimera (rho 0.1: 63 cells per row, so only the overfull row's path)
+    v0 = 0.3
+    sysc = O.chimera(n, L, 0.1, v0, 1.2, 1.4, 0.05, threads=8)
+    d0, a0 = np.zeros(3 * n), np.zeros(2 * n)
+    rd, ra, _ = O.rhs_ito(sysc, st, d0, a0)
+    _, _, ts = O.rhs_k(sysc, st, np.zeros(3 * n), 0.0, 0.0, d0, term_scale=True)
+    with spc2.Sim(chim_params(n, L, 0.1, v0, 1.2, 1.4, 0.05, prec, neighbour="cells")) as s:
+        s.set_state(st)
+        gd, ga = s.eval_rhs_ito(d0, a0)
+    tol = TOL[prec]
+    assert_rhs_terms(gd, rd, ts, v0, tol, f"chimera clustered/{prec} derivative", rd[2::3] == d0[2::3])
+    for c, what in ((0, "add0"), (1, "add1")):
+        scale = np.abs(ra[c::2]).max()
+        err = np.abs(ga[c::2] - ra[c::2]).max()
+        assert err <= tol * scale, f"chimera clustered/{prec} {what}: {err:.3e} > {tol} x {scale:.3e}"
+
+
 # ---- steps with the reference's own noise ----------------------------------------
 
 @pytest.mark.parametrize("prec", ["fp64", "fp64f", "fp32k"])
