@@ -206,12 +206,21 @@ __device__ __forceinline__ void cell_run(Acc &acc, const Src32 &src, int a, int 
                                          const KParams &P) {
   acc.cell_begin();
   if constexpr (HasPair2<Acc>::value) {
-    int m = a;
-    for (; m + 1 < b; m += 2) {
-      const float oxa = SELF && m == self ? kSelfOff : ox, oxb = SELF && m + 1 == self ? kSelfOff : ox;
-      acc.pair2_x(src, m, oxa, oxb, P);
+    // the pairs walked by tile address, so that the loop keeps one induction
+    // variable, then the odd last candidate
+    const float4 *t2 = src.f4 + a, *const last = src.f4 + (b - 1), *const own = src.f4 + self;
+    if constexpr (std::is_same<Acc, Chim2Acc<P32K>>::value) {  // reads aj[m]: the index too
+      for (int m = a; t2 < last; t2 += 2, m += 2) {
+        const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
+        acc.pair2_x(src, t2, m, oxa, oxb, P);
+      }
+    } else {
+      for (; t2 < last; t2 += 2) {
+        const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
+        acc.pair2_x(src, t2, oxa, oxb, P);
+      }
     }
-    if (m < b) acc.template pair_x<SUM_GROUP>(src, m, SELF && m == self ? kSelfOff : ox, P);
+    if ((b - a) & 1) acc.template pair_x<SUM_GROUP>(src, b - 1, SELF && b - 1 == self ? kSelfOff : ox, P);
     acc.flush_cell();
   } else {
 #pragma unroll 2
