@@ -25,11 +25,14 @@ inline int blocks_for(long long n, int b = kBlock) { return (int)((n + b - 1) / 
 // (k_coef * dt) * k_prev like the reference); on the linked-cell path the
 // position is wrapped if outside the box and binned (ChimeraSystem.cpp:291-309).
 // `local` is the arrival rank inside the cell, used only to place the particle
-// before the stable re-ranking by particle id.
+// before the stable re-ranking by particle id.  apf4 (the FP32 form on the
+// all-pairs path): each particle's tile, whose cos and sin of the stage phase
+// the pair sums read instead of an FP64 sincos per pair (the same values).
 __global__ __launch_bounds__(kBlock) void k_stage_bin(int n, int linked, Box b, int nx, int ny, const double *sx,
                                                       const double *sy, const double *sp, const double *kx,
                                                       const double *ky, const double *kp, double kcdt, double *gx,
-                                                      double *gy, double *gp, int *cell_of, int *count, int *local) {
+                                                      double *gy, double *gp, int *cell_of, int *count, int *local,
+                                                      float4 *apf4) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = h < n;
   int c = 0;
@@ -48,6 +51,11 @@ __global__ __launch_bounds__(kBlock) void k_stage_bin(int n, int linked, Box b, 
     gx[h] = x;
     gy[h] = y;
     gp[h] = p;
+    if (apf4) {
+      double sn, cs;
+      sincos(p, &sn, &cs);
+      apf4[h] = make_float4((float)x, (float)y, (float)cs, (float)sn);
+    }
   }
   if (linked) {
     int r = wave_aggregated_rank(count, c, valid);
@@ -190,7 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_rank_tiles(int n, int nx, double hx,
 void launch_stage_bin(Sim &s, const double *kx, const double *ky, const double *kp, double kcdt) {
   Box b{s.kp.Lx, s.kp.Ly, s.kp.rLx, s.kp.rLy};
   hipLaunchKernelGGL(k_stage_bin, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.stream, s.n, (int)s.linked, b, s.nx,
-                     s.ny, s.sx, s.sy, s.sp, kx, ky, kp, kcdt, s.gx, s.gy, s.gp, s.cell_of, s.count, s.local);
+                     s.ny, s.sx, s.sy, s.sp, kx, ky, kp, kcdt, s.gx, s.gy, s.gp, s.cell_of, s.count, s.local,
+                     !s.linked && s.prec == P32K ? s.t.f4 : nullptr);
 }
 
 int scan_blocks(int nc) { return (nc + kScanTile - 1) / kScanTile; }

@@ -47,12 +47,13 @@ __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__re
 
 template <int PREC, int SYS, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass1_allpairs(KParams P, const double *gx, const double *gy,
-                                                           const double *gp, EpiCtx e, unsigned long long *counts) {
+                                                           const double *gp, const float4 *tiles, EpiCtx e,
+                                                           unsigned long long *counts) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < P.n;
   Pass1Out o{};
   if (active) {
-    allpairs_pass1<PREC, SYS>(P, i, gx, gy, gp, o);
+    allpairs_pass1<PREC, SYS>(P, i, gx, gy, gp, o, tiles);
     epilogue1<EPI>(P, e, i, i, i, o);  // all pairs: home order is the id order
   }
   if (counts) count_add(counts, active ? P.n - 1 : 0, active ? o.n : 0, active ? o.n2 : 0);
@@ -91,8 +92,10 @@ void pass1_typed(Sim &s, const EpiCtx &e) {
       launch_pair(s, k_pass1_lds<VortAcc<PREC>, PREC, EPI>, grid, blk, sh, s.kp, (const int *)s.start, s.t,
                   (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e, cnt);
   } else {
+    // the FP32 form reads cos and sin of phi_j from the stage's tiles
     launch_pair(s, k_pass1_allpairs<PREC, SYS, EPI>, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.kp,
-                (const double *)s.gx, (const double *)s.gy, (const double *)s.gp, e, cnt);
+                (const double *)s.gx, (const double *)s.gy, (const double *)s.gp,
+                PREC == P32K ? (const float4 *)s.t.f4 : nullptr, e, cnt);
   }
 }
 
