@@ -142,7 +142,8 @@ __device__ __forceinline__ EpiCtx rep_ctx(const EpiCtx &e, const Rep &R, int k) 
   return c;
 }
 
-// All-pairs replicas (rho > L/4): thread per particle; their homes are in id
+// All-pairs replicas (rho > L/4): a thread per particle (the FP32 chimera
+// pass 1: a group of allpairs_lanes lanes per particle); their homes are in id
 // order (one cell, sorted by id), so a particle's slot is its home.  Block ab
 // of the all-pairs blocks.
 template <int PREC, int EPI>
@@ -151,17 +152,21 @@ __device__ __forceinline__ void b_allpairs1(const Rep *reps, const int *ap_rep, 
                                             int k) {
   const Rep &R = reps[ap_rep[ab]];
   if (k >= R.steps) return;
-  const int i = (ab - R.apoff) * kB + threadIdx.x;
-  if (i >= R.n) return;
+  const int G = allpairs_lanes(PREC, R.system);  // lanes per particle, as a single handle's
+  const int t = (ab - R.apoff) * kB + threadIdx.x, i = t / G, g = t % G;
+  if (i >= R.n) return;  // whole lane groups
   const KParams P = R.kp;
   const EpiCtx c = rep_ctx(e, R, k);
   Pass1Out o{};
-  if (R.system == SPC2_CHIMERA)  // the replica's one cell holds its particles in id order: tile j is id j
-    allpairs_pass1<PREC, SPC2_CHIMERA>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o,
-                                       PREC == P32K ? f4 + R.poff : nullptr);
-  else
+  if (R.system == SPC2_CHIMERA) {  // the replica's one cell holds its particles in id order: tile j is id j
+    if constexpr (PREC == P32K)
+      allpairs_pass1_group(P, i, g, gx + R.poff, gy + R.poff, gp + R.poff, f4 + R.poff, o);
+    else
+      allpairs_pass1<PREC, SPC2_CHIMERA>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
+  } else {
     allpairs_pass1<PREC, SPC2_VORTEX>(P, i, gx + R.poff, gy + R.poff, gp + R.poff, o);
-  epilogue1<EPI>(P, c, R.poff + i, i, R.poff + i, o);
+  }
+  if (g == 0) epilogue1<EPI>(P, c, R.poff + i, i, R.poff + i, o);
 }
 
 __device__ __forceinline__ void b_allpairs2_srk(const Rep *reps, const int *ap_rep, int ab, const double *gx,
@@ -442,7 +447,7 @@ spc2_status layout(spc2_batch &b, std::vector<int> &seg_rep, std::vector<int> &a
     } else {
       R.soff = R.nseg = 0;
       R.apoff = apoff;
-      R.apblocks = bfor(R.n);
+      R.apblocks = bfor((long long)R.n * allpairs_lanes(b.prec, p.system));  // pass 2 uses the first bfor(n)
       for (int q = 0; q < R.apblocks; ++q) ap_rep.push_back(r);
       apoff += R.apblocks;
       coff += 1;

@@ -1394,6 +1394,59 @@ void launch_pair(Sim &s, void (*kern)(KArgs...), dim3 grid, dim3 blk, size_t sh,
   }
 }
 
+// Lanes per particle of the FP32 chimera all-pairs pass (allpairs_group_lanes):
+// a group of kApLanes consecutive lanes shares one particle, lane g summing
+// partners j = g, g + kApLanes, ...; small all-pairs systems have too few
+// particles to fill the device one lane each.
+constexpr int kApLanes = 16;
+template <int PREC, int SYS>
+constexpr int allpairs_lanes() { return PREC == P32K && SYS == SPC2_CHIMERA ? kApLanes : 1; }
+__host__ __device__ inline int allpairs_lanes(int prec, int sys) {
+  return prec == P32K && sys == SPC2_CHIMERA ? kApLanes : 1;
+}
+
+// The FP32 chimera all-pairs pass 1 of particle i by its group's lane g (every
+// lane of the group calls it): each lane's Kahan sums over its partners, in
+// ascending j, then the lanes' FP64 values and counts added across the group in
+// a fixed butterfly (the same total in every lane).  Geometry and the cos, sin
+// of phi_j as allpairs_pass1 forms them.
+__device__ __forceinline__ void allpairs_pass1_group(const KParams &P, int i, int g, const double *gx,
+                                                     const double *gy, const double *gp, const float4 *tiles,
+                                                     Pass1Out &o) {
+  const double xi = gx[i], yi = gy[i], pi = gp[i];
+  Kahan Sf, Cf;
+  Sf.init();
+  Cf.init();
+  int n = 0;
+  for (int j = g; j < P.n; j += kApLanes) {
+    if (j == i) continue;
+    double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+    double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if (r2 <= P.rho2) {
+      const float4 tj = tiles[j];
+      Sf.add(tj.w);
+      Cf.add(tj.z);
+      ++n;
+    }
+  }
+  double S = Sf.value(), C = Cf.value();
+#pragma unroll
+  for (int d = 1; d < kApLanes; d <<= 1) {
+    S += __shfl_xor(S, d, kApLanes);
+    C += __shfl_xor(C, d, kApLanes);
+    n += __shfl_xor(n, d, kApLanes);
+  }
+  double sa, ca;
+  sincos(pi + P.alpha, &sa, &ca);
+  o.n = n;
+  o.written = n != 0;
+  o.aphi = n != 0 ? P.sigma * (ca * S - sa * C) / (double)n : 0.0;
+  o.n2 = 0;
+  o.vortex_add = 0;
+  o.phase = pi;
+}
+
 // All pairs (rho > L/4): thread per particle, j = 0..N-1 ascending, stage
 // positions not wrapped (ChimeraSystem.cpp:131-167, :184-254); arrays by
 // particle id.

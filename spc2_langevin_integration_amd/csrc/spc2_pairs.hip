@@ -41,7 +41,8 @@ __global__ __launch_bounds__(kBlock) void k_pass2_lds(KParams P, const int *__re
 }
 
 // =============================================================================
-// All pairs (rho > L/4): thread per particle, j = 0..N-1, stage positions not
+// All pairs (rho > L/4): thread per particle (the FP32 chimera pass 1: a group
+// of allpairs_lanes lanes per particle), j = 0..N-1, stage positions not
 // wrapped (ChimeraSystem.cpp:131-167, :184-254)
 // =============================================================================
 
@@ -49,14 +50,16 @@ template <int PREC, int SYS, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass1_allpairs(KParams P, const double *gx, const double *gy,
                                                            const double *gp, const float4 *tiles, EpiCtx e,
                                                            unsigned long long *counts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < P.n;
+  constexpr int G = allpairs_lanes<PREC, SYS>();  // lanes per particle
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, i = t / G, g = t % G;
+  const bool active = i < P.n, lead = active && g == 0;  // whole lane groups
   Pass1Out o{};
   if (active) {
-    allpairs_pass1<PREC, SYS>(P, i, gx, gy, gp, o, tiles);
-    epilogue1<EPI>(P, e, i, i, i, o);  // all pairs: home order is the id order
+    if constexpr (G > 1) allpairs_pass1_group(P, i, g, gx, gy, gp, tiles, o);
+    else allpairs_pass1<PREC, SYS>(P, i, gx, gy, gp, o, tiles);
+    if (lead) epilogue1<EPI>(P, e, i, i, i, o);  // all pairs: home order is the id order
   }
-  if (counts) count_add(counts, active ? P.n - 1 : 0, active ? o.n : 0, active ? o.n2 : 0);
+  if (counts) count_add(counts, lead ? P.n - 1 : 0, lead ? o.n : 0, lead ? o.n2 : 0);
 }
 
 template <int SYS, int EPI>
@@ -93,7 +96,8 @@ void pass1_typed(Sim &s, const EpiCtx &e) {
                   (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e, cnt);
   } else {
     // the FP32 form reads cos and sin of phi_j from the stage's tiles
-    launch_pair(s, k_pass1_allpairs<PREC, SYS, EPI>, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.kp,
+    launch_pair(s, k_pass1_allpairs<PREC, SYS, EPI>, dim3(blocks_for((long long)s.n * allpairs_lanes<PREC, SYS>())),
+                dim3(kBlock), 0, s.kp,
                 (const double *)s.gx, (const double *)s.gy, (const double *)s.gp,
                 PREC == P32K ? (const float4 *)s.t.f4 : nullptr, e, cnt);
   }
