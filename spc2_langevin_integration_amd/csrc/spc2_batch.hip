@@ -169,23 +169,29 @@ __device__ __forceinline__ void b_allpairs1(const Rep *reps, const int *ap_rep, 
   if (g == 0) epilogue1<EPI>(P, c, R.poff + i, i, R.poff + i, o);
 }
 
+template <int PREC>
 __device__ __forceinline__ void b_allpairs2_srk(const Rep *reps, const int *ap_rep, int ab, const double *gx,
                                                 const double *gy, const double *gp, const EpiCtx &e, int k) {
   const Rep &R = reps[ap_rep[ab]];
   if (k >= R.steps) return;
-  const int i = (ab - R.apoff) * kB + threadIdx.x;
-  if (i >= R.n) return;
+  const int G = allpairs_lanes(PREC, R.system);  // lanes per particle, as a single handle's
+  const int t = (ab - R.apoff) * kB + threadIdx.x, i = t / G, g = t % G;
+  if (i >= R.n) return;  // whole lane groups
   const KParams P = R.kp;
   const EpiCtx c = rep_ctx(e, R, k);
   const int o0 = R.poff;
   Pass2Out o;
-  if (R.system == SPC2_CHIMERA)
-    allpairs_pass2<SPC2_CHIMERA>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
-                                 0.0, 0.0, o);
-  else
+  if (R.system == SPC2_CHIMERA) {
+    if constexpr (PREC == P32K)
+      allpairs_pass2_group(P, i, g, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, 0.0, 0.0, o);
+    else
+      allpairs_pass2<SPC2_CHIMERA>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0,
+                                   e.n2_slot + o0, 0.0, 0.0, o);
+  } else {
     allpairs_pass2<SPC2_VORTEX>(P, i, gx + o0, gy + o0, gp + o0, e.aphi_slot + o0, e.n_slot + o0, e.n2_slot + o0,
                                 0.0, 0.0, o);
-  epilogue2<EPI_SRK>(P, c, o0 + i, i, o);
+  }
+  if (g == 0) epilogue2<EPI_SRK>(P, c, o0 + i, i, o);
 }
 
 // The all-pairs blocks lead the pass launches: the first ap.pad blocks (ap.n
@@ -229,7 +235,7 @@ __global__ __launch_bounds__(kB) void k_b_pass2_srk(const Rep *reps, const int *
                                                     ApBlocks ap) {
   extern __shared__ float4 dyn_lds[];
   if ((int)blockIdx.x < ap.pad) {
-    if ((int)blockIdx.x < ap.n) b_allpairs2_srk(reps, ap.rep, blockIdx.x, gx, gy, gp, e, k);
+    if ((int)blockIdx.x < ap.n) b_allpairs2_srk<PREC>(reps, ap.rep, blockIdx.x, gx, gy, gp, e, k);
     return;
   }
   const int sg = xcd_segment(blockIdx.x - ap.pad, gridDim.x - ap.pad);
@@ -447,7 +453,7 @@ spc2_status layout(spc2_batch &b, std::vector<int> &seg_rep, std::vector<int> &a
     } else {
       R.soff = R.nseg = 0;
       R.apoff = apoff;
-      R.apblocks = bfor((long long)R.n * allpairs_lanes(b.prec, p.system));  // pass 2 uses the first bfor(n)
+      R.apblocks = bfor((long long)R.n * allpairs_lanes(b.prec, p.system));
       for (int q = 0; q < R.apblocks; ++q) ap_rep.push_back(r);
       apoff += R.apblocks;
       coff += 1;

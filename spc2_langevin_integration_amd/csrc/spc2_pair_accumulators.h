@@ -1556,8 +1556,9 @@ __device__ __forceinline__ void allpairs_pass1(const KParams &P, int i, const do
 }
 
 
-// All-pairs pass 2 (rho > L/4) of particle i, arrays by particle id (FP64 form
-// for every precision: the small-N path); in0, in1: the additional values before
+// All-pairs pass 2 (rho > L/4) of particle i, arrays by particle id (FP64 terms
+// for every precision: the small-N path; the FP32 chimera form spreads them over
+// a lane group, allpairs_pass2_group); in0, in1: the additional values before
 // the call (zero for the SRK step).
 template <int SYS>
 __device__ __forceinline__ void allpairs_pass2(const KParams &P, int i, const double *gx, const double *gy,
@@ -1619,6 +1620,38 @@ __device__ __forceinline__ void allpairs_pass2(const KParams &P, int i, const do
   o.ai = ai;
   o.a0 = a0 * -sqrt(2.0 * P.D);
   o.a1 = a1 + -2.0 * P.D * ai;
+}
+
+// Chimera all-pairs pass 2 of particle i by lane g of its group (the FP32
+// form, allpairs_lanes lanes per particle; every lane of the group calls it):
+// allpairs_pass2's FP64 terms, each lane's over its partners j = g, g + G, ...,
+// added across the group in a fixed butterfly and then to in0, in1.
+__device__ __forceinline__ void allpairs_pass2_group(const KParams &P, int i, int g, const double *gx,
+                                                     const double *gy, const double *gp, const double *aphi,
+                                                     const double *nslot, double in0, double in1, Pass2Out &o) {
+  const double xi = gx[i], yi = gy[i], pi = gp[i];
+  const double ai = aphi[i], n = nslot[i];
+  double s0 = 0.0, s1 = 0.0;
+  for (int j = g; j < P.n; j += kApLanes) {
+    if (j == i) continue;
+    double dx = min_image(gx[j] - xi, P.Lx, P.rLx);
+    double dy = min_image(gy[j] - yi, P.Ly, P.rLy);
+    double r2 = dx * dx + dy * dy;
+    if ((r2 <= P.rho2) && (n != 0)) {
+      double cph = cos(gp[j] - pi - P.alpha);
+      s0 += cph / n;
+      s1 += (aphi[j] - ai) * cph / n;
+    }
+  }
+#pragma unroll
+  for (int d = 1; d < kApLanes; d <<= 1) {
+    s0 += __shfl_xor(s0, d, kApLanes);
+    s1 += __shfl_xor(s1, d, kApLanes);
+  }
+  o.phase = pi;
+  o.ai = ai;
+  o.a0 = (in0 + s0) * -sqrt(2.0 * P.D);
+  o.a1 = (in1 + s1) + -2.0 * P.D * ai;
 }
 
 }  // namespace

@@ -62,15 +62,19 @@ __global__ __launch_bounds__(kBlock) void k_pass1_allpairs(KParams P, const doub
   if (counts) count_add(counts, lead ? P.n - 1 : 0, lead ? o.n : 0, lead ? o.n2 : 0);
 }
 
-template <int SYS, int EPI>
+template <int PREC, int SYS, int EPI>
 __global__ __launch_bounds__(kBlock) void k_pass2_allpairs(KParams P, const double *gx, const double *gy,
                                                            const double *gp, EpiCtx e) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  constexpr int G = allpairs_lanes<PREC, SYS>();  // lanes per particle
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, i = t / G, g = t % G;
+  if (i >= P.n) return;  // whole lane groups
   Pass2Out o;
-  allpairs_pass2<SYS>(P, i, gx, gy, gp, e.aphi_slot, e.n_slot, e.n2_slot, EPI == EPI_ITO2 ? e.a0[i] : 0.0,
-                      EPI == EPI_ITO2 ? e.a1[i] : 0.0, o);
-  epilogue2<EPI>(P, e, i, i, o);
+  const double in0 = EPI == EPI_ITO2 ? e.a0[i] : 0.0, in1 = EPI == EPI_ITO2 ? e.a1[i] : 0.0;
+  if constexpr (G > 1)
+    allpairs_pass2_group(P, i, g, gx, gy, gp, e.aphi_slot, e.n_slot, in0, in1, o);
+  else
+    allpairs_pass2<SYS>(P, i, gx, gy, gp, e.aphi_slot, e.n_slot, e.n2_slot, in0, in1, o);
+  if (g == 0) epilogue2<EPI>(P, e, i, i, o);
 }
 
 // =============================================================================
@@ -133,7 +137,8 @@ void pass2_typed(Sim &s, const EpiCtx &e) {
       launch_pair(s, k_pass2_lds<Vort2Acc<PREC>, PREC, EPI>, grid, blk, sh, s.kp, (const int *)s.start, s.t,
                   (const int *)s.slot_h, (const int *)s.slot_id, (const double *)s.gp, e);
   } else {
-    launch_pair(s, k_pass2_allpairs<SYS, EPI>, dim3(blocks_for(s.n)), dim3(kBlock), 0, s.kp, (const double *)s.gx,
+    launch_pair(s, k_pass2_allpairs<PREC, SYS, EPI>, dim3(blocks_for((long long)s.n * allpairs_lanes<PREC, SYS>())),
+                dim3(kBlock), 0, s.kp, (const double *)s.gx,
                 (const double *)s.gy, (const double *)s.gp, e);
   }
 }
