@@ -193,6 +193,42 @@ def test_batch_matches_single_handles(gpu, prec):
             assert np.array_equal(frames[r], got.astype(np.float32))
 
 
+def test_batch_clustered_rows_equal_single_handles(gpu):
+    """Replicas with an overfull and a sparse cell row (their row chunks'
+    overflow workgroup and global-memory path): bit-identical to single
+    handles, and the first step within the bars of the oracle's step_se."""
+    n = 160000
+    L = math.sqrt(n / 4000.0)
+    pv = spc2.params("vortex", "fp32k", n=n, L=L, rho=0.05, velocity=0.025, mu_plus=100.0, mu_minus=100.0,
+                     xi_a=0.01, xi_r=0.005, kappa=20.0, D=0.01, seed=5)
+    pc = spc2.params("chimera", "fp32k", n=n, L=L, rho=0.1, v0=0.01, sigma=1.0, alpha=1.45, D=0.01, seed=6)
+    cell = L / O.num_cells(L, 0.05)
+    st = O.make_state(9, 0, n, L, 0.05)
+    y = st[1::3].copy()
+    idx = np.flatnonzero(np.floor(y / cell).astype(int) == 5)
+    y[idx[: int(0.8 * idx.size)]] += 5 * cell  # row 5 thinned into row 10
+    st[1::3] = y
+    dt, steps = 0.01, 3
+    with spc2.Batch([pv, pc]) as b:
+        b.init_random_state(0.0)
+        for r in range(2):
+            b.set_state(r, st)
+        b.step(dt, 1)
+        first = [b.get_state(r) for r in range(2)]
+        b.step(dt, steps - 1)
+        got = [b.get_state(r) for r in range(2)]
+    for r, p in enumerate((pv, pc)):
+        with spc2.Sim(p) as s:
+            s.init_random_state(0.0)
+            s.set_state(st)
+            s.step("se", dt, 0, steps)
+            assert np.array_equal(s.get_state(), got[r]), f"replica {r}"
+        chim = p.system == spc2.CHIMERA
+        ref = O.step_se(oracle_sys(p), st, dt, O.philox_normals(p.seed, 0, 0, n, 1),
+                        omega=np.zeros(n) if chim else None)
+        assert_state(first[r], ref, st, L, TOL["fp32k"], f"clustered replica {r}")
+
+
 def test_batch_limits_and_repacking(gpu):
     """Per-replica step limits, then a re-pack that drops, reorders and adds
     replicas; carried replicas keep state, frequencies and step counters."""
