@@ -1,15 +1,15 @@
-// Row-segment LDS staging of the 3 x 3 cell stencil and the per-segment
-// pass-1 / pass-2 bodies, shared by the single-system kernels
-// (spc2_pairs.hip) and the batched kernels (spc2_batch.hip).
+// LDS staging of the 3 x 3 cell stencil and the pass-1 / pass-2 bodies of one
+// pair workgroup, shared by the single-system kernels (spc2_pairs.hip) and the
+// batched kernels (spc2_batch.hip).
 //
 // Work decomposition: one 256-lane workgroup per row chunk (a run of
 // consecutive slots of one cell row, which spans W cells) or, in sparse
 // systems, per row segment of W cells.  It stages the 3 x (W + 2) halo cells'
 // tiles in LDS with coalesced reads (cells of a row are contiguous in the
-// cell-sorted arrays), then each lane takes one centre particle.  In LDS the three stencil cells of one stencil
-// row are adjacent, so a particle visits its 9 cells as 3 contiguous ranges,
-// in the reference order (cells (-1,dy), (0,dy), (1,dy), ascending particle
-// index inside a cell).  A workgroup whose halo does not fit the LDS capacity
+// cell-sorted arrays), then each lane takes one centre particle.  In LDS the
+// three stencil cells of one stencil row are adjacent, so a particle visits its
+// 9 cells as 3 contiguous ranges, in the reference order (cells (-1,dy),
+// (0,dy), (1,dy), ascending particle index inside a cell).  A workgroup whose halo does not fit the LDS capacity
 // falls back to reading the same tiles from global memory with the same
 // arithmetic and order.  Compiled with -ffp-contract=off.
 #pragma once
