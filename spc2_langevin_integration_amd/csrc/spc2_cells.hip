@@ -135,6 +135,40 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(int *count, int nc, const
   }
 }
 
+// K2 for at most kScanSingle counts: the whole exclusive scan in one
+// workgroup, tile by tile with a carried total (the three-launch scan's values),
+// writing start[nc] and clearing the histogram.
+constexpr int kScanSingle = 4 * kScanTile;
+__global__ __launch_bounds__(kBlock) void k_scan_single(int *count, int nc, int *start) {
+  __shared__ int wtot[kBlock / 64];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int tile = 0; tile < nc; tile += kScanTile) {
+    const int base = tile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+      v[q] = (base + q < nc) ? count[base + q] : 0;
+      s += v[q];
+    }
+    int run = carry + block_inclusive_scan(s, wtot) - s;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+      if (base + q < nc) {
+        start[base + q] = run;
+        count[base + q] = 0;
+      }
+      run += v[q];
+    }
+    __syncthreads();  // every lane has read carry
+    if (threadIdx.x == kBlock - 1) carry = run;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[nc] = carry;
+}
+
 // K3a: place each particle at cell start + arrival rank (home and id).
 __global__ __launch_bounds__(kBlock) void k_place(int n, const int *cell_of, const int *local, const int *start,
                                                   const int *hid, int *tmp, int *tmpid) {
@@ -207,6 +241,10 @@ int scan_blocks(int nc) { return (nc + kScanTile - 1) / kScanTile; }
 // Exclusive scan of count[0..m) into out[0..m], out[m] = total; clears count.
 // bsum holds scan_blocks(m) + 1 ints.
 void launch_scan_raw(hipStream_t st, int *bsum, int *count, int m, int *out) {
+  if (m <= kScanSingle) {  // small grids: one launch instead of three
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, count, m, out);
+    return;
+  }
   const int nb = scan_blocks(m);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, st, count, m, bsum);
   hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kBlock), 0, st, bsum, nb, out, m);
