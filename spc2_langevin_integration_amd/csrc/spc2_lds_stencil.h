@@ -208,17 +208,29 @@ __device__ __forceinline__ void cell_run(Acc &acc, const Src32 &src, int a, int 
   acc.cell_begin();
   if constexpr (HasPair2<Acc>::value) {
     // the pairs walked by tile address, so that the loop keeps one induction
-    // variable, then the odd last candidate
+    // variable, each pair's tiles requested a pair ahead; then the odd last
+    // candidate.  The read ahead of the last pair stays within the workgroup's
+    // LDS allocation (at most two tiles past the staged halo, in the arrays
+    // that follow it) and is not used.
     const float4 *t2 = src.f4 + a, *const last = src.f4 + (b - 1), *const own = src.f4 + self;
-    if constexpr (std::is_same<Acc, Chim2Acc<P32K>>::value) {  // reads aj[m]: the index too
-      for (int m = a; t2 < last; t2 += 2, m += 2) {
-        const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
-        acc.pair2_x(src, t2, m, oxa, oxb, P);
-      }
-    } else {
+    if constexpr (std::is_same<Acc, VortAcc<P32K>>::value) {  // its longer pair step hides the latency
       for (; t2 < last; t2 += 2) {
         const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
-        acc.pair2_x(src, t2, oxa, oxb, P);
+        acc.pair2_x(src, t2, t2[0], t2[1], oxa, oxb, P);
+      }
+    } else if (t2 < last) {
+      float4 ta = t2[0], tb = t2[1];
+      for (int m = a;; m += 2) {
+        const float4 na = t2[2], nb = t2[3];
+        const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
+        if constexpr (std::is_same<Acc, Chim2Acc<P32K>>::value)  // reads aj[m]: the index too
+          acc.pair2_x(src, t2, m, ta, tb, oxa, oxb, P);
+        else
+          acc.pair2_x(src, t2, ta, tb, oxa, oxb, P);
+        t2 += 2;
+        if (!(t2 < last)) break;
+        ta = na;
+        tb = nb;
       }
     }
     if ((b - a) & 1) acc.template pair_x<SUM_GROUP>(src, b - 1, SELF && b - 1 == self ? kSelfOff : ox, P);

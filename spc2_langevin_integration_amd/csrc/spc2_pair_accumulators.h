@@ -306,13 +306,13 @@ struct ChimAcc<P32K> {
     flush();
     n += (int)pn;
   }
-  // Two consecutive cell candidates, the staged tiles at t2 (LDS indices m =
-  // t2 - s.f4 and m + 1; column offsets oxa, oxb), into the cell's partials,
+  // Two consecutive cell candidates, the staged tiles a, b read from t2 (LDS
+  // indices m = t2 - s.f4 and m + 1; column offsets oxa, oxb), into the cell's partials,
   // with one branch for both band re-checks.  An in-range
   // flag f in {0, 1} enters as fma(f, x, p): p + x when in range and p + 0
   // otherwise, the same values as pair_x<SUM_GROUP> for each in turn.
-  __device__ __forceinline__ void pair2_x(const Src32 &s, const float4 *t2, float oxa, float oxb, const KParams &P) {
-    const float4 a = t2[0], b = t2[1];
+  __device__ __forceinline__ void pair2_x(const Src32 &s, const float4 *t2, const float4 &a, const float4 &b,
+                                          float oxa, float oxb, const KParams &P) {
     const float dxa = a.x + oxa, dya = a.y + co.oy, dxb = b.x + oxb, dyb = b.y + co.oy;
     const float r2a = __builtin_fmaf(dxa, dxa, dya * dya), r2b = __builtin_fmaf(dxb, dxb, dyb * dyb);
     bool ina = r2a <= P.rho2f_hi, inb = r2b <= P.rho2f_hi;
@@ -561,13 +561,13 @@ struct VortAcc<P32K> {
     group_add_prod<MODE>(pcw, w, v.z);
     n01 += inr;
   }
-  // Two consecutive cell candidates, the staged tiles at t2 (LDS indices m =
-  // t2 - s.f4 and m + 1; column offsets oxa, oxb), with one branch for both rho
+  // Two consecutive cell candidates, the staged tiles a, b read from t2 (LDS
+  // indices m = t2 - s.f4 and m + 1; column offsets oxa, oxb), with one branch for both rho
   // band re-checks and one for both cohesion tests.
   // The terms, counts and cohesion sums are those of pair_x<SUM_GROUP> for
   // each in turn (the W partials and the cohesion sums are separate chains).
-  __device__ __forceinline__ void pair2_x(const Src32 &s, const float4 *t2, float oxa, float oxb, const KParams &P) {
-    const float4 a = t2[0], b = t2[1];
+  __device__ __forceinline__ void pair2_x(const Src32 &s, const float4 *t2, const float4 &a, const float4 &b,
+                                          float oxa, float oxb, const KParams &P) {
     const float dxa = a.x + oxa, dya = a.y + co.oy, dxb = b.x + oxb, dyb = b.y + co.oy;
     const float r2a = __builtin_fmaf(dxa, dxa, dya * dya), r2b = __builtin_fmaf(dxb, dxb, dyb * dyb);
     bool ina = r2a <= P.rho2f_hi, inb = r2b <= P.rho2f_hi;
@@ -833,13 +833,13 @@ struct Chim2Acc<P32K> {
     Qs.add(pqs);
   }
   __device__ __forceinline__ void flush_cell() { flush(); }
-  // Two consecutive cell candidates (tiles at t2, LDS index m = t2 - s.f4), as
+  // Two consecutive cell candidates (tiles a, b read from t2, LDS index m =
+  // t2 - s.f4), as
   // ChimAcc<P32K>::pair2_x: the in-range
   // flag f and f * aj enter as fma factors, the same values as pair_x<SUM_GROUP>
   // for each in turn (the halo's pass-1 derivatives are finite).
-  __device__ __forceinline__ void pair2_x(const Src32 &s, const float4 *t2, int m, float oxa, float oxb,
-                                          const KParams &P) {
-    const float4 a = t2[0], b = t2[1];
+  __device__ __forceinline__ void pair2_x(const Src32 &s, const float4 *t2, int m, const float4 &a, const float4 &b,
+                                          float oxa, float oxb, const KParams &P) {
     const float dxa = a.x + oxa, dya = a.y + co.oy, dxb = b.x + oxb, dyb = b.y + co.oy;
     const float r2a = __builtin_fmaf(dxa, dxa, dya * dya), r2b = __builtin_fmaf(dxb, dxb, dyb * dyb);
     bool ina = r2a <= P.rho2f_hi, inb = r2b <= P.rho2f_hi;
