@@ -476,7 +476,6 @@ struct VortAcc<P32K> {
   ColOffsets co;
   Kahan SW, CW, Tx, Ty;
   float psw, pcw;    // the current list group's partial sums
-  float pn;          // pair2_x's within-rho count in the current cell
   float mu_pv, wav;  // mu_pf and w_af in vector registers
   int n01, n2, k;
   const double *tx, *ty;
@@ -490,7 +489,7 @@ struct VortAcc<P32K> {
     CW.init();
     Tx.init();
     Ty.init();
-    psw = pcw = pn = 0.0f;
+    psw = pcw = 0.0f;
     mu_pv = in_vgpr(P.mu_pf);
     wav = in_vgpr(P.w_af);
     n01 = n2 = 0;
@@ -517,7 +516,7 @@ struct VortAcc<P32K> {
     pair_d<MODE>(s, m, v, v.x + ox, v.y + co.oy, P);
   }
   // the cell walks' partial sums over one stencil cell start at zero
-  __device__ __forceinline__ void cell_begin() { psw = pcw = pn = 0.0f; }
+  __device__ __forceinline__ void cell_begin() { psw = pcw = 0.0f; }
   // a list entry with the partner's tile already loaded (aj: pass-2 input,
   // unused here).  DEFER: the cohesion term is left to the caller, who gets
   // whether the partner lies within xi_r's band and calls cohesion() later, with
@@ -531,11 +530,7 @@ struct VortAcc<P32K> {
     SW.add(psw);
     CW.add(pcw);
   }
-  // flush() after a cell walked with pair2_x, with its count
-  __device__ __forceinline__ void flush_cell() {
-    flush();
-    n01 += (int)pn;
-  }
+  __device__ __forceinline__ void flush_cell() { flush(); }
   // W(r) of an FP32 r^2, masked to zero outside rho
   __device__ __forceinline__ float weight(float r2, bool inr, const KParams &P) const {
     float r = __builtin_amdgcn_sqrtf(r2);
@@ -582,7 +577,7 @@ struct VortAcc<P32K> {
     pcw = __builtin_fmaf(wa, a.z, pcw);
     psw = __builtin_fmaf(wb, b.w, psw);
     pcw = __builtin_fmaf(wb, b.z, pcw);
-    pn += (ina ? 1.0f : 0.0f) + (inb ? 1.0f : 0.0f);
+    n01 += (int)ina + (int)inb;
     const bool ca = r2a <= P.xr2f_hi, cb = r2b <= P.xr2f_hi;
     if (ca | cb) {
       const int m = (int)(t2 - s.f4);
