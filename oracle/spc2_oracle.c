@@ -15,10 +15,12 @@
  *    by the reference's own classes (oracle/ref_harness.cpp);
  *  - mt19937 + std::normal_distribution replay: bit-exact against the noise
  *    the reference consumed;
- *  - vortex RHS (the reference has it only in MPI translation units, which
- *    need MPICH and are not built): brute force on small N, W(r) closed forms
- *    and continuity, the cohesion atan2 identity, phi-shift invariance, and
- *    the two-body analytic values;
+ *  - vortex RHS and the *Ptr steppers (the reference has them only in its MPI
+ *    shared-memory translation units): bit-exact against golden vectors
+ *    written by those classes under mpiexec (oracle/ref_harness_mpi.cpp, at
+ *    1, 2 and 4 ranks); besides, brute force on small N, W(r) closed forms and
+ *    continuity, the cohesion atan2 identity, phi-shift invariance, and the
+ *    two-body analytic values;
  *  - Philox4x32-10: published known-answer vectors;
  *  - order parameter: closed forms (all phases equal, equispaced phases).
  */
