@@ -139,6 +139,7 @@ __device__ __forceinline__ EpiCtx rep_ctx(const EpiCtx &e, const Rep &R, int k) 
   c.sa.omega = R.system == SPC2_CHIMERA ? e.sa.omega : nullptr;  // no natural frequency in the vortex drift
   c.sa.D = R.D;
   c.sa.h1 = R.h1;
+  c.bcoff = R.coff;
   return c;
 }
 
@@ -530,12 +531,19 @@ spc2_status ensure_io(spc2_batch &b, size_t count) {
   return SPC2_OK;
 }
 
+// One step of every replica.  staged: the previous step's state-writing pass
+// already staged and binned this step's state (gx, gy, gp, cell_of, local and
+// the cell counts), so the stage-and-bin launch is skipped.  fuse: this step's
+// state-writing pass stages and bins the next step's state into spare[0..2]
+// (the caller then swaps them with gx, gy, gp); every replica must take the
+// next step and no re-homing may come between.
 template <int PREC>
-void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk) {
+void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk, bool staged, bool fuse) {
   const dim3 g(bfor(b.nt)), blk(kB);
-  hipLaunchKernelGGL(k_b_stage_bin, g, blk, 0, b.stream, b.nt, (const Rep *)b.d_reps, (const int *)b.rep_of,
-                     (const double *)b.sx, (const double *)b.sy, (const double *)b.sp, b.gx, b.gy, b.gp, b.cell_of,
-                     b.count_, b.local);
+  if (!staged)
+    hipLaunchKernelGGL(k_b_stage_bin, g, blk, 0, b.stream, b.nt, (const Rep *)b.d_reps, (const int *)b.rep_of,
+                       (const double *)b.sx, (const double *)b.sy, (const double *)b.sp, b.gx, b.gy, b.gp,
+                       b.cell_of, b.count_, b.local);
   launch_scan_raw(b.stream, b.bsum, b.count_, b.nct, b.start);
   launch_place_raw(b.stream, b.nt, b.cell_of, b.local, b.start, b.hid, b.tmp, b.tmpid);
   hipLaunchKernelGGL(k_b_rank_tiles<PREC>, g, blk, 0, b.stream, b.nt, (const Rep *)b.d_reps, (const int *)b.rep_of,
@@ -544,11 +552,20 @@ void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk) {
   const ApBlocks ap{b.ap_rep, b.nap, (b.nap + 7) / 8 * 8};
   const dim3 gpass(ap.pad + b.nseg);
   if (gpass.x == 0) return;
+  EpiCtx es = e;  // the state-writing pass's context
+  if (fuse) {
+    es.bgx = b.spare[0];
+    es.bgy = b.spare[1];
+    es.bgp = b.spare[2];
+    es.bcell = b.cell_of;
+    es.bcount = b.count_;
+    es.blocal = b.local;
+  }
   if (!srk) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b_pass1<PREC, EPI_SE>), gpass, blk, b.lds, b.stream, (const Rep *)b.d_reps,
                        (const int *)b.seg_rep, (const int *)b.start, b.t, (const int *)b.slot_h,
-                       (const int *)b.slot_id, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, e, k,
-                       ap);
+                       (const int *)b.slot_id, (const double *)b.gx, (const double *)b.gy, (const double *)b.gp, es,
+                       k, ap);
     return;
   }
   // SRK (StochasticRungeKuttaStepperPtr.hpp:32-97): both passes read the same
@@ -561,7 +578,7 @@ void launch_step_kernels(spc2_batch &b, const EpiCtx &e, int k, bool srk) {
                      ap);
   hipLaunchKernelGGL(k_b_pass2_srk<PREC>, gpass, blk, b.lds, b.stream, (const Rep *)b.d_reps, (const int *)b.seg_rep,
                      (const int *)b.start, b.t, (const int *)b.slot_h, (const int *)b.slot_id, (const double *)b.gx,
-                     (const double *)b.gy, (const double *)b.gp, e, k, ap);
+                     (const double *)b.gy, (const double *)b.gp, es, k, ap);
 }
 
 void launch_rehome(spc2_batch &b) {
@@ -584,7 +601,7 @@ spc2_status run_steps(spc2_batch &b, int method, double dt, int64_t nsteps, cons
     B_TRY(balloc(&b.n2_slot, n));
     B_TRY(balloc(&b.aphi_slot_f, n));
   }
-  int64_t kmax = 0;
+  int64_t kmax = 0, kmin = std::numeric_limits<int64_t>::max();
   for (int r = 0; r < b.count; ++r) {
     Rep &R = b.reps[r];
     R.step = b.steps_done[r];
@@ -595,6 +612,7 @@ spc2_status run_steps(spc2_batch &b, int method, double dt, int64_t nsteps, cons
     R.D = b.params[r].D;
     R.h1 = 0.5 * dt * std::sqrt(dt);
     kmax = std::max(kmax, R.steps);
+    kmin = std::min(kmin, R.steps);
   }
   B_TRY(hipMemcpyAsync(b.d_reps, b.reps.data(), sizeof(Rep) * b.reps.size(), hipMemcpyHostToDevice, b.stream));
   EpiCtx e{};
@@ -607,16 +625,26 @@ spc2_status run_steps(spc2_batch &b, int method, double dt, int64_t nsteps, cons
   e.aphi_slot_f = b.aphi_slot_f;
   e.n_slot = b.n_slot;
   e.n2_slot = b.n2_slot;
+  bool staged = false;
   for (int64_t k = 0; k < kmax; ++k) {
     e.sx = b.sx;
     e.sy = b.sy;
     e.sp = b.sp;
     e.sa.omega = b.omega;
+    // the next step's stage and bins come from this step's state-writing pass
+    // when every replica takes that step and no re-homing comes between
+    const bool fuse = k + 1 < kmin && b.since_rehome + 1 < kRehomeEvery && b.nap + b.nseg > 0;
     switch (b.prec) {
-      case P64: launch_step_kernels<P64>(b, e, (int)k, srk); break;
-      case P64F: launch_step_kernels<P64F>(b, e, (int)k, srk); break;
-      default: launch_step_kernels<P32K>(b, e, (int)k, srk);
+      case P64: launch_step_kernels<P64>(b, e, (int)k, srk, staged, fuse); break;
+      case P64F: launch_step_kernels<P64F>(b, e, (int)k, srk, staged, fuse); break;
+      default: launch_step_kernels<P32K>(b, e, (int)k, srk, staged, fuse);
     }
+    if (fuse) {
+      std::swap(b.gx, b.spare[0]);
+      std::swap(b.gy, b.spare[1]);
+      std::swap(b.gp, b.spare[2]);
+    }
+    staged = fuse;
     if (++b.since_rehome >= kRehomeEvery) {
       launch_rehome(b);
       b.since_rehome = 0;

@@ -1148,6 +1148,13 @@ struct EpiCtx {
   double *gpn;
   double kcdt_n;
   int next;
+  // Batch steps: the state-writing epilogue (SE, SRK) also stages and bins the
+  // new state for the next step (stage arrays bgx/bgy/bgp by particle, its
+  // cell, its rank among the cell's count); null bcell: a separate launch does
+  // that at the next step.  bcoff: the replica's first cell.
+  double *bgx, *bgy, *bgp;
+  int *bcell, *bcount, *blocal;
+  int bcoff;
 };
 
 // What a pass-1 epilogue computed for its particle: the velocity, the phi
@@ -1160,6 +1167,25 @@ __device__ __forceinline__ void wrap_state(const KParams &P, double &x, double &
   // the array form of the wrap, applied to every particle (PeriodicBoundaryConditions.cpp:133-141)
   x = x - floor(x * P.rLx) * P.Lx;
   y = y - floor(y * P.rLy) * P.Ly;
+}
+
+// The new state (x, y, p) of particle h staged for the next step as the
+// batch's stage-and-bin launch stages it: the conditional wrap, the cell of the
+// wrapped position (the replica's one cell on the all-pairs path) and a rank
+// among that cell's particles from a wave-aggregated atomic on its count.
+__device__ __forceinline__ void bin_next(const KParams &P, const EpiCtx &e, int h, double x, double y, double p) {
+  int lc = 0;
+  if (P.linked) {
+    const Box b{P.Lx, P.Ly, P.rLx, P.rLy};
+    cond_wrap(b, x, y);
+    lc = cell_coord(y, b.Ly, P.ny) * P.nx + cell_coord(x, b.Lx, P.nx);
+  }
+  const int c = e.bcoff + lc;
+  e.bcell[h] = c;
+  e.bgx[h] = x;
+  e.bgy[h] = y;
+  e.bgp[h] = p;
+  e.blocal[h] = wave_aggregated_rank(e.bcount, c, true);
 }
 
 // Final phi derivative given the in/out value (k_next or derivative entry
@@ -1257,6 +1283,7 @@ __device__ __forceinline__ EpiState epilogue1(const KParams &P, const EpiCtx &e,
     e.sx[h] = x;
     e.sy[h] = y;
     e.sp[h] = p;
+    if (e.bcell) bin_next(P, e, h, x, y, p);
     r.ap = ap;
     r.x = x;
     r.y = y;
@@ -1295,6 +1322,7 @@ __device__ __forceinline__ EpiState epilogue2(const KParams &P, const EpiCtx &e,
     e.sx[h] = x;
     e.sy[h] = y;
     e.sp[h] = p;
+    if (e.bcell) bin_next(P, e, h, x, y, p);
     r.x = x;
     r.y = y;
     r.p = p;
