@@ -219,18 +219,25 @@ __device__ __forceinline__ void cell_run(Acc &acc, const Src32 &src, int a, int 
         acc.pair2_x(src, t2, t2[0], t2[1], oxa, oxb, P);
       }
     } else if (t2 < last) {
-      float4 ta = t2[0], tb = t2[1];
-      for (int m = a;; m += 2) {
-        const float4 na = t2[2], nb = t2[3];
+      // the pair at t2 from the tiles ta, tb; whether another pair follows
+      auto step = [&](const float4 &ta, const float4 &tb, int m) {
         const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
         if constexpr (std::is_same<Acc, Chim2Acc<P32K>>::value)  // reads aj[m]: the index too
           acc.pair2_x(src, t2, m, ta, tb, oxa, oxb, P);
         else
           acc.pair2_x(src, t2, ta, tb, oxa, oxb, P);
         t2 += 2;
-        if (!(t2 < last)) break;
-        ta = na;
-        tb = nb;
+        return t2 < last;
+      };
+      // two sets of tile registers taken in turn: each pair's tiles are read
+      // into the set the previous pair has just used, with no copies
+      float4 a0 = t2[0], b0 = t2[1];
+      for (int m = a;; m += 4) {
+        const float4 a1 = t2[2], b1 = t2[3];
+        if (!step(a0, b0, m)) break;
+        a0 = t2[2];
+        b0 = t2[3];
+        if (!step(a1, b1, m + 2)) break;
       }
     }
     if ((b - a) & 1) acc.template pair_x<SUM_GROUP>(src, b - 1, SELF && b - 1 == self ? kSelfOff : ox, P);
