@@ -213,12 +213,7 @@ __device__ __forceinline__ void cell_run(Acc &acc, const Src32 &src, int a, int 
     // LDS allocation (at most two tiles past the staged halo, in the arrays
     // that follow it) and is not used.
     const float4 *t2 = src.f4 + a, *const last = src.f4 + (b - 1), *const own = src.f4 + self;
-    if constexpr (std::is_same<Acc, VortAcc<P32K>>::value) {  // its longer pair step hides the latency
-      for (; t2 < last; t2 += 2) {
-        const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
-        acc.pair2_x(src, t2, t2[0], t2[1], oxa, oxb, P);
-      }
-    } else if (t2 < last) {
+    if (t2 < last) {
       // the pair at t2 from the tiles ta, tb; whether another pair follows
       auto step = [&](const float4 &ta, const float4 &tb, int m) {
         const float oxa = SELF && t2 == own ? kSelfOff : ox, oxb = SELF && t2 + 1 == own ? kSelfOff : ox;
