@@ -193,6 +193,28 @@ def test_batch_matches_single_handles(gpu, prec):
             assert np.array_equal(frames[r], got.astype(np.float32))
 
 
+@pytest.mark.parametrize("method", ["se", "srk"])
+def test_batch_long_calls_equal_single_handles(gpu, method):
+    """Calls of 1, 17 and 22 steps: the steps whose state-writing pass also
+    stages and bins the next step's state, the steps around each re-homing
+    (every 16 steps) and each call's first step, which bin in a launch of their
+    own, must all give a single handle's trajectory bit for bit."""
+    reps = [replica(sy, n, "fp32k", seed, sg, al, D) for sy, n, seed, sg, al, D in MIX]
+    dt = 0.01
+    with spc2.Batch(reps) as b:
+        b.init_random_state(0.1)
+        for k in (1, 17, 22):
+            b.step(dt, k, method=method)
+        assert (b.steps_done() == 40).all()
+        got = [b.get_state(r) for r in range(len(reps))]
+    for r, p in enumerate(reps):
+        with spc2.Sim(p) as s:
+            s.init_random_state(0.1)
+            s.step(method, dt, 0, 40)
+            ref = s.get_state()
+        assert np.array_equal(got[r], ref), f"replica {r} (N={p.n}): {np.abs(got[r] - ref).max():.3e}"
+
+
 def test_batch_clustered_rows_equal_single_handles(gpu):
     """Replicas with an overfull and a sparse cell row (their row chunks'
     overflow workgroup and global-memory path): bit-identical to single
