@@ -691,15 +691,18 @@ __device__ __forceinline__ void walk32(Acc &acc, const Src32 &src, const int4 *l
 // FP32 list walk over an LDS list: groups of eight 16-bit entries, taken as two
 // halves of four whose staged tiles (and pass-2 derivatives) are requested
 // before the half is summed; the lane's next group of entries is requested
-// before the current group is summed.  Sentinel padding is summed like any entry (it adds
-// zero), and each group's partials are flushed once.  RHO_BITS (chimera pass 1):
-// the group's eight within-rho decisions are stored as one byte, bit k for
-// entry k, at mb (the list's interleaved layout, a byte per lane and group).
-// COH_QUEUE (vortex pass 1): entries within xi_r's band, a few per particle, are
-// queued in the lane's LDS column cq (kCohQueue slots, stride kBlock) and their
-// cohesion terms summed after the walk, in list order (a full queue is emptied
-// first), so that the divergent cohesion branch runs once per wavefront rather
-// than at every entry where one lane needs it.
+// before the current group is summed (walk_lds_sets, for the walks without a
+// cohesion queue: in two sets of word registers taken in turn).  Sentinel
+// padding is summed like any entry (it adds zero), and each group's partials
+// are flushed once.  RHO_BITS (chimera pass 1): the group's eight within-rho
+// decisions are stored as one byte, bit k for entry k, at mb (the list's
+// interleaved layout, a byte per lane and group).  COH_QUEUE (vortex pass 1):
+// entries within xi_r's band, a few per particle, are queued in the lane's LDS
+// column cq (kCohQueue slots, stride kBlock) and their cohesion terms summed
+// after the walk, in list order (a full queue is emptied first), so that the
+// divergent cohesion branch runs once per wavefront rather than at every entry
+// where one lane needs it.  The vortex walk keeps one set of word registers: a
+// second costs it more in registers than it saves in copies.
 
 template <class Acc>
 __device__ __forceinline__ void coh_drain(Acc &acc, const Src32 &src, const uint32_t *cq, int qn,
@@ -714,30 +717,92 @@ __device__ __forceinline__ void coh_drain(Acc &acc, const Src32 &src, const uint
   }
 }
 
+// One group of walk_lds's list without a cohesion queue (chimera pass 1 with or
+// without rho bits, pass 2): the eight entries as two halves of four, each
+// half's staged tiles (and pass-2 derivatives) requested before it is summed.
+template <int PASS, bool RHO_BITS, class Acc>
+__device__ __forceinline__ void walk_group(Acc &acc, const Src32 &src, const uint32_t (&w)[4], const KParams &P,
+                                           uint8_t *&mb) {
+  using F = EntLds;
+  const char *ajb = (const char *)src.aj;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t e[4] = {w[2 * h] & 0xffffu, w[2 * h] >> 16, w[2 * h + 1] & 0xffffu, w[2 * h + 1] >> 16};
+    float4 v[4];
+    float a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t off = F::off(e[u]);
+      v[u] = F::tile(off);
+      a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if constexpr (RHO_BITS) {
+        const bool in =
+            h == 0 && u == 0
+                ? acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P)
+                : acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+        bits |= (uint32_t)in << (4 * h + u);
+      } else if (h == 0 && u == 0) {
+        acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+      } else {
+        acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
+      }
+    }
+  }
+  acc.flush();
+  if constexpr (RHO_BITS) {
+    *mb = (uint8_t)bits;
+    mb += 64;
+  }
+}
+
+// walk_lds without a cohesion queue: the groups two at a time in two sets of
+// word registers taken in turn, so that the lane's next group is requested
+// before the current one is summed with no copies between the sets.  The
+// terms, groups and rho bits are those of the one-set walk, in the same order.
+template <int PASS, bool RHO_BITS, class Acc>
+__device__ __forceinline__ void walk_lds_sets(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P,
+                                              uint8_t *mb) {
+  if (cnt <= 0) return;
+  int4 g0 = *l, g1;
+  for (int q = 0;; q += 16) {
+    l += 64;
+    if (q + 8 < cnt) g1 = *l;  // the lane's next group, if it has one
+    const uint32_t w0[4] = {(uint32_t)g0.x, (uint32_t)g0.y, (uint32_t)g0.z, (uint32_t)g0.w};
+    walk_group<PASS, RHO_BITS>(acc, src, w0, P, mb);
+    if (q + 8 >= cnt) break;
+    l += 64;
+    if (q + 16 < cnt) g0 = *l;
+    const uint32_t w1[4] = {(uint32_t)g1.x, (uint32_t)g1.y, (uint32_t)g1.z, (uint32_t)g1.w};
+    walk_group<PASS, RHO_BITS>(acc, src, w1, P, mb);
+    if (q + 16 >= cnt) break;
+  }
+}
+
 template <int PASS, bool RHO_BITS = false, bool COH_QUEUE = false, class Acc>
 __device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 *l, int cnt, const KParams &P,
                                          uint8_t *mb = nullptr, uint32_t *cq = nullptr) {
   using F = EntLds;
+  if constexpr (!COH_QUEUE) {
+    walk_lds_sets<PASS, RHO_BITS>(acc, src, l, cnt, P, mb);
+    return;
+  }
   if (cnt <= 0) return;
-  const char *ajb = (const char *)src.aj;
   int qn = 0;
   int4 g = *l;
   for (int q = 0; q < cnt; q += 8) {
     const uint32_t w[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
     l += 64;
     if (q + 8 < cnt) g = *l;  // the lane's next group, if it has one
-    uint32_t bits = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t e[4] = {w[2 * h] & 0xffffu, w[2 * h] >> 16, w[2 * h + 1] & 0xffffu, w[2 * h + 1] >> 16};
       float4 v[4];
-      float a[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t off = F::off(e[u]);
-        v[u] = F::tile(off);
-        a[u] = PASS == 2 ? *(const float *)(ajb + (off >> 2)) : 0.0f;
-      }
+      for (int u = 0; u < 4; ++u) v[u] = F::tile(F::off(e[u]));
       if constexpr (COH_QUEUE) {
         // two entries at a time: one branch for both rho band re-checks and one
         // for both queue tests
@@ -781,26 +846,8 @@ __device__ __forceinline__ void walk_lds(Acc &acc, const Src32 &src, const int4 
         }
         continue;
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if constexpr (RHO_BITS) {
-          const bool in =
-              h == 0 && u == 0
-                  ? acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P)
-                  : acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
-          bits |= (uint32_t)in << (4 * h + u);
-        } else if (h == 0 && u == 0) {
-          acc.template pair_c<SUM_GROUP_FIRST>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
-        } else {
-          acc.template pair_c<SUM_GROUP>(src, F::index(e[u]), F::fx(e[u]), F::fy(e[u]), v[u], a[u], P);
-        }
-      }
     }
     acc.flush();
-    if constexpr (RHO_BITS) {
-      *mb = (uint8_t)bits;
-      mb += 64;
-    }
   }
   if constexpr (COH_QUEUE) coh_drain(acc, src, cq, qn, P);
 }
